@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""SAV steps/s of the seven-step pseudo-spectral timestep on one MI355X.
+
+Default workload: two seeded vesicles on 1024^2 (eps = 0.015, dt = 2e-5, BiCGSTAB
+rtol 1e-10, itmax 20), FP32 fields with FP64 reductions.  The same line reports
+the FP64 form at 1024^2, both forms at 128^3 (two ellipsoids), the psi-solve
+iteration counts, the FP32-vs-FP64 field error, the roofline of the dominant
+kernel and the CPU oracle timed on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _setup(jv, wl, precision):
+    conf = wl.conf
+    phi, psi = wl.initial_fields()
+    ops = jv.Operators(conf, precision=precision)
+    st = jv.FieldState.zeros(conf)
+    st.phi[...] = phi.T
+    st.psi[...] = psi.T
+    st.phi_hat[...] = ops.rfft(st.phi)
+    st.psi_hat[...] = ops.rfft(st.psi)
+    st.Q = st.R1 = st.R2 = st.R3 = 1.0
+    ops.set_state(jv.SAV_PRESENT, st)
+    ops.init_sav()
+    return ops
+
+
+def run_case(jv, wl, precision, steps, warmup, timed_kernel=0):
+    """Warmup steps untimed, then `steps` timed steps (inputs resident in HBM)."""
+    conf = wl.conf
+    ops = _setup(jv, wl, precision)
+    t = np.arange(warmup + steps + 1) * conf.dt
+    if warmup:
+        ops.run(t[: warmup + 1], 1)
+    if timed_kernel:
+        ops.kernel_timing(timed_kernel)
+    _sync()
+    t0 = time.perf_counter()
+    rc, E, A, kit = ops.run(t[warmup:], warmup + 1)
+    _sync()
+    t1 = time.perf_counter()
+    res = dict(rc=rc, seconds=t1 - t0, steps_per_s=steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps,
+               krylov_iters=kit, energy_first=float(E[0]), energy_last=float(E[-1]),
+               energy_monotone=bool(np.all(np.diff(E) <= 0)))
+    if timed_kernel:
+        n, ms, b = ops.kernel_timing_get()
+        res["kernel"] = dict(name=ops.KERNELS[timed_kernel], launches=n, avg_ms=ms / max(n, 1), bytes=b)
+    st = ops.get_state(jv.SAV_PRESENT)
+    ops.close()
+    return res, st
+
+
+def cpu_baseline(jv, wl):
+    """The FP64 CPU oracle on the host cores: one BDF1 step of the same workload
+    after the SAV initialisation (a bounded sample of the timed work)."""
+    from oracle import sav_oracle as so
+    workers = min(os.cpu_count() or 1, 16)
+    conf = wl.conf
+    phi, psi = wl.initial_fields()
+    orc = so.Oracle(conf, workers=workers)
+    pres, old = orc.init_sav(so.state_from_physical(phi, psi, orc.ops, conf.ndim))
+    t0 = time.perf_counter()
+    orc.step(pres, old, conf.dt, 1)
+    dt = time.perf_counter() - t0
+    return dict(value=1.0 / dt, unit="SAV steps/s", cores=workers, kind="port",
+                sample=f"1 BDF1 step of {wl.name} ({conf.Nx}^{conf.ndim}, N={conf.N}) with the monitor, "
+                       f"numpy/scipy FP64 oracle, scipy.fft workers={workers}, pointwise single-threaded; "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--only", choices=["all", "2d", "3d"], default="all")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    import __graft_entry__ as ge
+    jv = ge.load_package()
+    import torch  # noqa: F401  (device-wide synchronisation around the timed region)
+
+    out = {}
+    wl3 = jv.workload_c3(n=1024, steps=a.steps)
+    if a.only in ("all", "2d"):
+        r32, s32 = run_case(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup, timed_kernel=2)
+        r64, s64 = run_case(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup)
+        err = {k: float(np.max(np.abs(getattr(s32, k) - getattr(s64, k))) / np.max(np.abs(getattr(s64, k))))
+               for k in ("phi", "psi", "mu", "nu")}
+        err["u"] = float(np.max(np.abs(s32.u - s64.u)) / max(np.max(np.abs(s64.u)), 1e-300))
+        out["2d"] = dict(fp32=r32, fp64=r64, fp32_vs_fp64_max_rel_err=err)
+    if a.only in ("all", "3d"):
+        wl4 = jv.workload_c4(n=128, steps=a.steps)
+        q32, t32 = run_case(jv, wl4, jv.SAV_PREC_FP32, a.steps, a.warmup)
+        q64, t64 = run_case(jv, wl4, jv.SAV_PREC_FP64, a.steps, a.warmup)
+        err3 = {k: float(np.max(np.abs(getattr(t32, k) - getattr(t64, k))) / np.max(np.abs(getattr(t64, k))))
+                for k in ("phi", "psi")}
+        out["3d"] = dict(fp32=q32, fp64=q64, fp32_vs_fp64_max_rel_err=err3)
+
+    head = out["2d"]["fp32"] if "2d" in out else out["3d"]["fp32"]
+    line = {
+        "metric": "SAV steps/s at 1024^2 (two vesicles, FP32 fields + FP64 reductions, diagonal psi preconditioner)",
+        "value": round(head["steps_per_s"], 3),
+        "unit": "SAV steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64-reductions",
+        "data": "synthetic (seeded initial vesicle fields)",
+        "config": {"workload": wl3.name if "2d" in out else "two_ellipsoids_128cubed",
+                   "grid": "1024x1024" if "2d" in out else "128^3", "N": 2,
+                   "epsilon": 0.015, "dt": 2e-5, "krylov_rtol": 1e-10, "itmax": 20,
+                   "precond": "diag", "parallelism": "single GPU"},
+    }
+    if "2d" in out:
+        k = out["2d"]["fp32"]["kernel"]
+        achieved = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9
+        line["roofline"] = {"bound": "hbm", "kernel": k["name"], "achieved": round(achieved, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": None, "avg_launch_ms": round(k["avg_ms"], 5), "launches": k["launches"],
+                            "bytes_per_launch": k["bytes"]}
+        line["psi_iters_per_step"] = {"fp32": out["2d"]["fp32"]["krylov_iters"] / a.steps,
+                                      "fp64": out["2d"]["fp64"]["krylov_iters"] / a.steps}
+        line["fp64_steps_per_s_1024"] = round(out["2d"]["fp64"]["steps_per_s"], 3)
+        line["fp32_vs_fp64_max_rel_err_1024"] = out["2d"]["fp32_vs_fp64_max_rel_err"]
+        line["energy_monotone_1024"] = out["2d"]["fp64"]["energy_monotone"]
+    if "3d" in out:
+        line["fp32_steps_per_s_128cubed"] = round(out["3d"]["fp32"]["steps_per_s"], 3)
+        line["fp64_steps_per_s_128cubed"] = round(out["3d"]["fp64"]["steps_per_s"], 3)
+        line["fp32_vs_fp64_max_rel_err_128cubed"] = out["3d"]["fp32_vs_fp64_max_rel_err"]
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(jv, wl3 if "2d" in out else jv.workload_c4(n=128, steps=1))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

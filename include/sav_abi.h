@@ -1,0 +1,163 @@
+/*
+ * sav_abi.h -- C ABI of libsav_hip.so, an MI355X (gfx950) implementation of the
+ * seven-step pseudo-spectral SAV timestep of the Julia code
+ * Forstylime/Julia-Vesicles-growth-and-shrinkage.
+ *
+ * Conventions
+ *   - Plain C types only; every array argument is a caller-owned HOST pointer.
+ *   - Arrays are column-major with x fastest, exactly the Julia layout:
+ *       phi  : (Nx, Ny, N)        3D: (Nx, Ny, Nz, N)     double
+ *       phi_hat: (Nx/2+1, Ny, N)  3D: (Nx/2+1, Ny, Nz, N) complex (interleaved re, im)
+ *       u    : (Nx, Ny, 2)        3D: (Nx, Ny, Nz, 3)
+ *       p    : (Nx, Ny)           3D: (Nx, Ny, Nz)
+ *     Complex values are interleaved double[2] -- layout-compatible with ComplexF64.
+ *   - The library copies during the call and never retains host pointers.
+ *   - Every entry point returns an int status: 0 = OK, >0 = warning, <0 = error.
+ *   - A handle is single-threaded; it owns one HIP stream and all device state.
+ */
+#ifndef SAV_ABI_H
+#define SAV_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAV_ABI_VERSION 1
+
+/* status codes */
+#define SAV_OK 0
+#define SAV_W_KRYLOV_NOT_CONVERGED 1 /* @warn "BiCGSTAB 未收敛" (src/Solvers.jl:76); step committed */
+#define SAV_W_THETA2_NONNEG 2        /* @warn when theta_2 >= 0 (src/Solvers_3d.jl:388); step committed */
+#define SAV_E_ARG (-1)
+#define SAV_E_HIP (-2)
+#define SAV_E_WC_NEGATIVE (-3) /* error("FATAL: W + C < 0") (src/Utils.jl:162,180,188); step not committed */
+#define SAV_E_NONFINITE (-4)   /* new: NaN/Inf in the committed scalars */
+#define SAV_E_UNSUPPORTED (-5)
+
+#define SAV_PREC_FP64 0
+#define SAV_PREC_FP32 1 /* FP32 fields/FFTs, FP64 reductions, 3x3 solve and Krylov scalars */
+
+#define SAV_PRECOND_DIAG 0
+#define SAV_PRECOND_LEARNED 1
+
+#define SAV_PRESENT 0
+#define SAV_OLD 1
+
+typedef struct sav_handle sav_handle_t;
+
+/*
+ * Mirrors `Config` field for field (src/Types.jl:27-74, kw-ctor :101-134;
+ * src/Types_3d.jl adds Nz, Lz).  The Int fields of Config are int64 here.
+ * psi_in_v / psi_out_v point to N doubles each (copied at sav_create).
+ */
+typedef struct sav_config {
+  int32_t ndim; /* 2 or 3 */
+  int32_t N;
+  int32_t Nx, Ny, Nz; /* Nz ignored in 2D */
+  double Lx, Ly, Lz;
+  double epsilon, M_phi, M0_psi, eta, gamma_surf;
+  int64_t gamma_area;
+  double gamma_bend;
+  int64_t gamma_in, beta_in, gamma_out, beta_out, lamda;
+  const double* psi_in_v;
+  const double* psi_out_v;
+  double S1, S2, S3, S4, C1, C2, C3;
+  double dt, T, tol;
+  /* additions of this library */
+  int32_t precision; /* SAV_PREC_* */
+  int32_t precond;   /* SAV_PRECOND_* */
+  int32_t itmax;     /* BiCGSTAB iteration cap; 20 in src/Solvers.jl:75 */
+  double atol;       /* BiCGSTAB absolute tolerance; 1e-14 in src/Solvers.jl:74 */
+  int32_t device;
+} sav_config_t;
+
+typedef struct sav_step_stats {
+  int32_t krylov_iters[3]; /* psi_11, psi_12, psi_21 solves (0 in 3D) */
+  int32_t krylov_solved[3];
+  double krylov_rnorm[3];  /* final preconditioned residual norm */
+  double energy;           /* compute_modified_energy after the step (src/Utils.jl:96-141) */
+  double area_ratio;       /* sum|A - A0| / sum A0 (src/simulation.jl:147) */
+  double Q, R1, R2, R3;
+  int32_t status;
+} sav_step_stats_t;
+
+/* ABI version (SAV_ABI_VERSION). */
+int sav_abi_version(void);
+
+/* Builds the handle: the `Operators` bundle of build_operators (src/SpectralUtils.jl:36-118),
+ * device state, twiddles and work buffers.  Replaces Config(...)+build_operators(conf). */
+int sav_create(const sav_config_t* cfg, sav_handle_t** out);
+
+/* Frees the handle and its device memory. */
+int sav_destroy(sav_handle_t* h);
+
+/* Last error message of the handle (or the last failed sav_create when h == NULL). */
+const char* sav_last_error(const sav_handle_t* h);
+
+/*
+ * Copies a FieldState in or out (src/Types.jl:167-201).  which = SAV_PRESENT or SAV_OLD.
+ * scalars = {Q, R1, R2, R3}; A0 has N entries (present only; ignored for OLD); dt is
+ * FieldState.dt.  Any array pointer may be NULL on get (skipped); on set all are required.
+ */
+int sav_set_state(sav_handle_t* h, int which, const double* phi, const double* phi_hat, const double* psi,
+                  const double* psi_hat, const double* u, const double* u_hat, const double* p,
+                  const double* p_hat, const double* mu, const double* mu_hat, const double* nu,
+                  const double* nu_hat, const double* scalars, const double* A0, double dt);
+int sav_get_state(sav_handle_t* h, int which, double* phi, double* phi_hat, double* psi, double* psi_hat,
+                  double* u, double* u_hat, double* p, double* p_hat, double* mu, double* mu_hat, double* nu,
+                  double* nu_hat, double* scalars, double* A0, double* dt);
+
+/* SAV initialisation of src/simulation.jl:24-53 on the PRESENT state (phi, psi and their
+ * spectra must be set): A0, R1..R3, Q = 1, mu_hat, nu_hat, mu, nu; then old = present. */
+int sav_init_sav(sav_handle_t* h);
+
+/* One time-loop body (src/simulation.jl:73-110, monitor :146-148): solve_step1..7 of
+ * src/Solvers.jl (src/Solvers_3d.jl), state rotation, back-transforms and the monitor.
+ * n is the 1-based loop index (BDF1 for n == 1, bdf_coeff, src/Types.jl:426). */
+int sav_step(sav_handle_t* h, double dt, int n, sav_step_stats_t* stats);
+
+/* Runs n_nodes-1 steps over the time nodes t_nodes (dt = t[k+1]-t[k]; loop index
+ * starts at n0), writing energy/area histories (length n_nodes-1, may be NULL).
+ * The run_simulation loop of src/simulation.jl:71-152 without plots or .mat I/O. */
+int sav_run(sav_handle_t* h, const double* t_nodes, int n_nodes, int n0, double* energy_hist,
+            double* area_hist, int32_t* krylov_iters_total);
+
+/* Transforms on host buffers (test hooks for to_spectral!/to_physical!,
+ * src/SpectralUtils.jl:132-148, and the full-complex plans :69-71).
+ * count = number of scalar fields (e.g. N, or 2 for a 2D velocity). */
+int sav_rfft(sav_handle_t* h, const double* in, double* out_hat, int count);
+int sav_irfft(sav_handle_t* h, const double* in_hat, double* out, int count);
+int sav_fft_full(sav_handle_t* h, const double* in, double* out, int count, int inverse);
+
+/* Functionals at (phi, psi): H1, H2, H3, G (each N fields), W[3], area[N]
+ * (get_H1/get_H2/get_H3/get_MG, get_W1..3, calculate_area; src/Utils.jl:66-249).
+ * A0 is taken from the present state.  Returns SAV_E_WC_NEGATIVE like the reference. */
+int sav_eval_functionals(sav_handle_t* h, const double* phi, const double* psi, double* H1, double* H2, double* H3,
+                         double* G, double* W, double* area);
+
+/* 2D psi solve of src/Solvers.jl:39-81 with M_psi built from phi_star and the
+ * BDF coefficient a and dt: out_hat = solve_psi!(rhs_hat).  iters/solved/rnorm may be NULL. */
+int sav_psi_solve(sav_handle_t* h, const double* phi_star, double a, double dt, const double* rhs_hat,
+                  double* out_hat, int32_t* iters, int32_t* solved, double* rnorm);
+
+/* compute_modified_energy(present, old) (src/Utils.jl:96-141) and the area ratio. */
+int sav_modified_energy(sav_handle_t* h, double* energy, double* area_ratio);
+
+/* Milliseconds of device time of the last sav_run / sav_step (HIP events on the handle's stream). */
+int sav_last_elapsed_ms(sav_handle_t* h, double* ms);
+
+/* Measurement hook (no reference counterpart): brackets every launch of one 2D
+ * psi-matvec kernel with HIP events on the handle's stream during the following
+ * steps.  kernel_id: 1 column inverse FFTs, 2 row FFT*M*FFT, 3 column forward FFTs
+ * + preconditioner, 0 off.  Launches queued past convergence are not counted.
+ * _get returns the launch count, their summed duration and the algorithmic HBM
+ * bytes per launch (reads + writes of whole vectors). */
+int sav_kernel_timing(sav_handle_t* h, int kernel_id);
+int sav_kernel_timing_get(sav_handle_t* h, int64_t* launches, double* total_ms, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,197 @@
+// C ABI of libsav_hip.so (declared in include/sav_abi.h).
+#include <cstring>
+#include <string>
+
+#include "solver.h"
+
+struct sav_handle {
+  std::unique_ptr<sav::ISolver> s;
+  std::string err;
+  int ndim = 2, N = 1;
+};
+
+static thread_local std::string g_create_error;
+
+#define SAV_GUARD(h, body)                                  \
+  try {                                                     \
+    body;                                                   \
+  } catch (const sav::HipError& e) {                        \
+    (h)->err = e.what();                                    \
+    return e.code == SAV_E_UNSUPPORTED ? SAV_E_UNSUPPORTED : SAV_E_HIP; \
+  } catch (const std::exception& e) {                       \
+    (h)->err = e.what();                                    \
+    return SAV_E_ARG;                                       \
+  }
+
+extern "C" {
+
+int sav_abi_version(void) { return SAV_ABI_VERSION; }
+
+static bool size_ok(int n) { return sav::fft_size_supported(n) && n % 2 == 0 && sav::fft_size_supported(n / 2); }
+
+int sav_create(const sav_config_t* cfg, sav_handle_t** out) {
+  if (!cfg || !out) return SAV_E_ARG;
+  *out = nullptr;
+  try {
+    if (cfg->ndim != 2 && cfg->ndim != 3) throw std::invalid_argument("ndim must be 2 or 3");
+    if (cfg->N < 1 || cfg->N > sav::MAXN) throw std::invalid_argument("N out of range");
+    if (!size_ok(cfg->Nx) || !sav::fft_size_supported(cfg->Ny) || (cfg->ndim == 3 && !sav::fft_size_supported(cfg->Nz)))
+      throw std::invalid_argument("grid sizes must be 2^a 3^b, Nx even, Nx/2, Ny, Nz <= 1024");
+    if (cfg->ndim == 2 && cfg->Ny < 4) throw std::invalid_argument("Ny must be >= 4");
+    if (!cfg->psi_in_v || !cfg->psi_out_v) throw std::invalid_argument("psi_in_v/psi_out_v required");
+    if (cfg->precision != SAV_PREC_FP64 && cfg->precision != SAV_PREC_FP32)
+      throw std::invalid_argument("unknown precision");
+    if (cfg->precond != SAV_PRECOND_DIAG) throw std::invalid_argument("only the diagonal preconditioner is built");
+    sav::HostCfg hc;
+    hc.c = *cfg;
+    hc.psi_in.assign(cfg->psi_in_v, cfg->psi_in_v + cfg->N);
+    hc.psi_out.assign(cfg->psi_out_v, cfg->psi_out_v + cfg->N);
+    if (hc.c.itmax <= 0) hc.c.itmax = 20;
+    auto h = new sav_handle();
+    try {
+      h->s = sav::make_solver(hc);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    h->ndim = cfg->ndim;
+    h->N = cfg->N;
+    *out = h;
+    return SAV_OK;
+  } catch (const sav::HipError& e) {
+    g_create_error = e.what();
+    return SAV_E_HIP;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    return SAV_E_ARG;
+  }
+}
+
+int sav_destroy(sav_handle_t* h) {
+  delete h;
+  return SAV_OK;
+}
+
+const char* sav_last_error(const sav_handle_t* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int sav_set_state(sav_handle_t* h, int which, const double* phi, const double* phi_hat, const double* psi,
+                  const double* psi_hat, const double* u, const double* u_hat, const double* p, const double* p_hat,
+                  const double* mu, const double* mu_hat, const double* nu, const double* nu_hat, const double* scalars,
+                  const double* A0, double dt) {
+  if (!h) return SAV_E_ARG;
+  const double* a[12] = {phi, phi_hat, psi, psi_hat, u, u_hat, p, p_hat, mu, mu_hat, nu, nu_hat};
+  for (auto x : a)
+    if (!x) {
+      h->err = "sav_set_state: all arrays are required";
+      return SAV_E_ARG;
+    }
+  if (!scalars || (which != SAV_PRESENT && which != SAV_OLD)) return SAV_E_ARG;
+  SAV_GUARD(h, h->s->set_state(which, a, scalars, A0, dt));
+  return SAV_OK;
+}
+
+int sav_get_state(sav_handle_t* h, int which, double* phi, double* phi_hat, double* psi, double* psi_hat, double* u,
+                  double* u_hat, double* p, double* p_hat, double* mu, double* mu_hat, double* nu, double* nu_hat,
+                  double* scalars, double* A0, double* dt) {
+  if (!h || (which != SAV_PRESENT && which != SAV_OLD)) return SAV_E_ARG;
+  double* a[12] = {phi, phi_hat, psi, psi_hat, u, u_hat, p, p_hat, mu, mu_hat, nu, nu_hat};
+  SAV_GUARD(h, h->s->get_state(which, a, scalars, A0, dt));
+  return SAV_OK;
+}
+
+int sav_init_sav(sav_handle_t* h) {
+  if (!h) return SAV_E_ARG;
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->init_sav());
+  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0 at initialisation";
+  return rc;
+}
+
+int sav_step(sav_handle_t* h, double dt, int n, sav_step_stats_t* stats) {
+  if (!h || n < 1 || !(dt > 0)) return SAV_E_ARG;
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->step(dt, n, stats));
+  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0; step not committed";
+  if (rc == SAV_E_NONFINITE) h->err = "non-finite SAV scalars; step not committed";
+  return rc;
+}
+
+int sav_run(sav_handle_t* h, const double* t_nodes, int n_nodes, int n0, double* energy_hist, double* area_hist,
+            int32_t* krylov_iters_total) {
+  if (!h || !t_nodes || n_nodes < 1 || n0 < 1) return SAV_E_ARG;
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->run(t_nodes, n_nodes, n0, energy_hist, area_hist, krylov_iters_total));
+  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0; step not committed";
+  return rc;
+}
+
+int sav_rfft(sav_handle_t* h, const double* in, double* out_hat, int count) {
+  if (!h || !in || !out_hat || count < 1) return SAV_E_ARG;
+  SAV_GUARD(h, h->s->rfft_host(in, out_hat, count));
+  return SAV_OK;
+}
+
+int sav_irfft(sav_handle_t* h, const double* in_hat, double* out, int count) {
+  if (!h || !in_hat || !out || count < 1) return SAV_E_ARG;
+  SAV_GUARD(h, h->s->irfft_host(in_hat, out, count));
+  return SAV_OK;
+}
+
+int sav_fft_full(sav_handle_t* h, const double* in, double* out, int count, int inverse) {
+  if (!h || !in || !out || count < 1) return SAV_E_ARG;
+  if (h->ndim != 2) {
+    h->err = "sav_fft_full: 2D only";
+    return SAV_E_UNSUPPORTED;
+  }
+  SAV_GUARD(h, h->s->fft_full_host(in, out, count, inverse));
+  return SAV_OK;
+}
+
+int sav_eval_functionals(sav_handle_t* h, const double* phi, const double* psi, double* H1, double* H2, double* H3,
+                         double* G, double* W, double* area) {
+  if (!h || !phi || !psi) return SAV_E_ARG;
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->eval_functionals(phi, psi, H1, H2, H3, G, W, area));
+  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0";
+  return rc;
+}
+
+int sav_psi_solve(sav_handle_t* h, const double* phi_star, double a, double dt, const double* rhs_hat,
+                  double* out_hat, int32_t* iters, int32_t* solved, double* rnorm) {
+  if (!h || !phi_star || !rhs_hat || !out_hat) return SAV_E_ARG;
+  if (h->ndim != 2) {
+    h->err = "sav_psi_solve: the Krylov psi solve exists in 2D only";
+    return SAV_E_UNSUPPORTED;
+  }
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->psi_solve_host(phi_star, a, dt, rhs_hat, out_hat, iters, solved, rnorm));
+  return rc;
+}
+
+int sav_modified_energy(sav_handle_t* h, double* energy, double* area_ratio) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, h->s->modified_energy(energy, area_ratio));
+  return SAV_OK;
+}
+
+int sav_kernel_timing(sav_handle_t* h, int kernel_id) {
+  if (!h || kernel_id < 0 || kernel_id > 3) return SAV_E_ARG;
+  SAV_GUARD(h, h->s->kernel_timing(kernel_id));
+  return SAV_OK;
+}
+
+int sav_kernel_timing_get(sav_handle_t* h, int64_t* launches, double* total_ms, double* bytes_per_launch) {
+  if (!h) return SAV_E_ARG;
+  long long n = 0;
+  SAV_GUARD(h, h->s->kernel_timing_get(&n, total_ms, bytes_per_launch));
+  if (launches) *launches = n;
+  return SAV_OK;
+}
+
+int sav_last_elapsed_ms(sav_handle_t* h, double* ms) {
+  if (!h || !ms) return SAV_E_ARG;
+  *ms = h->s->last_elapsed_ms();
+  return SAV_OK;
+}
+
+}  // extern "C"

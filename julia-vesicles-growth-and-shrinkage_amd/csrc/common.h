@@ -1,0 +1,102 @@
+// Common device/host helpers for libsav_hip (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace sav {
+
+// ---------------------------------------------------------------------------
+// Complex numbers in interleaved (re, im) layout -- the memory layout of Julia's
+// ComplexF64 / ComplexF32 and of FFTW's fftw_complex.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct alignas(2 * sizeof(R)) cplx {
+  R x, y;
+};
+
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mk(R a, R b) { return cplx<R>{a, b}; }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R> __host__ __device__ __forceinline__ cplx<R> operator*(R s, cplx<R> a) { return {s * a.x, s * a.y}; }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> conj(cplx<R> a) { return {a.x, -a.y}; }
+// multiply by +i or -i
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_i(cplx<R> a) { return {-a.y, a.x}; }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_mi(cplx<R> a) { return {a.y, -a.x}; }
+
+// complex scalar held in double precision (Krylov scalars)
+struct zd {
+  double x, y;
+};
+__host__ __device__ __forceinline__ zd zmul(zd a, zd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
+  // Smith's algorithm
+  if (fabs(b.x) >= fabs(b.y)) {
+    double r = b.y / b.x, d = b.x + b.y * r;
+    return {(a.x + a.y * r) / d, (a.y - a.x * r) / d};
+  } else {
+    double r = b.x / b.y, d = b.x * r + b.y;
+    return {(a.x * r + a.y) / d, (a.y * r - a.x) / d};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+struct HipError : std::runtime_error {
+  int code;
+  HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+#define SAV_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::sav::HipError(std::string(#call) + ": " + hipGetErrorString(e_), (int)e_); \
+  } while (0)
+
+#define SAV_LAUNCH_CHECK() SAV_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Wave / block reductions in double precision (wave64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// Sum K values across the block; result valid in thread 0.  `scratch` must hold
+// K * (blockDim.x / 64) doubles.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&acc)[K], double* scratch) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) scratch[k * nw + wid] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += scratch[k * nw + w];
+      acc[k] = s;
+    }
+  }
+}
+
+template <typename R> __device__ __forceinline__ double to_d(R v) { return (double)v; }
+
+}  // namespace sav

@@ -1,0 +1,275 @@
+// Batched mixed-radix (2/3/4/8) Stockham FFTs for gfx950.
+//
+// One transform of length L is carried by a "team" of TT = L/E threads, each
+// holding E elements in registers.  A radix-R pass makes every thread do E/R
+// butterflies in registers; passes exchange data through LDS.  The first pass
+// reads its inputs straight from registers that the caller loaded from HBM in
+// the order  pos = t + q*TT + k*(L/R0)  (consecutive threads -> consecutive
+// addresses), and the last pass leaves its outputs in registers in the order
+// pos = t + q*TT + m*(L/RL), again coalesced for the caller's store.
+//
+// Stockham autosort pass with radix R after passes of total length Ls:
+//   butterfly j in [0, L/R): inputs x[j + k*L/R], twiddles W_{Ls*R}^{(j mod Ls)*k},
+//   outputs at (j / Ls)*Ls*R + (j mod Ls) + m*Ls.
+#pragma once
+#include "common.h"
+
+namespace sav {
+
+struct FftPlanData {
+  int np;
+  int rad[12];
+  int E;
+};
+
+constexpr FftPlanData make_fft_plan(int L) {
+  FftPlanData p{};
+  int n = L, np = 0, has3 = 0;
+  while (n % 3 == 0) { p.rad[np++] = 3; n /= 3; has3 = 1; }
+  int a = 0;
+  while (n % 2 == 0) { n /= 2; ++a; }
+  int maxr = 1;
+  int n8 = a / 3, rem = a % 3;
+  for (int i = 0; i < n8; ++i) { p.rad[np++] = 8; maxr = 8; }
+  if (rem == 1) { p.rad[np++] = 2; if (maxr < 2) maxr = 2; }
+  if (rem == 2) { p.rad[np++] = 4; if (maxr < 4) maxr = 4; }
+  p.np = np;
+  int lcm = has3 ? 3 * maxr : maxr;
+  if (lcm < 1) lcm = 1;
+  int E = lcm;
+  while (L / E > 64 && (L % (2 * E)) == 0 && 2 * E <= 32) E *= 2;
+  p.E = E;
+  return p;
+}
+
+constexpr bool fft_len_supported(int L) {
+  int n = L;
+  while (n % 3 == 0) n /= 3;
+  while (n % 2 == 0) n /= 2;
+  return n == 1 && L >= 2 && L <= 1024;
+}
+
+// ---------------------------------------------------------------------------
+// Butterflies.  INV selects e^{+i} kernels.
+// ---------------------------------------------------------------------------
+template <typename R, bool INV>
+__device__ __forceinline__ void bfly2(cplx<R>* a) {
+  cplx<R> t = a[0];
+  a[0] = t + a[1];
+  a[1] = t - a[1];
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void bfly3(cplx<R>* a) {
+  const R s = R(0.866025403784438646763723170752936183);
+  cplx<R> b = a[1] + a[2];
+  cplx<R> d = s * (a[1] - a[2]);
+  cplx<R> t1 = a[0] - R(0.5) * b;
+  a[0] = a[0] + b;
+  cplx<R> t2 = INV ? mul_i(d) : mul_mi(d);
+  a[1] = t1 + t2;
+  a[2] = t1 - t2;
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void bfly4(cplx<R>& a0, cplx<R>& a1, cplx<R>& a2, cplx<R>& a3) {
+  cplx<R> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3;
+  cplx<R> t3 = INV ? mul_i(a1 - a3) : mul_mi(a1 - a3);
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+
+template <typename R, bool INV>
+__device__ __forceinline__ void bfly8(cplx<R>* a) {
+  const R h = R(0.707106781186547524400844362104849039);
+  cplx<R> e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  cplx<R> o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  bfly4<R, INV>(e0, e1, e2, e3);
+  bfly4<R, INV>(o0, o1, o2, o3);
+  // W8^m * O_m
+  cplx<R> w1, w2, w3;
+  if (!INV) {
+    w1 = {h * (o1.x + o1.y), h * (o1.y - o1.x)};
+    w2 = mul_mi(o2);
+    w3 = {h * (o3.y - o3.x), -h * (o3.x + o3.y)};
+  } else {
+    w1 = {h * (o1.x - o1.y), h * (o1.x + o1.y)};
+    w2 = mul_i(o2);
+    w3 = {-h * (o3.x + o3.y), h * (o3.x - o3.y)};
+  }
+  a[0] = e0 + o0;
+  a[4] = e0 - o0;
+  a[1] = e1 + w1;
+  a[5] = e1 - w1;
+  a[2] = e2 + w2;
+  a[6] = e2 - w2;
+  a[3] = e3 + w3;
+  a[7] = e3 - w3;
+}
+
+template <typename R, bool INV, int RAD>
+__device__ __forceinline__ void bfly(cplx<R>* a) {
+  if constexpr (RAD == 2) bfly2<R, INV>(a);
+  else if constexpr (RAD == 3) bfly3<R, INV>(a);
+  else if constexpr (RAD == 4) bfly4<R, INV>(a[0], a[1], a[2], a[3]);
+  else if constexpr (RAD == 8) bfly8<R, INV>(a);
+}
+
+// ---------------------------------------------------------------------------
+// Team FFT.  `lds` points at this team's element 0, element p lives at
+// lds[p * ls] (ls = 1 for private row storage, = team count for interleaved
+// column storage).  `tw` = exp(-2 pi i n / L), n in [0, L).
+// ---------------------------------------------------------------------------
+template <typename R, int L, bool INV>
+struct TeamFFT {
+  static constexpr FftPlanData P = make_fft_plan(L);
+  static constexpr int E = P.E;
+  static constexpr int TT = L / E;
+  static constexpr int R0 = P.rad[0];
+  static constexpr int RL = P.rad[P.np - 1];
+
+  __device__ static __forceinline__ int in_pos(int t, int e) { return t + (e / R0) * TT + (e % R0) * (L / R0); }
+  __device__ static __forceinline__ int out_pos(int t, int e) { return t + (e / RL) * TT + (e % RL) * (L / RL); }
+
+  template <int PI, int LS>
+  __device__ static __forceinline__ void passes(cplx<R> (&v)[E], cplx<R>* lds, int ls, int t,
+                                                const cplx<R>* __restrict__ tw) {
+    if constexpr (PI < P.np) {
+      constexpr int RAD = P.rad[PI];
+      constexpr int NB = E / RAD;
+      constexpr int LR = L / RAD;
+      constexpr int TWS = L / (LS * RAD);
+      if constexpr (PI > 0) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int j = t + q * TT;
+#pragma unroll
+          for (int k = 0; k < RAD; ++k) v[q * RAD + k] = lds[(j + k * LR) * ls];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        const int j = t + q * TT;
+        if constexpr (LS > 1) {
+          const int jm = j % LS;
+#pragma unroll
+          for (int k = 1; k < RAD; ++k) {
+            cplx<R> w = tw[jm * k * TWS];
+            if (INV) w.y = -w.y;
+            v[q * RAD + k] = v[q * RAD + k] * w;
+          }
+        }
+        bfly<R, INV, RAD>(&v[q * RAD]);
+      }
+      if constexpr (PI + 1 < P.np) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int j = t + q * TT;
+          const int base = (j / LS) * LS * RAD + (j % LS);
+#pragma unroll
+          for (int m = 0; m < RAD; ++m) lds[(base + m * LS) * ls] = v[q * RAD + m];
+        }
+        __syncthreads();
+        passes<PI + 1, LS * RAD>(v, lds, ls, t, tw);
+      }
+    }
+  }
+
+  __device__ static __forceinline__ void run(cplx<R> (&v)[E], cplx<R>* lds, int ls, int t,
+                                             const cplx<R>* __restrict__ tw) {
+    passes<0, 1>(v, lds, ls, t, tw);
+  }
+};
+
+// Multipliers applied while loading the first inverse pass.
+enum SpecMul : int {
+  MUL_NONE = 0,
+  MUL_DX = 1,   // i*kx
+  MUL_DY = 2,   // i*ky
+  MUL_DZ = 3,   // i*kz
+  MUL_LAP = 4,  // -(kx^2+ky^2+kz^2)
+  MUL_BIH = 5,  // (kx^2+ky^2+kz^2)^2
+};
+
+// Device wavenumber tables (FP64): kx rfftfreq (Nyquist +), ky/kz fftfreq (Nyquist -),
+// kx0 fftfreq over Nx (Nyquist -).
+struct KTables {
+  const double* kx;   // Nxh
+  const double* ky;   // Ny
+  const double* kz;   // Nz (3D) or nullptr
+  const double* kx0;  // Nx
+};
+
+template <typename R>
+__device__ __forceinline__ cplx<R> apply_mul(cplx<R> a, int mul, double kx, double ky, double kz) {
+  switch (mul) {
+    // (0 + i k)(a.x + i a.y) = (-k a.y) + i (k a.x)
+    case MUL_DX: { R k = (R)kx; return cplx<R>{-k * a.y, k * a.x}; }
+    case MUL_DY: { R k = (R)ky; return cplx<R>{-k * a.y, k * a.x}; }
+    case MUL_DZ: { R k = (R)kz; return cplx<R>{-k * a.y, k * a.x}; }
+    case MUL_LAP: {
+      R k2 = (R)(kx * kx + ky * ky + kz * kz);
+      R l = -k2;
+      return cplx<R>{l * a.x, l * a.y};
+    }
+    case MUL_BIH: {
+      R k2 = (R)(kx * kx + ky * ky + kz * kz);
+      R b = k2 * k2;
+      return cplx<R>{b * a.x, b * a.y};
+    }
+    default: return a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Job descriptors for batched launches (up to 8 jobs per launch).
+// ---------------------------------------------------------------------------
+constexpr int MAXJ = 12;
+
+struct SpecJob {
+  const void* src;   // half spectrum (or physical for r2c)
+  const void* src2;  // optional: load 2*src - src2
+  void* dst;         // destination
+  int mul;           // SpecMul applied on load (inverse only)
+  int count;         // number of scalar fields in this job
+  double scale;      // extra real factor applied on load
+  long long toff;    // offset (complex elements) of this job inside the irfft scratch
+};
+
+struct SpecJobs {
+  SpecJob j[MAXJ];
+  int n;
+};
+
+// Host API -------------------------------------------------------------------
+template <typename R>
+struct FftGeom {
+  int ndim, Nx, Ny, Nz, Nxh;
+  const cplx<R>* tw_x2;   // table for L = Nx (length Nx) -- r2c/c2r post-twiddles
+  const cplx<R>* tw_xh;   // table for L = Nx/2
+  const cplx<R>* tw_x;    // table for L = Nx (full c2c rows)
+  const cplx<R>* tw_y;    // table for L = Ny
+  const cplx<R>* tw_z;    // table for L = Nz
+  KTables k;
+};
+
+// rfft over the spatial axes: dst_j = rfft(src_j), count_j fields each.
+template <typename R>
+void rfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs, hipStream_t s);
+
+// irfft over the spatial axes with an optional spectral multiplier on load:
+// dst_j = irfft(mul_j * scale_j * (src_j or 2 src_j - src2_j)).  `tmp` must hold
+// sum(count_j) half-spectrum fields.
+template <typename R>
+void irfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs, cplx<R>* tmp, hipStream_t s);
+
+// Full complex 2D transforms over (x, y) of `count` fields, in place.
+template <typename R>
+void fft_full_2d(const FftGeom<R>& g, cplx<R>* data, int count, bool inverse, hipStream_t s);
+
+bool fft_size_supported(int n);
+
+}  // namespace sav

@@ -1,0 +1,47 @@
+// psi BiCGSTAB of src/Solvers.jl:39-81 on full complex spectra (2D).
+#pragma once
+#include "fft.h"
+#include "solver.h"
+
+namespace sav {
+
+template <typename R>
+struct KrylovCtx {
+  int Nx, Ny, N;
+  long long nfull;            // N * Nx * Ny
+  const double* kx0;          // fftfreq wavenumbers over Nx (Nyquist -)
+  const double* ky;
+  const cplx<R>* tw_x;        // length-Nx table
+  const cplx<R>* tw_y;        // length-Ny table
+  const R* M;                 // M_psi(phi*), physical (N, Ny, Nx)
+  const DScal* ds;            // a, dt, Mmax
+  double S4, atol, rtol;
+  int itmax;
+  cplx<R> *x, *r, *p, *v, *s, *t, *b, *Bx, *By;
+  double* partials;           // >= 3 * max blocks doubles
+  KScal* ks;                  // device, one per solve slot
+  KScal* hks;                 // pinned host mirror
+  hipStream_t st;
+  int guess[3];               // expected iteration count per slot (batching of convergence checks)
+  // optional event timing of one matvec kernel (1: column inverse, 2: rows, 3: column forward)
+  int timed = 0;
+  hipEvent_t* ev = nullptr;   // 2 * ev_cap events
+  int ev_cap = 0, ev_used = 0;
+  int cur_iter = 0;
+  int ev_iter[512];
+  double ev_bytes[512];
+  long long tk_count = 0;
+  double tk_ms = 0.0, tk_bytes = 0.0;
+};
+
+// Solves A x = b with b already in ctx.b; x in ctx.x on return.  Returns the
+// number of iterations; ctx.hks[slot] holds the final Krylov scalars.
+template <typename R>
+int bicgstab_2d(KrylovCtx<R>& k, int slot);
+
+// A*y then the diagonal preconditioner: out = P^{-1} (a/dt y - sum_d D1f_d F(M F^{-1}(D1f_d S4 y)))
+// (one matvec + preconditioner, used by tests through the solver).
+template <typename R>
+void matvec_prec_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot);
+
+}  // namespace sav

@@ -1,0 +1,549 @@
+// BiCGSTAB for the variable-mobility psi operator (2D), restated from the
+// algorithm of Krylov.jl 0.10.6 `bicgstab` as called at src/Solvers.jl:73-75:
+// left preconditioner M = P^{-1} applied by multiplication, shadow vector c = b,
+// x0 = 0, eps = atol + rtol*||M b||, stop on ||r|| <= eps, iter >= itmax, alpha == 0.
+//
+// Operator (src/Solvers.jl:39-53), on full complex spectra of N stacked fields:
+//   A x = (a/dt) x - sum_d D1f_d .* FFT( M_psi .* IFFT( D1f_d .* (S4 x) ) )
+// Preconditioner (src/Solvers.jl:57): P^{-1} = 1 / (a/dt - Mmax * Lap_full * S4).
+//
+// A matvec is three kernels: column inverse FFTs of D1f_x S4 x and D1f_y S4 x,
+// one row kernel doing inverse FFT -> *M_psi -> forward FFT, and column forward
+// FFTs fused with the combination, the preconditioner and the next dot products.
+// All Krylov scalars stay on the device; every kernel of an iteration returns at
+// once when the solve has finished, so iterations can be queued ahead of the
+// host's convergence check.
+#include <cmath>
+
+#include "krylov.h"
+#include "launch.h"
+
+namespace sav {
+
+template <typename R, int L>
+struct KColCfg {
+  using F = TeamFFT<R, L, false>;
+  static constexpr int TT = F::TT;
+  static constexpr int ES = (int)sizeof(cplx<R>);
+  static constexpr int want = (64 / ES) > (256 / TT) ? (64 / ES) : (256 / TT);
+  static constexpr int cap0 = 65536 / (L * ES);
+  static constexpr int capp2 = cap0 >= 256 ? 256 : cap0 >= 128 ? 128 : cap0 >= 64 ? 64 : cap0 >= 32 ? 32 : cap0 >= 16 ? 16 : cap0 >= 8 ? 8 : cap0 >= 4 ? 4 : cap0 >= 2 ? 2 : 1;
+  static constexpr int C0 = want < capp2 ? want : capp2;
+  static constexpr int C = (C0 * TT > 512) ? (512 / TT) : C0;
+  static constexpr int NT = C * TT;
+};
+
+template <typename R, int L>
+struct KRowCfg {
+  using F = TeamFFT<R, L, false>;
+  static constexpr int TT = F::TT;
+  static constexpr int ROWS = TT >= 256 ? 1 : 256 / TT;
+  static constexpr int NT = ROWS * TT;
+};
+
+__device__ __forceinline__ double pinv_diag(double a_dt, double Mmax, double S4, double kx0, double ky) {
+  const double lapf = -(kx0 * kx0 + ky * ky);
+  return 1.0 / (a_dt - Mmax * lapf * S4);
+}
+
+// ---------------------------------------------------------------------------
+// K1: column inverse FFTs of D1f_x*(S4 x) and D1f_y*(S4 x)
+// ---------------------------------------------------------------------------
+template <typename R, int L>
+__global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
+                                                                    cplx<R>* __restrict__ Bx,
+                                                                    cplx<R>* __restrict__ By, int Nx, double S4,
+                                                                    const double* __restrict__ kx0,
+                                                                    const double* __restrict__ ky,
+                                                                    const cplx<R>* __restrict__ tw,
+                                                                    const KScal* __restrict__ ks) {
+  if (ks->done) return;
+  using F = TeamFFT<R, L, true>;
+  constexpr int C = KColCfg<R, L>::C;
+  constexpr int E = F::E;
+  __shared__ cplx<R> lds[C * L];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const bool act = col < Nx;
+  const long long base = (long long)blockIdx.y * Nx * L + col;
+  const R kxv = act ? (R)kx0[col] : R(0);
+  const R s4 = (R)S4;
+  cplx<R> vx[E], vy[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pos = F::in_pos(t, e);
+    cplx<R> a = act ? X[base + (long long)pos * Nx] : cplx<R>{R(0), R(0)};
+    const cplx<R> b1 = cplx<R>{s4 * a.x, s4 * a.y};
+    const R kyv = (R)ky[pos];
+    vx[e] = cplx<R>{-kxv * b1.y, kxv * b1.x};
+    vy[e] = cplx<R>{-kyv * b1.y, kyv * b1.x};
+  }
+  F::run(vx, lds + c, C, t, tw);
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) Bx[base + (long long)F::out_pos(t, e) * Nx] = vx[e];
+  }
+  F::run(vy, lds + c, C, t, tw);
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) By[base + (long long)F::out_pos(t, e) * Nx] = vy[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: rows: inverse FFT (scaled 1/(Nx Ny)) -> * M_psi -> forward FFT, in place
+// ---------------------------------------------------------------------------
+template <typename R, int L>
+__global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __restrict__ Bx, cplx<R>* __restrict__ By,
+                                                                const R* __restrict__ M, long long nrows_half,
+                                                                double scale, const cplx<R>* __restrict__ tw,
+                                                                const KScal* __restrict__ ks) {
+  if (ks->done) return;
+  using FI = TeamFFT<R, L, true>;
+  using FF = TeamFFT<R, L, false>;
+  constexpr int TT = FI::TT, E = FI::E, ROWS = KRowCfg<R, L>::ROWS;
+  __shared__ cplx<R> lds[ROWS * (L + 1)];
+  const int rloc = threadIdx.x / TT, t = threadIdx.x % TT;
+  const long long row = (long long)blockIdx.x * ROWS + rloc;
+  const bool act = row < 2 * nrows_half;
+  const long long rr = act ? (row % nrows_half) : 0;
+  cplx<R>* data = (row < nrows_half ? Bx : By) + rr * L;
+  const R* m = M + rr * L;
+  cplx<R>* my = lds + rloc * (L + 1);
+  cplx<R> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = act ? data[FI::in_pos(t, e)] : cplx<R>{R(0), R(0)};
+  FI::run(v, my, 1, t, tw);
+  const R sc = (R)scale;
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int pos = FI::out_pos(t, e);
+    const R mv = act ? m[pos] : R(0);
+    const cplx<R> z = cplx<R>{sc * v[e].x, sc * v[e].y};
+    my[pos] = cplx<R>{mv * z.x, mv * z.y};
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = my[FF::in_pos(t, e)];
+  FF::run(v, my, 1, t, tw);
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) data[FF::out_pos(t, e)] = v[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K3: column forward FFTs of Bx, By, combine, precondition, partial dots.
+//   MODE 1: out = P^{-1} A y and partial <c, out>
+//   MODE 2: out = P^{-1} A y and partials <out, y>, <out, out>
+//   MODE 0: out only
+// ---------------------------------------------------------------------------
+template <typename R, int L, int MODE>
+__global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_fwd(
+    const cplx<R>* __restrict__ Bx, const cplx<R>* __restrict__ By, const cplx<R>* __restrict__ Y,
+    const cplx<R>* __restrict__ Cv, cplx<R>* __restrict__ out, int Nx, double S4, const double* __restrict__ kx0,
+    const double* __restrict__ ky, const cplx<R>* __restrict__ tw, const DScal* __restrict__ ds,
+    const KScal* __restrict__ ks, double* __restrict__ partials) {
+  if (ks->done) return;
+  using F = TeamFFT<R, L, false>;
+  constexpr int C = KColCfg<R, L>::C;
+  constexpr int E = F::E;
+  __shared__ cplx<R> lds[C * L];
+  __shared__ double scratch[3 * 16];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const bool act = col < Nx;
+  const long long base = (long long)blockIdx.y * Nx * L + col;
+  cplx<R> vx[E], vy[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const long long off = base + (long long)F::in_pos(t, e) * Nx;
+    vx[e] = act ? Bx[off] : cplx<R>{R(0), R(0)};
+    vy[e] = act ? By[off] : cplx<R>{R(0), R(0)};
+  }
+  F::run(vx, lds + c, C, t, tw);
+  F::run(vy, lds + c, C, t, tw);
+  const double a_dt = ds->a / ds->dt;
+  const double Mmax = ds->Mmax;
+  const double kxd = act ? kx0[col] : 0.0;
+  const R kxv = (R)kxd;
+  const R adt = (R)a_dt;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int pos = F::out_pos(t, e);
+      const long long off = base + (long long)pos * Nx;
+      const double kyd = ky[pos];
+      const R kyv = (R)kyd;
+      const cplx<R> y = Y[off];
+      const cplx<R> dx = cplx<R>{-kxv * vx[e].y, kxv * vx[e].x};
+      const cplx<R> dy = cplx<R>{-kyv * vy[e].y, kyv * vy[e].x};
+      const cplx<R> sum = dx + dy;
+      const cplx<R> q = cplx<R>{adt * y.x - sum.x, adt * y.y - sum.y};
+      const R pv = (R)pinv_diag(a_dt, Mmax, S4, kxd, kyd);
+      const cplx<R> o = cplx<R>{q.x * pv, q.y * pv};
+      out[off] = o;
+      if constexpr (MODE == 1) {
+        const cplx<R> cc = Cv[off];
+        acc[0] += (double)cc.x * (double)o.x + (double)cc.y * (double)o.y;
+        acc[1] += (double)cc.x * (double)o.y - (double)cc.y * (double)o.x;
+      } else if constexpr (MODE == 2) {
+        acc[0] += (double)o.x * (double)y.x + (double)o.y * (double)y.y;
+        acc[1] += (double)o.x * (double)y.y - (double)o.y * (double)y.x;
+        acc[2] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+      }
+    }
+  }
+  if constexpr (MODE != 0) {
+    block_sum<3>(acc, scratch);
+    if (threadIdx.x == 0) {
+      const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+      partials[b * 3 + 0] = acc[0];
+      partials[b * 3 + 1] = acc[1];
+      partials[b * 3 + 2] = acc[2];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_kry_init(const cplx<R>* __restrict__ b, cplx<R>* __restrict__ r,
+                                                  cplx<R>* __restrict__ p, cplx<R>* __restrict__ x, long long n,
+                                                  int Nx, int Ny, double S4, const double* __restrict__ kx0,
+                                                  const double* __restrict__ ky, const DScal* __restrict__ ds,
+                                                  double* __restrict__ partials) {
+  __shared__ double scratch[3 * 4];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const int ix = (int)(i % Nx);
+    const int iy = (int)((i / Nx) % Ny);
+    const R pv = (R)pinv_diag(a_dt, Mmax, S4, kx0[ix], ky[iy]);
+    const cplx<R> bb = b[i];
+    const cplx<R> rr = cplx<R>{bb.x * pv, bb.y * pv};
+    r[i] = rr;
+    p[i] = rr;
+    x[i] = cplx<R>{R(0), R(0)};
+    acc[0] += (double)bb.x * (double)rr.x + (double)bb.y * (double)rr.y;
+    acc[1] += (double)bb.x * (double)rr.y - (double)bb.y * (double)rr.x;
+    acc[2] += (double)rr.x * (double)rr.x + (double)rr.y * (double)rr.y;
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 3 + 0] = acc[0];
+    partials[blockIdx.x * 3 + 1] = acc[1];
+    partials[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+__device__ __forceinline__ void sum_partials(const double* partials, int nb, double* out3) {
+  // single-thread, fixed order
+  double a = 0, b = 0, c = 0;
+  for (int i = 0; i < nb; ++i) {
+    a += partials[i * 3 + 0];
+    b += partials[i * 3 + 1];
+    c += partials[i * 3 + 2];
+  }
+  out3[0] = a;
+  out3[1] = b;
+  out3[2] = c;
+}
+
+// Finalise a 3-wide partial array with one block (deterministic order).
+__global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ partials, int nb, KScal* __restrict__ ks,
+                                                 int stage, double atol, double rtol, int itmax) {
+  __shared__ double scratch[3 * 4];
+  if (stage != 0 && ks->done) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    acc[0] += partials[i * 3 + 0];
+    acc[1] += partials[i * 3 + 1];
+    acc[2] += partials[i * 3 + 2];
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x != 0) return;
+  KScal k = *ks;
+  if (stage == 0) {  // after init: <c, r0>, ||r0||^2
+    k.iter = 0;
+    k.done = 0;
+    k.solved = 0;
+    k.status = 0;
+    k.rnorm = sqrt(acc[2]);
+    k.ar = k.ai = 0;
+    k.wr = k.wi = 0;
+    if (k.rnorm == 0.0) {
+      k.done = 1;
+      k.solved = 1;
+      k.status = 4;
+    } else {
+      k.eps = atol + rtol * k.rnorm;
+      k.nr = acc[0];
+      k.ni = acc[1];
+      if (k.nr == 0.0 && k.ni == 0.0) {
+        k.done = 1;
+        k.status = 5;
+      } else if (k.rnorm <= k.eps) {
+        k.done = 1;
+        k.solved = 1;
+        k.status = 1;
+      } else if (itmax <= 0) {
+        k.done = 1;
+        k.status = 2;
+      }
+    }
+  } else if (stage == 1) {  // alpha = rho / <c, v>
+    k.rr = k.nr;
+    k.ri = k.ni;
+    const zd al = zdiv(zd{k.rr, k.ri}, zd{acc[0], acc[1]});
+    k.ar = al.x;
+    k.ai = al.y;
+  } else if (stage == 2) {  // omega = <t, s> / <t, t>
+    const zd om = zdiv(zd{acc[0], acc[1]}, zd{acc[2], 0.0});
+    k.wr = om.x;
+    k.wi = om.y;
+  } else {  // stage 3: next rho, ||r||, beta, stopping tests
+    k.nr = acc[0];
+    k.ni = acc[1];
+    k.rnorm = sqrt(acc[2]);
+    const zd q1 = zdiv(zd{k.nr, k.ni}, zd{k.rr, k.ri});
+    const zd q2 = zdiv(zd{k.ar, k.ai}, zd{k.wr, k.wi});
+    const zd be = zmul(q1, q2);
+    k.br = be.x;
+    k.bi = be.y;
+    k.iter += 1;
+    const bool solved = (k.rnorm <= k.eps) || (k.rnorm + 1.0 <= 1.0);
+    const bool tired = k.iter >= itmax;
+    const bool breakdown = (k.ar == 0.0 && k.ai == 0.0) || (k.ar != k.ar) || (k.ai != k.ai);
+    if (solved || tired || breakdown) {
+      k.done = 1;
+      k.solved = solved ? 1 : 0;
+      k.status = solved ? 1 : (breakdown ? 3 : 2);
+    }
+  }
+  *ks = k;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_kry_upd1(const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ v,
+                                                  const cplx<R>* __restrict__ p, cplx<R>* __restrict__ s,
+                                                  cplx<R>* __restrict__ x, long long n, const KScal* __restrict__ ks) {
+  if (ks->done) return;
+  const R ar = (R)ks->ar, ai = (R)ks->ai;
+  const cplx<R> ma = cplx<R>{-ar, -ai};
+  const cplx<R> al = cplx<R>{ar, ai};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    s[i] = r[i] + ma * v[i];
+    x[i] = x[i] + al * p[i];
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_kry_upd2(const cplx<R>* __restrict__ s, const cplx<R>* __restrict__ t,
+                                                  const cplx<R>* __restrict__ c, cplx<R>* __restrict__ x,
+                                                  cplx<R>* __restrict__ r, long long n, const KScal* __restrict__ ks,
+                                                  double* __restrict__ partials) {
+  __shared__ double scratch[3 * 4];
+  if (ks->done) return;
+  const R wr = (R)ks->wr, wi = (R)ks->wi;
+  const cplx<R> om = cplx<R>{wr, wi};
+  const cplx<R> mom = cplx<R>{-wr, -wi};
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const cplx<R> si = s[i];
+    x[i] = x[i] + om * si;
+    const cplx<R> ri = si + mom * t[i];
+    r[i] = ri;
+    const cplx<R> ci = c[i];
+    acc[0] += (double)ci.x * (double)ri.x + (double)ci.y * (double)ri.y;
+    acc[1] += (double)ci.x * (double)ri.y - (double)ci.y * (double)ri.x;
+    acc[2] += (double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y;
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 3 + 0] = acc[0];
+    partials[blockIdx.x * 3 + 1] = acc[1];
+    partials[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_kry_upd3(const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ v,
+                                                  cplx<R>* __restrict__ p, long long n, const KScal* __restrict__ ks) {
+  if (ks->done) return;
+  const cplx<R> mom = cplx<R>{(R)-ks->wr, (R)-ks->wi};
+  const cplx<R> be = cplx<R>{(R)ks->br, (R)ks->bi};
+  const cplx<R> one = cplx<R>{R(1), R(0)};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const cplx<R> pa = p[i] + mom * v[i];
+    p[i] = one * r[i] + be * pa;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static int vec_blocks(long long n) {
+  long long nb = (n + 1023) / 1024;
+  if (nb < 1) nb = 1;
+  if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
+  return (int)nb;
+}
+
+#define SAV_KRY_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
+template <typename R>
+static void tev_pre(KrylovCtx<R>& k, int id) {
+  if (k.timed == id && k.ev_used < k.ev_cap) SAV_HIP(hipEventRecord(k.ev[2 * k.ev_used], k.st));
+}
+template <typename R>
+static void tev_post(KrylovCtx<R>& k, int id, double bytes) {
+  if (k.timed == id && k.ev_used < k.ev_cap) {
+    SAV_HIP(hipEventRecord(k.ev[2 * k.ev_used + 1], k.st));
+    k.ev_iter[k.ev_used] = k.cur_iter;
+    k.ev_bytes[k.ev_used] = bytes;
+    k.ev_used++;
+  }
+}
+
+// Accumulates the timed launches that belonged to real iterations (<= iters).
+template <typename R>
+static void tev_collect(KrylovCtx<R>& k, int iters) {
+  for (int i = 0; i < k.ev_used; ++i) {
+    if (k.ev_iter[i] > iters) continue;
+    float ms = 0.f;
+    SAV_HIP(hipEventElapsedTime(&ms, k.ev[2 * i], k.ev[2 * i + 1]));
+    k.tk_ms += ms;
+    k.tk_bytes += k.ev_bytes[i];
+    k.tk_count += 1;
+  }
+  k.ev_used = 0;
+}
+
+template <typename R>
+static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot) {
+  const KScal* ks = k.ks + slot;
+  int nb3 = 0;
+  const double sn = (double)sizeof(R) * (double)k.nfull;
+  tev_pre(k, 1);
+  switch (k.Ny) {
+#define X(L)                                                                                                  \
+  case L: {                                                                                                   \
+    using CC = KColCfg<R, L>;                                                                                 \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);                                                               \
+    hipLaunchKernelGGL((k_mv_cols_inv<R, L>), grid, dim3(CC::NT), 0, k.st, y, k.Bx, k.By, k.Nx, k.S4, k.kx0,  \
+                       k.ky, k.tw_y, ks);                                                                     \
+    break;                                                                                                    \
+  }
+    SAV_KRY_SIZES(X)
+#undef X
+    default: throw HipError("unsupported Ny for the psi solve: " + std::to_string(k.Ny), -1);
+  }
+  SAV_LAUNCH_CHECK();
+  tev_post(k, 1, 6.0 * sn);
+  tev_pre(k, 2);
+  const long long nrh = (long long)k.N * k.Ny;
+  const double scale = 1.0 / ((double)k.Nx * k.Ny);
+  switch (k.Nx) {
+#define X(L)                                                                                             \
+  case L: {                                                                                              \
+    using CC = KRowCfg<R, L>;                                                                            \
+    dim3 grid((unsigned)((2 * nrh + CC::ROWS - 1) / CC::ROWS));                                          \
+    hipLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, k.M, nrh, scale, k.tw_x, \
+                       ks);                                                                              \
+    break;                                                                                               \
+  }
+    SAV_KRY_SIZES(X)
+#undef X
+    default: throw HipError("unsupported Nx for the psi solve: " + std::to_string(k.Nx), -1);
+  }
+  SAV_LAUNCH_CHECK();
+  tev_post(k, 2, 9.0 * sn);
+  tev_pre(k, 3);
+  switch (k.Ny) {
+#define X(L)                                                                                                   \
+  case L: {                                                                                                    \
+    using CC = KColCfg<R, L>;                                                                                  \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);                                                                \
+    nb3 = (int)(grid.x * grid.y);                                                                              \
+    if (mode == 1)                                                                                             \
+      hipLaunchKernelGGL((k_mv_cols_fwd<R, L, 1>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, y, k.b, out, k.Nx, \
+                         k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials);                                     \
+    else if (mode == 2)                                                                                        \
+      hipLaunchKernelGGL((k_mv_cols_fwd<R, L, 2>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, y, k.b, out, k.Nx, \
+                         k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials);                                     \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_mv_cols_fwd<R, L, 0>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, y, k.b, out, k.Nx, \
+                         k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials);                                     \
+    break;                                                                                                     \
+  }
+    SAV_KRY_SIZES(X)
+#undef X
+    default: break;
+  }
+  SAV_LAUNCH_CHECK();
+  tev_post(k, 3, (mode == 1 ? 10.0 : mode == 2 ? 8.0 : 6.0) * sn);
+  return nb3;
+}
+
+template <typename R>
+void matvec_prec_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot) {
+  // make sure the slot is "running" so the kernels do not skip
+  KScal z{};
+  SAV_HIP(hipMemcpyAsync(k.ks + slot, &z, sizeof(KScal), hipMemcpyHostToDevice, k.st));
+  matvec_launch(k, y, out, 0, slot);
+  SAV_HIP(hipStreamSynchronize(k.st));
+}
+
+template <typename R>
+int bicgstab_2d(KrylovCtx<R>& k, int slot) {
+  const long long n = k.nfull;
+  const int nbv = vec_blocks(n);
+  KScal* ks = k.ks + slot;
+  hipLaunchKernelGGL((k_kry_init<R>), dim3(nbv), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4, k.kx0,
+                     k.ky, k.ds, k.partials);
+  SAV_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 0, k.atol, k.rtol, k.itmax);
+  SAV_LAUNCH_CHECK();
+  int queued = 0;
+  int batch = k.guess[slot] > 0 ? k.guess[slot] : 8;
+  while (true) {
+    const int nb = batch < k.itmax - queued ? batch : k.itmax - queued;
+    for (int it = 0; it < nb; ++it) {
+      k.cur_iter = queued + it + 1;
+      // v = P^{-1} A p ; alpha = rho / <c, v>
+      int nb3 = matvec_launch(k, k.p, k.v, 1, slot);
+      hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 1, k.atol, k.rtol, k.itmax);
+      // s = r - alpha v ; x += alpha p
+      hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks);
+      // t = P^{-1} A s ; omega = <t, s> / <t, t>
+      nb3 = matvec_launch(k, k.s, k.t, 2, slot);
+      hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 2, k.atol, k.rtol, k.itmax);
+      // x += omega s ; r = s - omega t ; <c, r>, ||r||
+      hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks, k.partials);
+      hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 3, k.atol, k.rtol, k.itmax);
+      // p = r + beta (p - omega v)
+      hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks);
+      SAV_LAUNCH_CHECK();
+    }
+    queued += nb;
+    SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
+    SAV_HIP(hipStreamSynchronize(k.st));
+    if (k.hks[slot].done || queued >= k.itmax) break;
+    batch = 2;
+  }
+  const int iters = k.hks[slot].iter;
+  k.guess[slot] = iters > 0 ? iters : 1;
+  if (k.timed) tev_collect(k, iters);
+  return iters;
+}
+
+template int bicgstab_2d<double>(KrylovCtx<double>&, int);
+template int bicgstab_2d<float>(KrylovCtx<float>&, int);
+template void matvec_prec_2d<double>(KrylovCtx<double>&, const cplx<double>*, cplx<double>*, int);
+template void matvec_prec_2d<float>(KrylovCtx<float>&, const cplx<float>*, cplx<float>*, int);
+
+}  // namespace sav

@@ -1,0 +1,74 @@
+// Device-resident SAV solver (one grid, 2D or 3D, FP64 or FP32 fields).
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sav_abi.h"
+#include "fft.h"
+
+namespace sav {
+
+constexpr int MAXN = 16;  // vesicles per instance
+
+// Scalars of one FieldState (Q, R1, R2, R3 of src/Types.jl:193-196).
+struct DStateScal {
+  double Q, R1, R2, R3;
+};
+
+// Device scalars of one step.  All reductions and scalar solves land here, so a
+// step runs without host round trips except for Krylov convergence checks.
+struct DScal {
+  double a, b, c, dt, dt5;  // BDF coefficients, dt of the step, dt used by step 5
+  double W1, W2, W3, R1s, R2s, R3s;
+  double area[MAXN];
+  double Mmax;
+  double Rs[6];  // R11 R21 R31 R12 R22 R32
+  double lhs[9], rhs1[3], rhs2[3];
+  double theta1, theta2, Q;
+  double energy, area_ratio;
+  double area_mon[MAXN];
+  int flags;  // 1: W1+C1<0, 2: W2+C2<0, 4: W3+C3<0, 8: theta2 >= 0 (3D), 16: non-finite
+  int pad;
+};
+
+// Krylov scalars of one BiCGSTAB solve (Krylov.jl bicgstab, restated).
+struct KScal {
+  double ar, ai;  // alpha
+  double wr, wi;  // omega
+  double rr, ri;  // rho
+  double nr, ni;  // next rho
+  double br, bi;  // beta
+  double rnorm, eps;
+  int iter, done, solved, status;  // status: 0 running, 1 solved, 2 itmax, 3 breakdown, 4 zero rhs, 5 bc=0
+};
+
+struct HostCfg {
+  sav_config_t c;
+  std::vector<double> psi_in, psi_out;
+};
+
+class ISolver {
+ public:
+  virtual ~ISolver() = default;
+  virtual void set_state(int which, const double* const* arrays, const double* scal, const double* A0, double dt) = 0;
+  virtual void get_state(int which, double* const* arrays, double* scal, double* A0, double* dt) = 0;
+  virtual int init_sav() = 0;
+  virtual int step(double dt, int n, sav_step_stats_t* st) = 0;
+  virtual int run(const double* t, int nn, int n0, double* eh, double* ah, int32_t* kit) = 0;
+  virtual void rfft_host(const double* in, double* out, int count) = 0;
+  virtual void irfft_host(const double* in, double* out, int count) = 0;
+  virtual void fft_full_host(const double* in, double* out, int count, int inverse) = 0;
+  virtual int eval_functionals(const double* phi, const double* psi, double* H1, double* H2, double* H3, double* G,
+                               double* W, double* area) = 0;
+  virtual int psi_solve_host(const double* phi_star, double a, double dt, const double* rhs, double* out,
+                             int32_t* iters, int32_t* solved, double* rnorm) = 0;
+  virtual int modified_energy(double* E, double* ar) = 0;
+  virtual double last_elapsed_ms() = 0;
+  virtual void kernel_timing(int id) = 0;
+  virtual void kernel_timing_get(long long* n, double* ms, double* bytes) = 0;
+};
+
+std::unique_ptr<ISolver> make_solver(const HostCfg& cfg);
+
+}  // namespace sav

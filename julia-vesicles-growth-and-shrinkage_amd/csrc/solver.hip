@@ -1,0 +1,1512 @@
+// Device-resident seven-step SAV timestep (src/Solvers.jl, src/Solvers_3d.jl),
+// functionals (src/Utils.jl), SAV initialisation and monitor (src/simulation.jl).
+//
+// Layout in HBM: every field is x-fastest, one scalar field = Nx*Ny*Nz values
+// ("P"), one half spectrum = (Nx/2+1)*Ny*Nz complex values ("H").  A FieldState
+// is two contiguous blocks:
+//   physical: [phi (N) | psi (N) | mu (N) | nu (N) | u (ndim) | p (1)]
+//   spectral: [phi_hat | psi_hat | mu_hat | nu_hat | u_hat | p_hat]
+// Three such buffers rotate (present, old, next) so update_state! is a pointer
+// swap and the back-transform of a new state is a single batched irfft.
+#include <cmath>
+#include <cstring>
+
+#include "krylov.h"
+#include "launch.h"
+#include "solver.h"
+
+namespace sav {
+
+__global__ __launch_bounds__(256) void k_red_fin(const double* __restrict__ partials, int nb, int K,
+                                                 double* __restrict__ out) {
+  __shared__ double sc[4];
+  const int k = blockIdx.x, g = blockIdx.y;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += partials[((long long)g * nb + b) * K + k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[g * K + k] = sc[0] + sc[1] + sc[2] + sc[3];
+}
+
+namespace {
+
+constexpr double SQ2 = 1.41421356237309504880168872420969808;
+
+// Twiddle table exp(-2 pi i n / L), n in [0, L), accurate to long double.
+template <typename R>
+std::vector<cplx<R>> twiddles(int L) {
+  std::vector<cplx<R>> w(L);
+  for (int n = 0; n < L; ++n) {
+    long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)n / (long double)L;
+    w[n] = cplx<R>{(R)cosl(ang), (R)sinl(ang)};
+  }
+  return w;
+}
+
+std::vector<double> rfftfreq_k(int n, double L) {  // 2 pi * rfftfreq(n, n/L)
+  std::vector<double> k(n / 2 + 1);
+  for (int i = 0; i <= n / 2; ++i) k[i] = 2.0 * M_PI * ((double)i * ((double)n / L) / (double)n);
+  return k;
+}
+std::vector<double> fftfreq_k(int n, double L) {  // 2 pi * fftfreq(n, n/L)
+  std::vector<double> k(n);
+  const double fs = (double)n / L;
+  for (int i = 0; i < n; ++i) {
+    int j = (i < (n + 1) / 2) ? i : i - n;
+    k[i] = 2.0 * M_PI * ((double)j * fs / (double)n);
+  }
+  return k;
+}
+
+struct KIdx {
+  int Nxh, Ny, Nz;
+  __device__ __forceinline__ void operator()(long long i, int& ix, int& iy, int& iz, int& f) const {
+    ix = (int)(i % Nxh);
+    long long r = i / Nxh;
+    iy = (int)(r % Ny);
+    r /= Ny;
+    iz = (int)(r % Nz);
+    f = (int)(r / Nz);
+  }
+};
+
+}  // namespace
+
+enum { F_PHI = 0, F_PSI = 1, F_MU = 2, F_NU = 3 };
+
+template <typename R>
+class Solver final : public ISolver {
+ public:
+  using C = cplx<R>;
+  HostCfg hc;
+  sav_config_t c;
+  int nd, N, Nx, Ny, Nz, Nxh;
+  long long P, H;
+  double dv;  // dx*dy(*dz) in the reference's multiplication order is applied explicitly
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  FftGeom<R> g{};
+  std::vector<void*> allocs;
+  // tables
+  double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_kx0 = nullptr;
+  // states
+  R* sphys[3];
+  C* shalf[3];
+  int ip = 0, io = 1, inew = 2;
+  DStateScal* dstate = nullptr;  // [3]
+  double* dA0 = nullptr;
+  double *d_psi_in = nullptr, *d_psi_out = nullptr;
+  DScal* ds = nullptr;
+  DScal* hs = nullptr;  // pinned
+  double* partials = nullptr;
+  double* red = nullptr;  // reduction outputs (512 doubles)
+  // work
+  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_s2, *w_du, *w_dmn, *w_conv,
+      *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy;
+  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp;
+  // Krylov
+  KrylovCtx<R> kc{};
+  KScal* hks = nullptr;
+  C* w_bphys_c = nullptr;
+  float last_ms = 0.f;
+
+  template <class T>
+  T* dalloc(size_t n) {
+    void* p = nullptr;
+    size_t bytes = n * sizeof(T) + 256;
+    SAV_HIP(hipMalloc(&p, bytes));
+    SAV_HIP(hipMemsetAsync(p, 0, bytes, s));
+    allocs.push_back(p);
+    return reinterpret_cast<T*>(p);
+  }
+
+  explicit Solver(const HostCfg& h) : hc(h), c(h.c) {
+    nd = c.ndim;
+    N = c.N;
+    Nx = c.Nx;
+    Ny = c.Ny;
+    Nz = nd == 3 ? c.Nz : 1;
+    Nxh = Nx / 2 + 1;
+    P = (long long)Nx * Ny * Nz;
+    H = (long long)Nxh * Ny * Nz;
+    c.psi_in_v = hc.psi_in.data();
+    c.psi_out_v = hc.psi_out.data();
+    SAV_HIP(hipSetDevice(c.device));
+    SAV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    SAV_HIP(hipEventCreate(&ev0));
+    SAV_HIP(hipEventCreate(&ev1));
+    // wavenumbers (SpectralUtils.jl:42-44; SpectralUtils_3d.jl)
+    auto kx = rfftfreq_k(Nx, c.Lx), ky = fftfreq_k(Ny, c.Ly), kx0 = fftfreq_k(Nx, c.Lx);
+    std::vector<double> kz = nd == 3 ? fftfreq_k(Nz, c.Lz) : std::vector<double>{0.0};
+    d_kx = upload(kx);
+    d_ky = upload(ky);
+    d_kz = upload(kz);
+    d_kx0 = upload(kx0);
+    g.ndim = nd;
+    g.Nx = Nx;
+    g.Ny = Ny;
+    g.Nz = Nz;
+    g.Nxh = Nxh;
+    g.tw_x = upload(twiddles<R>(Nx));
+    g.tw_x2 = g.tw_x;
+    g.tw_xh = upload(twiddles<R>(Nx / 2));
+    g.tw_y = upload(twiddles<R>(Ny));
+    g.tw_z = nd == 3 ? upload(twiddles<R>(Nz)) : g.tw_y;
+    g.k = KTables{d_kx, d_ky, d_kz, d_kx0};
+    // states
+    const long long nphys = (4LL * N + nd + 1) * P, nhalf = (4LL * N + nd + 1) * H;
+    for (int b = 0; b < 3; ++b) {
+      sphys[b] = dalloc<R>(nphys);
+      shalf[b] = dalloc<C>(nhalf);
+    }
+    dstate = dalloc<DStateScal>(3);
+    dA0 = dalloc<double>(MAXN);
+    d_psi_in = upload(hc.psi_in);
+    d_psi_out = upload(hc.psi_out);
+    ds = dalloc<DScal>(1);
+    SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(DScal), hipHostMallocDefault));
+    partials = dalloc<double>(3 * 65536);
+    red = dalloc<double>(1024);
+    // work buffers
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    w_phis = dalloc<R>(NP);
+    w_psis = dalloc<R>(NP);
+    w_M = dalloc<R>(NP);
+    w_us = dalloc<R>(nd * P);
+    w_d4 = dalloc<R>(6 * NP);
+    w_om = dalloc<R>(NP);
+    w_lom = dalloc<R>(NP);
+    w_H = dalloc<R>(4 * NP);
+    w_w = dalloc<R>(3 * NP);
+    w_adv = dalloc<R>(2 * NP);
+    w_s2 = dalloc<R>(8 * NP);
+    w_du = dalloc<R>(9 * P);
+    w_dmn = dalloc<R>(6 * NP);
+    w_conv = dalloc<R>(3 * P);
+    w_coupl = dalloc<R>(3 * P);
+    w_rhs2 = dalloc<R>(3 * P);
+    w_ut12 = dalloc<R>(6 * P);
+    w_s5 = dalloc<R>(4 * NP);
+    w_mon = dalloc<R>(6 * NP + 2 * P);
+    w_gxy = dalloc<R>(2 * NP);
+    w_phat = dalloc<C>(NH);
+    w_omh = dalloc<C>(NH);
+    w_Hh = dalloc<C>(4 * NH);
+    cache = dalloc<C>(16 * NH);
+    w_wh = dalloc<C>(3 * NH);
+    w_rhs = dalloc<C>(3 * NH);
+    w_advh = dalloc<C>(2 * NH);
+    s3 = dalloc<C>(8 * NH);
+    w_rhs2h = dalloc<C>(3 * H);
+    w_uth = dalloc<C>(6 * H);
+    tmp = dalloc<C>((16LL * N + 16) * H);
+    // Krylov workspace (2D)
+    SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks), 4 * sizeof(KScal), hipHostMallocDefault));
+    std::memset(hks, 0, 4 * sizeof(KScal));
+    if (nd == 2) {
+      const long long nf = (long long)N * Nx * Ny;
+      kc.Nx = Nx;
+      kc.Ny = Ny;
+      kc.N = N;
+      kc.nfull = nf;
+      kc.kx0 = d_kx0;
+      kc.ky = d_ky;
+      kc.tw_x = g.tw_x;
+      kc.tw_y = g.tw_y;
+      kc.M = w_M;
+      kc.ds = ds;
+      kc.S4 = c.S4;
+      kc.atol = c.atol;
+      kc.rtol = c.tol;
+      kc.itmax = c.itmax;
+      kc.x = dalloc<C>(nf);
+      kc.r = dalloc<C>(nf);
+      kc.p = dalloc<C>(nf);
+      kc.v = dalloc<C>(nf);
+      kc.s = dalloc<C>(nf);
+      kc.t = dalloc<C>(nf);
+      kc.b = dalloc<C>(nf);
+      kc.Bx = dalloc<C>(nf);
+      kc.By = dalloc<C>(nf);
+      kc.partials = partials;
+      kc.ks = dalloc<KScal>(4);
+      kc.hks = hks;
+      kc.st = s;
+      kc.guess[0] = kc.guess[1] = kc.guess[2] = 8;
+    }
+    SAV_HIP(hipStreamSynchronize(s));
+  }
+
+  ~Solver() override {
+    if (s) hipStreamSynchronize(s);
+    for (auto e : tev) hipEventDestroy(e);
+    for (void* p : allocs) hipFree(p);
+    if (hs) hipHostFree(hs);
+    if (hks) hipHostFree(hks);
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (s) hipStreamDestroy(s);
+  }
+
+  template <class T>
+  T* upload(const std::vector<T>& v) {
+    T* p = dalloc<T>(v.size());
+    SAV_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    return p;
+  }
+
+  // ---- field accessors ---------------------------------------------------
+  R* fphys(int b, int f) { return sphys[b] + (long long)f * N * P; }
+  C* fhalf(int b, int f) { return shalf[b] + (long long)f * N * H; }
+  R* uphys(int b) { return sphys[b] + 4LL * N * P; }
+  C* uhalf(int b) { return shalf[b] + 4LL * N * H; }
+  R* pphys(int b) { return sphys[b] + (4LL * N + nd) * P; }
+  C* phalf(int b) { return shalf[b] + (4LL * N + nd) * H; }
+  C* cf(int k) { return cache + (long long)k * N * H; }  // Step1Cache field k
+  enum {
+    PHI11 = 0, MU11, PSI11, NU11, PHI12, MU12, PSI12, NU12, PHI13, MU13, PHI14, MU14, PHI21, MU21, PSI21, NU21
+  };
+
+  // ---- transform helpers ---------------------------------------------------
+  static SpecJob job(const void* src, void* dst, int count, int mul = MUL_NONE, const void* src2 = nullptr,
+                     double scale = 1.0) {
+    SpecJob j{};
+    j.src = src;
+    j.src2 = src2;
+    j.dst = dst;
+    j.mul = mul;
+    j.count = count;
+    j.scale = scale;
+    return j;
+  }
+  void rfft(std::initializer_list<SpecJob> l) {
+    SpecJobs js{};
+    for (auto& j : l) js.j[js.n++] = j;
+    rfft_jobs<R>(g, js, s);
+  }
+  void irfft(std::initializer_list<SpecJob> l) {
+    SpecJobs js{};
+    for (auto& j : l) {
+      if (js.n == MAXJ) {
+        irfft_jobs<R>(g, js, tmp, s);
+        js.n = 0;
+      }
+      js.j[js.n++] = j;
+    }
+    irfft_jobs<R>(g, js, tmp, s);
+  }
+  void irfft_v(const std::vector<SpecJob>& l) {
+    SpecJobs js{};
+    for (auto& j : l) {
+      if (js.n == MAXJ) {
+        irfft_jobs<R>(g, js, tmp, s);
+        js.n = 0;
+      }
+      js.j[js.n++] = j;
+    }
+    irfft_jobs<R>(g, js, tmp, s);
+  }
+  int DX(int d) const { return MUL_DX + d; }
+
+  // ---- device-side constants ------------------------------------------------
+  struct PhysC {
+    double eps, S1, S2, S3, S4, gs, gb, gin, gout, bin, bout, M0, C1, C2, C3, garea, dx, dy, dz;
+  };
+  PhysC pc() const {
+    return PhysC{c.epsilon, c.S1, c.S2, c.S3, c.S4, c.gamma_surf, c.gamma_bend, (double)c.gamma_in,
+                 (double)c.gamma_out, (double)c.beta_in, (double)c.beta_out, c.M0_psi, c.C1, c.C2, c.C3,
+                 (double)c.gamma_area, c.Lx / c.Nx, c.Ly / c.Ny, nd == 3 ? c.Lz / c.Nz : 1.0};
+  }
+  // integrate(...) = sum * dx * dy [* dz]
+  static __host__ __device__ double integ(double sum, const PhysC& q, int nd) {
+    double v = sum * q.dx * q.dy;
+    if (nd == 3) v = v * q.dz;
+    return v;
+  }
+
+  // =========================================================================
+  // Functionals at (phi, psi): fills w_H = [H1 | G | H2 | H3] (physical), the
+  // scalars W1..W3, R1s..R3s, area[n] (Utils.jl:36-249).  When set_A0 is true
+  // (SAV initialisation) A0 := area first, as simulation.jl:25-26 does.
+  // =========================================================================
+  void functionals(const R* phi, const R* psi, bool set_A0) {
+    const long long NP = (long long)N * P;
+    const PhysC q = pc();
+    const int ndl = nd;
+    // derivatives of rfft(phi): gx, gy, lap, bih
+    rfft({job(phi, w_phat, N)});
+    R* gx = w_d4;
+    R* gy = w_d4 + NP;
+    R* lap = w_d4 + 2 * NP;
+    R* bih = w_d4 + 3 * NP;
+    irfft({job(w_phat, gx, N, MUL_DX), job(w_phat, gy, N, MUL_DY), job(w_phat, lap, N, MUL_LAP),
+           job(w_phat, bih, N, MUL_BIH)});
+    // per-vesicle sums: f_surf, f_bend, E2 integrand, f_osm, psi^2 ; omega -> w_om
+    R* om = w_om;
+    const double* pin = d_psi_in;
+    const double* pout = d_psi_out;
+    const long long Pl = P;
+    reduce_n<5>(s, P, N,
+                [=] __device__(long long i, int n, double(&acc)[5]) {
+                  const long long k = (long long)n * Pl + i;
+                  const R ph = phi[k], ps = psi[k], gxx = gx[k], gyy = gy[k], lp = lap[k];
+                  const R eps = (R)q.eps;
+                  // f_surf (Utils.jl:41)
+                  const R gg = R(0.25) * (ph * ph - R(1)) * (ph * ph - R(1));
+                  const R c1 = (R)(3.0 * SQ2 / 4.0);
+                  const R fs = c1 * (gg / eps) + eps * (gxx * gxx + gyy * gyy) / R(2);
+                  // f_bend (Utils.jl:49)
+                  const R gp = ph * (ph * ph - R(1));
+                  const R c2 = (R)(3.0 * SQ2 / (16.0 * q.eps));
+                  const R tb = gp / eps - eps * lp;
+                  const R fb = c2 * (tb * tb);
+                  // E2 integrand (Utils.jl:175-177)
+                  const R e2 = R(0.5) * (R)q.S1 * (lp * lp) + R(0.5) * (R)q.S2 * (gxx * gxx + gyy * gyy) +
+                               R(0.5) * (R)q.S3 * (ph * ph);
+                  // f_osmotic (Utils.jl:53-60)
+                  const R P_ = R(-0.5) * (ph * ph * ph) + R(1.5) * ph;
+                  const R di = ps - (R)pin[n], dou = ps - (R)pout[n];
+                  const R fin = R(0.5) * (R)q.gin * (di * di) + (R)q.bin;
+                  const R fou = R(0.5) * (R)q.gout * (dou * dou) + (R)q.bout;
+                  const R fo = ((R(1) + P_) / R(2)) * fin + ((R(1) - P_) / R(2)) * fou;
+                  // omega (Utils.jl:152)
+                  om[k] = (ph * ph * ph - ph) / eps - eps * lp;
+                  acc[0] += (double)fs;
+                  acc[1] += (double)fb;
+                  acc[2] += (double)e2;
+                  acc[3] += (double)fo;
+                  acc[4] += (double)ps * (double)ps;
+                },
+                partials, red);
+    // scalars (Utils.jl:159-190) -- red[n*5 + k]
+    DScal* dsl = ds;
+    double* A0 = dA0;
+    const double* rd = red;
+    const int Nl = N;
+    single(s, [=] __device__() {
+      double A[MAXN];
+      double s_fs = 0, s_fb = 0, s_e2 = 0, s_fo = 0, s_p2 = 0;
+      for (int n = 0; n < Nl; ++n) {
+        A[n] = integ(rd[n * 5 + 0], q, ndl);
+        s_fs += rd[n * 5 + 0];
+        s_fb += rd[n * 5 + 1];
+        s_e2 += rd[n * 5 + 2];
+        s_fo += rd[n * 5 + 3];
+        s_p2 += rd[n * 5 + 4];
+      }
+      if (set_A0)
+        for (int n = 0; n < Nl; ++n) A0[n] = A[n];
+      double W1 = 0.0;
+      for (int n = 0; n < Nl; ++n) {
+        const double d = A[n] - A0[n];
+        W1 += 0.5 * q.garea * (d * d);
+        dsl->area[n] = A[n];
+      }
+      const double Fs = q.gs * integ(s_fs, q, ndl);
+      const double Fb = q.gb * integ(s_fb, q, ndl);
+      const double E2 = integ(s_e2, q, ndl);
+      const double W2 = Fs + Fb - E2;
+      const double Wo = integ(s_fo, q, ndl);
+      const double E3 = 0.5 * q.S4 * integ(s_p2, q, ndl);
+      const double W3 = Wo - E3;
+      dsl->W1 = W1;
+      dsl->W2 = W2;
+      dsl->W3 = W3;
+      int fl = dsl->flags;
+      if (W1 + q.C1 < 0) fl |= 1;
+      if (W2 + q.C2 < 0) fl |= 2;
+      if (W3 + q.C3 < 0) fl |= 4;
+      dsl->flags = fl;
+      dsl->R1s = sqrt(W1 + q.C1);
+      dsl->R2s = sqrt(W2 + q.C2);
+      dsl->R3s = sqrt(W3 + q.C3);
+    });
+    // Laplacian of omega
+    rfft({job(om, w_omh, N)});
+    irfft({job(w_omh, w_lom, N, MUL_LAP)});
+    // H1, G, H2, H3 (Utils.jl:198-249)
+    R* H1 = w_H;
+    R* G = w_H + NP;
+    R* H2 = w_H + 2 * NP;
+    R* H3 = w_H + 3 * NP;
+    const R* lom = w_lom;
+    foreach_n(s, NP, [=] __device__(long long k) {
+      const int n = (int)(k / Pl);
+      const R ph = phi[k], ps = psi[k], w = om[k], lw = lom[k], lp = lap[k], bh = bih[k];
+      const R eps = (R)q.eps;
+      const double fac = q.garea * (dsl->area[n] - A0[n]) * (3.0 * SQ2 / 4.0);
+      H1[k] = ((R)fac * w) / (R)dsl->R1s;
+      const R sb = (R)q.gs * (R)(3.0 * SQ2 / 4.0) * w +
+                   (R)q.gb * (R)(3.0 * SQ2 / (8.0 * q.eps)) * (w * (R(3) * (ph * ph) - R(1)) / eps - eps * lw);
+      const R lin = (R)q.S1 * bh - (R)q.S2 * lp + (R)q.S3 * ph;
+      H2[k] = (sb - lin) / (R)dsl->R2s;
+      const R pp = R(-1.5) * (ph * ph) + R(1.5);
+      const R di = ps - (R)pin[n], dou = ps - (R)pout[n];
+      const R fin = R(0.5) * (R)q.gin * (di * di) + (R)q.bin;
+      const R fou = R(0.5) * (R)q.gout * (dou * dou) + (R)q.bout;
+      H3[k] = (R(0.5) * pp * (fin - fou)) / (R)dsl->R3s;
+      const R P_ = R(-0.5) * (ph * ph * ph) + R(1.5) * ph;
+      const R G1 = R(0.5) * (R(1) + P_) * ((R)q.gin * di) + R(0.5) * (R(1) - P_) * ((R)q.gout * dou);
+      const R G2 = (R)q.S4 * ps;
+      G[k] = (G1 - G2) / (R)dsl->R3s;
+    });
+  }
+
+  // ---- spectral operator helpers (device) ------------------------------------
+  struct SpecC {
+    const double *kx, *ky, *kz;
+    double S1, S2, S3, S4, Mphi, eta, M0;
+  };
+  SpecC spc() const { return SpecC{d_kx, d_ky, d_kz, c.S1, c.S2, c.S3, c.S4, c.M_phi, c.eta, c.M0_psi}; }
+
+  // ==========================================================================
+  // psi solve (2D): out = solve_psi!(rhs) -- Solvers.jl:68-81
+  // ==========================================================================
+  int psi_solve(const C* rhs, C* out, int slot) {
+    const long long NP = (long long)N * P;
+    R* rphys = w_d4 + 4 * NP;  // scratch (d4 slots 4,5 are free during solves)
+    irfft({job(rhs, rphys, N)});
+    C* b = kc.b;
+    foreach_n(s, NP, [=] __device__(long long i) { b[i] = C{rphys[i], R(0)}; });
+    fft_full_2d<R>(g, b, N, false, s);
+    const int it = bicgstab_2d<R>(kc, slot);
+    const C* x = kc.x;
+    const int Nxl = Nx, Nxhl = Nxh;
+    foreach_n(s, (long long)N * H, [=] __device__(long long i) {
+      const long long row = i / Nxhl;
+      const int ix = (int)(i % Nxhl);
+      out[i] = x[row * Nxl + ix];
+    });
+    return it;
+  }
+
+  void set_bdf(double a, double b, double cc, double dt) {
+    DScal* dsl = ds;
+    const double dt5 = nd == 3 ? c.dt : dt;
+    single(s, [=] __device__() {
+      dsl->a = a;
+      dsl->b = b;
+      dsl->c = cc;
+      dsl->dt = dt;
+      dsl->dt5 = dt5;
+      dsl->flags = 0;
+    });
+  }
+
+  // ==========================================================================
+  // Step 1 (Solvers.jl:14-137; Solvers_3d.jl:14-92)
+  // ==========================================================================
+  void step1() {
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    const R *phN = fphys(ip, F_PHI), *phO = fphys(io, F_PHI), *psN = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
+    const R *uN = uphys(ip), *uO = uphys(io);
+    R *phs = w_phis, *pss = w_psis, *M = w_M, *us = w_us;
+    const double M0 = c.M0_psi;
+    foreach_n(s, NP, [=] __device__(long long i) {
+      const R a = R(2.0) * phN[i] - phO[i];
+      phs[i] = a;
+      pss[i] = R(2.0) * psN[i] - psO[i];
+      const R t = a * a - R(1.0);
+      M[i] = R(0.1) * (R(1.0) - (R)M0 * (t * t));
+    });
+    foreach_n(s, (long long)nd * P, [=] __device__(long long i) { us[i] = R(2.0) * uN[i] - uO[i]; });
+    if (nd == 2) {
+      // M_max = maximum(M_psi) (Solvers.jl:27)
+      max_reduce(M, NP, &ds->Mmax);
+    }
+    functionals(phs, pss, false);
+    // rfft of H1, G, H2, H3 -> w_Hh (same order)
+    rfft({job(w_H, w_Hh, 4 * N)});
+    // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115)
+    const C *phhN = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
+    const C* Hh = w_Hh;
+    C* cch = cache;
+    C* rhs = w_rhs;
+    const SpecC sc = spc();
+    const KIdx ki{Nxh, Ny, Nz};
+    const DScal* dsl = ds;
+    foreach_n(s, NH, [=] __device__(long long i) {
+      int ix, iy, iz, f;
+      ki(i, ix, iy, iz, f);
+      const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      const double lapd = -k2, bihd = k2 * k2;
+      const double Lphi = sc.S1 * bihd - sc.S2 * lapd + sc.S3;
+      const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
+      const R Lp = (R)Lphi, lh = (R)lhs;
+      const R b = (R)dsl->b, cc = (R)dsl->c, dt = (R)dsl->dt;
+      const C xN = phhN[i], xO = phhO[i];
+      C t = C{-(b * xN.x + cc * xO.x) / dt, -(b * xN.y + cc * xO.y) / dt};
+      const C p11 = C{t.x / lh, t.y / lh};
+      cch[PHI11 * NH + i] = p11;
+      cch[MU11 * NH + i] = C{Lp * p11.x, Lp * p11.y};
+      const C yN = pshN[i], yO = pshO[i];
+      rhs[i] = C{-(b * yN.x + cc * yO.x) / dt, -(b * yN.y + cc * yO.y) / dt};
+      const R mM = -(R)sc.Mphi;
+      const C h1 = Hh[i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
+      const C p12 = C{(mM * h1.x) / lh, (mM * h1.y) / lh};
+      const C p13 = C{(mM * h2.x) / lh, (mM * h2.y) / lh};
+      const C p14 = C{(mM * h3.x) / lh, (mM * h3.y) / lh};
+      cch[PHI12 * NH + i] = p12;
+      cch[MU12 * NH + i] = C{Lp * p12.x + h1.x, Lp * p12.y + h1.y};
+      cch[PHI13 * NH + i] = p13;
+      cch[PHI14 * NH + i] = p14;
+      cch[MU13 * NH + i] = C{Lp * p13.x + h2.x, Lp * p13.y + h2.y};
+      cch[MU14 * NH + i] = C{Lp * p14.x + h3.x, Lp * p14.y + h3.y};
+    });
+    // grad G -> M * grad G -> rfft -> rhs12 = sum_d D1_d * (.)   (Solvers.jl:99-105)
+    const C* Gh = w_Hh + NH;
+    {
+      std::vector<SpecJob> jl;
+      for (int d = 0; d < nd; ++d) jl.push_back(job(Gh, w_d4 + d * NP, N, DX(d)));
+      irfft_v(jl);
+      R* w = w_w;
+      const R* gG = w_d4;
+      const int ndl = nd;
+      const bool var = nd == 2;
+      const R m0 = (R)c.M0_psi;
+      foreach_n(s, (long long)nd * NP, [=] __device__(long long i) {
+        const R m = var ? M[i % NP] : m0;
+        w[i] = m * gG[i];
+      });
+      (void)ndl;
+      rfft({job(w_w, w_wh, nd * N)});
+      const C* wh = w_wh;
+      C* rhs12 = w_rhs + NH;
+      const int ndd = nd;
+      foreach_n(s, NH, [=] __device__(long long i) {
+        int ix, iy, iz, f;
+        ki(i, ix, iy, iz, f);
+        const R kk[3] = {(R)sc.kx[ix], (R)sc.ky[iy], (R)sc.kz[iz]};
+        C acc = C{R(0), R(0)};
+        for (int d = 0; d < ndd; ++d) {
+          const C a = wh[d * NH + i];
+          const C t = C{-kk[d] * a.y, kk[d] * a.x};
+          acc = d == 0 ? t : acc + t;
+        }
+        rhs12[i] = acc;
+      });
+    }
+    // 21: u* . grad(phi*), u* . grad(psi*) from state spectra (Solvers.jl:117-131)
+    {
+      std::vector<SpecJob> jl;
+      for (int d = 0; d < nd; ++d) jl.push_back(job(phhN, w_d4 + d * NP, N, DX(d), phhO));
+      for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, w_d4 + (nd + d) * NP, N, DX(d), pshO));
+      irfft_v(jl);
+      R* adv = w_adv;
+      const R* gr = w_d4;
+      const int ndd = nd;
+      const long long Pl = P;
+      foreach_n(s, NP, [=] __device__(long long i) {
+        const long long sp = i % Pl;
+        R a = us[sp] * gr[i];
+        R b = us[sp] * gr[ndd * NP + i];
+        for (int d = 1; d < ndd; ++d) {
+          a = a + us[d * Pl + sp] * gr[d * NP + i];
+          b = b + us[d * Pl + sp] * gr[(ndd + d) * NP + i];
+        }
+        adv[i] = a;
+        adv[NP + i] = b;
+      });
+      rfft({job(w_adv, w_advh, 2 * N)});
+      const C* ah = w_advh;
+      C* rhs21 = w_rhs + 2 * NH;
+      foreach_n(s, NH, [=] __device__(long long i) {
+        int ix, iy, iz, f;
+        ki(i, ix, iy, iz, f);
+        const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double Lphi = sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3;
+        const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
+        const R Lp = (R)Lphi, lh = (R)lhs;
+        const C t = ah[i];
+        const C p21 = C{(-t.x) / lh, (-t.y) / lh};
+        cch[PHI21 * NH + i] = p21;
+        cch[MU21 * NH + i] = C{Lp * p21.x, Lp * p21.y};
+        const C t2 = ah[NH + i];
+        rhs21[i] = C{-t2.x, -t2.y};
+      });
+    }
+    // psi solves
+    if (nd == 2) {
+      kit[0] = psi_solve(w_rhs, cf(PSI11), 0);
+      kit[1] = psi_solve(w_rhs + NH, cf(PSI12), 1);
+      kit[2] = psi_solve(w_rhs + 2 * NH, cf(PSI21), 2);
+    } else {
+      // spectral divide by a/dt - M0*S4*Lap (Solvers_3d.jl:30,40,57,88)
+      const C* r = w_rhs;
+      C* p11 = cf(PSI11);
+      C* p12 = cf(PSI12);
+      C* p21 = cf(PSI21);
+      foreach_n(s, NH, [=] __device__(long long i) {
+        int ix, iy, iz, f;
+        ki(i, ix, iy, iz, f);
+        const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
+        const double lapd = -(kx * kx + ky * ky + kz * kz);
+        const R lh = (R)((dsl->a / dsl->dt) - sc.M0 * sc.S4 * lapd);
+        const C a = r[i], b = r[NH + i], cc = r[2 * NH + i];
+        p11[i] = C{a.x / lh, a.y / lh};
+        p12[i] = C{b.x / lh, b.y / lh};
+        p21[i] = C{cc.x / lh, cc.y / lh};
+      });
+      kit[0] = kit[1] = kit[2] = 0;
+    }
+    // nu = S4 psi (+ G_hat for nu_12)
+    {
+      C* cch2 = cache;
+      const R S4 = (R)c.S4;
+      foreach_n(s, NH, [=] __device__(long long i) {
+        const C a = cch2[PSI11 * NH + i], b = cch2[PSI12 * NH + i], d = cch2[PSI21 * NH + i];
+        const C gh = Gh[i];
+        cch2[NU11 * NH + i] = C{S4 * a.x, S4 * a.y};
+        cch2[NU12 * NH + i] = C{S4 * b.x + gh.x, S4 * b.y + gh.y};
+        cch2[NU21 * NH + i] = C{S4 * d.x, S4 * d.y};
+      });
+    }
+  }
+
+  int kit[3] = {0, 0, 0};
+
+  void max_reduce(const R* x, long long n, double* out) {
+    reduce_max_impl(x, n, out);
+  }
+  void reduce_max_impl(const R* x, long long n, double* out);
+
+  // ==========================================================================
+  // Step 2 (Solvers.jl:144-234): 22 inner products, two 3x3 LU solves
+  // ==========================================================================
+  void step2() {
+    const long long NP = (long long)N * P;
+    // physical step-1 fields in the order phi12 phi13 phi14 psi12 phi11 psi11 phi21 psi21
+    const int order[8] = {PHI12, PHI13, PHI14, PSI12, PHI11, PSI11, PHI21, PSI21};
+    std::vector<SpecJob> jl;
+    for (int k = 0; k < 8; ++k) jl.push_back(job(cf(order[k]), w_s2 + k * NP, N));
+    irfft_v(jl);
+    const R *phN = fphys(ip, F_PHI), *phO = fphys(io, F_PHI), *psN = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
+    const R *H1 = w_H, *G = w_H + NP, *H2 = w_H + 2 * NP, *H3 = w_H + 3 * NP;
+    const R* f = w_s2;
+    const DScal* dsl = ds;
+    reduce_n<22>(s, NP, 1,
+                 [=] __device__(long long i, int, double(&acc)[22]) {
+                   const R b = (R)dsl->b, cc = (R)dsl->c;
+                   const double tph = (double)(b * phN[i] + cc * phO[i]);
+                   const double tps = (double)(b * psN[i] + cc * psO[i]);
+                   const double h1 = H1[i], h2 = H2[i], h3 = H3[i], gg = G[i];
+                   acc[0] += h1 * tph;
+                   acc[1] += h2 * tph;
+                   acc[2] += h3 * tph;
+                   acc[3] += gg * tps;
+                   const double p12 = f[i], p13 = f[NP + i], p14 = f[2 * NP + i], s12 = f[3 * NP + i];
+                   const double p11 = f[4 * NP + i], s11 = f[5 * NP + i], p21 = f[6 * NP + i], s21 = f[7 * NP + i];
+                   acc[4] += h1 * p12;
+                   acc[5] += h1 * p13;
+                   acc[6] += h1 * p14;
+                   acc[7] += h2 * p12;
+                   acc[8] += h2 * p13;
+                   acc[9] += h2 * p14;
+                   acc[10] += h3 * p12;
+                   acc[11] += h3 * p13;
+                   acc[12] += h3 * p14;
+                   acc[13] += gg * s12;
+                   acc[14] += h1 * p11;
+                   acc[15] += h2 * p11;
+                   acc[16] += h3 * p11;
+                   acc[17] += gg * s11;
+                   acc[18] += h1 * p21;
+                   acc[19] += h2 * p21;
+                   acc[20] += h3 * p21;
+                   acc[21] += gg * s21;
+                 },
+                 partials, red);
+    const PhysC q = pc();
+    const int ndl = nd;
+    const double* rd = red;
+    DScal* dsw = ds;
+    const DStateScal* stN = dstate + ip;
+    const DStateScal* stO = dstate + io;
+    single(s, [=] __device__() {
+      auto idot = [&](int k) { return integ(rd[k], q, ndl); };
+      const double a = dsw->a, b = dsw->b, cc = dsw->c;
+      const double U1 = -(b * stN->R1 + cc * stO->R1) / a + idot(0) / (2.0 * a);
+      const double U2 = -(b * stN->R2 + cc * stO->R2) / a + idot(1) / (2.0 * a);
+      const double U3 = -(b * stN->R3 + cc * stO->R3) / a + (idot(2) + idot(3)) / (2.0 * a);
+      double L[9];
+      L[0] = 1.0 - 0.5 * idot(4);
+      L[1] = -0.5 * idot(5);
+      L[2] = -0.5 * idot(6);
+      L[3] = -0.5 * idot(7);
+      L[4] = 1.0 - 0.5 * idot(8);
+      L[5] = -0.5 * idot(9);
+      L[6] = -0.5 * idot(10);
+      L[7] = -0.5 * idot(11);
+      L[8] = 1.0 - 0.5 * (idot(12) + idot(13));
+      double r1[3], r2[3];
+      r1[0] = 0.5 * idot(14) + U1;
+      r1[1] = 0.5 * idot(15) + U2;
+      r1[2] = 0.5 * (idot(16) + idot(17)) + U3;
+      r2[0] = 0.5 * idot(18);
+      r2[1] = 0.5 * idot(19);
+      r2[2] = 0.5 * (idot(20) + idot(21));
+      for (int k = 0; k < 9; ++k) dsw->lhs[k] = L[k];
+      for (int k = 0; k < 3; ++k) {
+        dsw->rhs1[k] = r1[k];
+        dsw->rhs2[k] = r2[k];
+      }
+      // LU with partial pivoting (LAPACK getrf/getrs), both right-hand sides
+      int piv[3] = {0, 1, 2};
+      double A[9];
+      for (int k = 0; k < 9; ++k) A[k] = L[k];
+      for (int k = 0; k < 3; ++k) {
+        int pr = k;
+        double mx = fabs(A[k * 3 + k]);
+        for (int r = k + 1; r < 3; ++r)
+          if (fabs(A[r * 3 + k]) > mx) {
+            mx = fabs(A[r * 3 + k]);
+            pr = r;
+          }
+        if (pr != k) {
+          for (int j = 0; j < 3; ++j) {
+            double t = A[k * 3 + j];
+            A[k * 3 + j] = A[pr * 3 + j];
+            A[pr * 3 + j] = t;
+          }
+          int t = piv[k];
+          piv[k] = piv[pr];
+          piv[pr] = t;
+          double u = r1[k];
+          r1[k] = r1[pr];
+          r1[pr] = u;
+          u = r2[k];
+          r2[k] = r2[pr];
+          r2[pr] = u;
+        }
+        for (int r = k + 1; r < 3; ++r) {
+          const double l = A[r * 3 + k] / A[k * 3 + k];
+          A[r * 3 + k] = l;
+          for (int j = k + 1; j < 3; ++j) A[r * 3 + j] -= l * A[k * 3 + j];
+          r1[r] -= l * r1[k];
+          r2[r] -= l * r2[k];
+        }
+      }
+      for (int k = 2; k >= 0; --k) {
+        double a1 = r1[k], a2 = r2[k];
+        for (int j = k + 1; j < 3; ++j) {
+          a1 -= A[k * 3 + j] * r1[j];
+          a2 -= A[k * 3 + j] * r2[j];
+        }
+        r1[k] = a1 / A[k * 3 + k];
+        r2[k] = a2 / A[k * 3 + k];
+      }
+      dsw->Rs[0] = r1[0];
+      dsw->Rs[1] = r1[1];
+      dsw->Rs[2] = r1[2];
+      dsw->Rs[3] = r2[0];
+      dsw->Rs[4] = r2[1];
+      dsw->Rs[5] = r2[2];
+    });
+  }
+
+  // ==========================================================================
+  // Step 3 (Solvers.jl:242-259)
+  // ==========================================================================
+  void step3() {
+    const long long NH = (long long)N * H;
+    const C* k = cache;
+    C* o = s3;
+    const DScal* dsl = ds;
+    foreach_n(s, NH, [=] __device__(long long i) {
+      const R R11 = (R)dsl->Rs[0], R21 = (R)dsl->Rs[1], R31 = (R)dsl->Rs[2];
+      const R R12 = (R)dsl->Rs[3], R22 = (R)dsl->Rs[4], R32 = (R)dsl->Rs[5];
+      auto at = [&](int f) { return k[f * NH + i]; };
+      auto ax = [](C a, R r, C b) { return C{a.x + r * b.x, a.y + r * b.y}; };
+      o[0 * NH + i] = ax(ax(ax(at(PHI11), R11, at(PHI12)), R21, at(PHI13)), R31, at(PHI14));
+      o[1 * NH + i] = ax(ax(ax(at(MU11), R11, at(MU12)), R21, at(MU13)), R31, at(MU14));
+      o[2 * NH + i] = ax(at(PSI11), R31, at(PSI12));
+      o[3 * NH + i] = ax(at(NU11), R31, at(NU12));
+      o[4 * NH + i] = ax(ax(ax(at(PHI21), R12, at(PHI12)), R22, at(PHI13)), R32, at(PHI14));
+      o[5 * NH + i] = ax(ax(ax(at(MU21), R12, at(MU12)), R22, at(MU13)), R32, at(MU14));
+      o[6 * NH + i] = ax(at(PSI21), R32, at(PSI12));
+      o[7 * NH + i] = ax(at(NU21), R32, at(NU12));
+    });
+  }
+
+  // ==========================================================================
+  // Step 4 (Solvers.jl:267-337; Solvers_3d.jl:221-292)
+  // ==========================================================================
+  void step4() {
+    const long long NP = (long long)N * P;
+    const C *uhN = uhalf(ip), *uhO = uhalf(io);
+    // du_c/dx_d -> w_du[c*nd + d]
+    {
+      std::vector<SpecJob> jl;
+      for (int cc = 0; cc < nd; ++cc)
+        for (int d = 0; d < nd; ++d) jl.push_back(job(uhN + cc * H, w_du + (cc * nd + d) * P, 1, DX(d), uhO + cc * H));
+      for (int d = 0; d < nd; ++d)
+        jl.push_back(job(fhalf(ip, F_MU), w_dmn + d * NP, N, DX(d), fhalf(io, F_MU)));
+      for (int d = 0; d < nd; ++d)
+        jl.push_back(job(fhalf(ip, F_NU), w_dmn + (nd + d) * NP, N, DX(d), fhalf(io, F_NU)));
+      irfft_v(jl);
+    }
+    const R *us = w_us, *du = w_du, *dmn = w_dmn, *phs = w_phis, *pss = w_psis;
+    R *conv = w_conv, *coupl = w_coupl, *rhs2 = w_rhs2;
+    const int ndd = nd, Nl = N;
+    const long long Pl = P;
+    const R lam = (R)c.lamda;
+    foreach_n(s, P, [=] __device__(long long i) {
+      for (int cc = 0; cc < ndd; ++cc) {
+        R a = us[i] * du[(cc * ndd) * Pl + i];
+        for (int d = 1; d < ndd; ++d) a = a + us[d * Pl + i] * du[(cc * ndd + d) * Pl + i];
+        conv[cc * Pl + i] = a;
+      }
+      for (int d = 0; d < ndd; ++d) {
+        R sum = R(0);
+        for (int n = 0; n < Nl; ++n) {
+          const long long k = (long long)n * Pl + i;
+          const R t = phs[k] * dmn[d * NP + k] + pss[k] * dmn[(ndd + d) * NP + k];
+          sum = n == 0 ? t : sum + t;
+        }
+        coupl[d * Pl + i] = sum;
+        rhs2[d * Pl + i] = -conv[d * Pl + i] - lam * sum;
+      }
+    });
+    rfft({job(w_rhs2, w_rhs2h, nd)});
+    const C* r2h = w_rhs2h;
+    const C* phat = phalf(ip);
+    C* uth = w_uth;  // [ut1h (nd) | ut2h (nd)]
+    const SpecC sc = spc();
+    const KIdx ki{Nxh, Ny, Nz};
+    const DScal* dsl = ds;
+    const long long Hl = H;
+    foreach_n(s, (long long)nd * H, [=] __device__(long long i) {
+      int ix, iy, iz, d;
+      ki(i, ix, iy, iz, d);
+      const long long sp = i % Hl;
+      const double kk[3] = {sc.kx[ix], sc.ky[iy], sc.kz[iz]};
+      const double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+      const R lh = (R)((dsl->a / dsl->dt) - sc.eta * (-k2));
+      const R b = (R)dsl->b, cc = (R)dsl->c, dt = (R)dsl->dt;
+      const C xN = uhN[i], xO = uhO[i];
+      const C pp = phat[sp];
+      const R kd = (R)kk[d];
+      const C dp = C{-kd * pp.y, kd * pp.x};
+      const C r1 = C{-(b * xN.x + cc * xO.x) / dt - dp.x, -(b * xN.y + cc * xO.y) / dt - dp.y};
+      uth[i] = C{r1.x / lh, r1.y / lh};
+      const C r2 = r2h[i];
+      uth[ndd * Hl + i] = C{r2.x / lh, r2.y / lh};
+    });
+    irfft({job(w_uth, w_ut12, 2 * nd)});
+  }
+
+  // ==========================================================================
+  // Step 5 (Solvers.jl:344-423; Solvers_3d.jl:299-392)
+  // ==========================================================================
+  void step5() {
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    irfft({job(s3 + 1 * NH, w_s5, N), job(s3 + 3 * NH, w_s5 + NP, N), job(s3 + 5 * NH, w_s5 + 2 * NP, N),
+           job(s3 + 7 * NH, w_s5 + 3 * NP, N)});
+    const R *adv = w_adv, *f5 = w_s5, *conv = w_conv, *coupl = w_coupl, *ut = w_ut12;
+    const long long Pl = P;
+    const int ndd = nd;
+    reduce_n<16>(s, NP, 1,
+                 [=] __device__(long long i, int, double(&acc)[16]) {
+                   const double ap = adv[i], as = adv[NP + i];
+                   acc[0] += ap * (double)f5[i];           // t11
+                   acc[1] += as * (double)f5[NP + i];      // t12
+                   acc[2] += ap * (double)f5[2 * NP + i];  // t21
+                   acc[3] += as * (double)f5[3 * NP + i];  // t22
+                   if (i < Pl) {
+                     for (int d = 0; d < ndd; ++d) {
+                       const double u1 = ut[d * Pl + i], u2 = ut[(ndd + d) * Pl + i];
+                       const double cp = coupl[d * Pl + i], cv = conv[d * Pl + i];
+                       acc[4 + d] += cp * u1;   // t13 parts
+                       acc[7 + d] += cv * u1;   // t14 parts
+                       acc[10 + d] += cp * u2;  // t23 parts
+                       acc[13 + d] += cv * u2;  // t24 parts
+                     }
+                   }
+                 },
+                 partials, red);
+    const PhysC q = pc();
+    const double* rd = red;
+    DScal* dsw = ds;
+    const DStateScal* stN = dstate + ip;
+    const DStateScal* stO = dstate + io;
+    const double lam = (double)c.lamda;
+    single(s, [=] __device__() {
+      double t13 = rd[4], t14 = rd[7], t23 = rd[10], t24 = rd[13];
+      for (int d = 1; d < ndd; ++d) {
+        t13 += rd[4 + d];
+        t14 += rd[7 + d];
+        t23 += rd[10 + d];
+        t24 += rd[13 + d];
+      }
+      double dvol = q.dx * q.dy;
+      if (ndd == 3) dvol = dvol * q.dz;
+      const double th1 = dvol * (lam * (rd[0] + rd[1] + t13) + t14);
+      const double th2 = dvol * (lam * (rd[2] + rd[3] + t23) + t24);
+      const double a = dsw->a, b = dsw->b, cc = dsw->c, dt = dsw->dt5;
+      const double rhsQ = -(b * stN->Q + cc * stO->Q) / dt + th1;
+      const double lhsQ = (a / dt) - th2;
+      dsw->theta1 = th1;
+      dsw->theta2 = th2;
+      dsw->Q = rhsQ / lhsQ;
+      if (ndd == 3 && !(th2 < 0)) dsw->flags |= 8;
+    });
+  }
+
+  // ==========================================================================
+  // Steps 6 and 7 (Solvers.jl:431-498) written into the next state buffer
+  // ==========================================================================
+  void step67() {
+    const long long NH = (long long)N * H;
+    const C* o = s3;
+    C *phN = fhalf(inew, F_PHI), *psN = fhalf(inew, F_PSI), *muN = fhalf(inew, F_MU), *nuN = fhalf(inew, F_NU);
+    C* uN = uhalf(inew);
+    C* pN = phalf(inew);
+    const C* pOld = phalf(ip);
+    const C* uth = w_uth;
+    const SpecC sc = spc();
+    const KIdx ki{Nxh, Ny, Nz};
+    const DScal* dsl = ds;
+    const int ndd = nd, Nl = N;
+    const long long Hl = H;
+    foreach_n(s, H, [=] __device__(long long i) {
+      const R Q = (R)dsl->Q;
+      for (int n = 0; n < Nl; ++n) {
+        const long long k = (long long)n * Hl + i;
+        auto cmb = [&](int f) {
+          const C a = o[f * NH + k], b = o[(f + 4) * NH + k];
+          return C{a.x + Q * b.x, a.y + Q * b.y};
+        };
+        phN[k] = cmb(0);
+        muN[k] = cmb(1);
+        psN[k] = cmb(2);
+        nuN[k] = cmb(3);
+      }
+      int ix, iy, iz, f;
+      ki(i, ix, iy, iz, f);
+      const double kk[3] = {sc.kx[ix], sc.ky[iy], sc.kz[iz]};
+      C ut[3];
+      C div = C{R(0), R(0)};
+      for (int d = 0; d < ndd; ++d) {
+        const C a = uth[d * Hl + i], b = uth[(ndd + d) * Hl + i];
+        ut[d] = C{a.x + Q * b.x, a.y + Q * b.y};
+        const R kd = (R)kk[d];
+        const C t = C{-kd * ut[d].y, kd * ut[d].x};
+        div = d == 0 ? t : div + t;
+      }
+      const double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+      const R lapv = (R)(-k2);
+      const R fac = (R)(dsl->a / dsl->dt);
+      C dp = C{(fac * div.x) / lapv, (fac * div.y) / lapv};
+      if (i == 0) dp = C{R(0), R(0)};
+      const double dta = dsl->dt / dsl->a;
+      for (int d = 0; d < ndd; ++d) {
+        const R kd = (R)(dta * kk[d]);
+        uN[d * Hl + i] = C{ut[d].x - (-kd * dp.y), ut[d].y - kd * dp.x};
+      }
+      const R eta = (R)sc.eta;
+      const C po = pOld[i];
+      pN[i] = C{(po.x + dp.x) - eta * div.x, (po.y + dp.y) - eta * div.y};
+    });
+    DStateScal* stw = dstate + inew;
+    DScal* dsw = ds;
+    single(s, [=] __device__() {
+      const double Q = dsw->Q;
+      stw->Q = Q;
+      stw->R1 = dsw->Rs[0] + Q * dsw->Rs[3];
+      stw->R2 = dsw->Rs[1] + Q * dsw->Rs[4];
+      stw->R3 = dsw->Rs[2] + Q * dsw->Rs[5];
+      if (!isfinite(Q) || !isfinite(stw->R1) || !isfinite(stw->R2) || !isfinite(stw->R3)) dsw->flags |= 16;
+    });
+  }
+
+  // ==========================================================================
+  // Monitor: compute_modified_energy (Utils.jl:96-141) and the area ratio
+  // (simulation.jl:147) for the present/old pair.
+  // ==========================================================================
+  void monitor() {
+    const long long NP = (long long)N * P;
+    const C *phh = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *ph = phalf(ip);
+    R* m = w_mon;
+    const bool s2 = c.S2 != 0.0;
+    std::vector<SpecJob> jl = {job(phh, m, N, MUL_LAP), job(phh, m + NP, N, MUL_LAP, phhO),
+                               job(ph, m + 6 * NP, 1, MUL_DX), job(ph, m + 6 * NP + P, 1, MUL_DY)};
+    if (s2) {
+      jl.push_back(job(phh, m + 2 * NP, N, MUL_DX));
+      jl.push_back(job(phh, m + 3 * NP, N, MUL_DY));
+      jl.push_back(job(phh, m + 4 * NP, N, MUL_DX, phhO));
+      jl.push_back(job(phh, m + 5 * NP, N, MUL_DY, phhO));
+    }
+    // area: f_surf of rfft(phi) (calculate_area)
+    const R* phi = fphys(ip, F_PHI);
+    rfft({job(phi, w_phat, N)});
+    jl.push_back(job(w_phat, w_gxy, N, MUL_DX));
+    jl.push_back(job(w_phat, w_gxy + NP, N, MUL_DY));
+    irfft_v(jl);
+    const PhysC q = pc();
+    const R* gxy = w_gxy;
+    const long long Pl = P;
+    double* redA = red + 256;
+    reduce_n<1>(s, P, N,
+                [=] __device__(long long i, int n, double(&acc)[1]) {
+                  const long long k = (long long)n * Pl + i;
+                  const R p = phi[k], gx = gxy[k], gy = gxy[NP + k];
+                  const R eps = (R)q.eps;
+                  const R gg = R(0.25) * (p * p - R(1)) * (p * p - R(1));
+                  const R fs = (R)(3.0 * SQ2 / 4.0) * (gg / eps) + eps * (gx * gx + gy * gy) / R(2);
+                  acc[0] += (double)fs;
+                },
+                partials, redA);
+    const R *u = uphys(ip), *uo = uphys(io), *phO = fphys(io, F_PHI), *ps = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
+    const long long nu = (long long)nd * P;
+    reduce_n<14>(s, NP > nu ? NP : nu, 1,
+                 [=] __device__(long long i, int, double(&acc)[14]) {
+                   if (i < nu) {
+                     const double a = u[i], b = uo[i];
+                     const double st = (double)(R(2) * u[i] - uo[i]);
+                     acc[0] += a * a;
+                     acc[1] += st * st;
+                     (void)b;
+                   }
+                   if (i < Pl) {
+                     const double gx = m[6 * NP + i], gy = m[6 * NP + Pl + i];
+                     acc[2] += gx * gx;
+                     acc[3] += gy * gy;
+                   }
+                   if (i < NP) {
+                     const double l0 = m[i], l1 = m[NP + i];
+                     acc[4] += l0 * l0;
+                     acc[5] += l1 * l1;
+                     if (s2) {
+                       const double a = m[2 * NP + i], b = m[3 * NP + i], cc = m[4 * NP + i], d = m[5 * NP + i];
+                       acc[6] += a * a;
+                       acc[7] += b * b;
+                       acc[8] += cc * cc;
+                       acc[9] += d * d;
+                     }
+                     const double p = phi[i], pst = (double)(R(2) * phi[i] - phO[i]);
+                     const double y = ps[i], yst = (double)(R(2) * ps[i] - psO[i]);
+                     acc[10] += p * p;
+                     acc[11] += pst * pst;
+                     acc[12] += y * y;
+                     acc[13] += yst * yst;
+                   }
+                 },
+                 partials, red);
+    const double* rd = red;
+    DScal* dsw = ds;
+    const DStateScal* stN = dstate + ip;
+    const DStateScal* stO = dstate + io;
+    const double lam = (double)c.lamda;
+    const double* A0 = dA0;
+    const int ndl = nd, Nl = N;
+    single(s, [=] __device__() {
+      auto isq = [&](int k) { return integ(rd[k], q, ndl); };
+      const double kinetic = 0.25 * isq(0) + 0.25 * isq(1);
+      const double dt = dsw->dt;
+      const double pressure = (dt * dt / 3.0) * (isq(2) + isq(3));
+      auto se = [](double rn, double ro) { return 0.5 * rn * rn + 0.5 * (2 * rn - ro) * (2 * rn - ro); };
+      const double Re = se(stN->R1, stO->R1) + se(stN->R2, stO->R2) + se(stN->R3, stO->R3);
+      const double Qe = 0.5 * se(stN->Q, stO->Q);
+      const double S1t = 0.25 * q.S1 * (isq(4) + isq(5));
+      const double S2t = 0.25 * q.S2 * (isq(6) + isq(7) + isq(8) + isq(9));
+      const double S3t = 0.25 * q.S3 * (isq(10) + isq(11));
+      const double S4t = 0.25 * q.S4 * (isq(12) + isq(13));
+      dsw->energy = (kinetic + pressure + lam * Re + Qe + lam * (S1t + S2t + S3t + S4t) -
+                     lam * (q.C1 + q.C2 + q.C3) - 0.5);
+      double num = 0.0, den = 0.0;
+      for (int n = 0; n < Nl; ++n) {
+        const double A = integ(redA[n], q, ndl);
+        dsw->area_mon[n] = A;
+        num += fabs(A - A0[n]);
+        den += A0[n];
+      }
+      dsw->area_ratio = num / den;
+    });
+  }
+
+  // ==========================================================================
+  // ISolver
+  // ==========================================================================
+  void set_state(int which, const double* const* a, const double* scal, const double* A0, double dt) override {
+    const int b = which == SAV_PRESENT ? ip : io;
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    // order: phi, phi_hat, psi, psi_hat, u, u_hat, p, p_hat, mu, mu_hat, nu, nu_hat
+    put_real(a[0], fphys(b, F_PHI), NP);
+    put_cplx(a[1], fhalf(b, F_PHI), NH);
+    put_real(a[2], fphys(b, F_PSI), NP);
+    put_cplx(a[3], fhalf(b, F_PSI), NH);
+    put_real(a[4], uphys(b), nd * P);
+    put_cplx(a[5], uhalf(b), nd * H);
+    put_real(a[6], pphys(b), P);
+    put_cplx(a[7], phalf(b), H);
+    put_real(a[8], fphys(b, F_MU), NP);
+    put_cplx(a[9], fhalf(b, F_MU), NH);
+    put_real(a[10], fphys(b, F_NU), NP);
+    put_cplx(a[11], fhalf(b, F_NU), NH);
+    DStateScal st{scal[0], scal[1], scal[2], scal[3]};
+    SAV_HIP(hipMemcpyAsync(dstate + b, &st, sizeof(st), hipMemcpyHostToDevice, s));
+    if (which == SAV_PRESENT && A0) {
+      SAV_HIP(hipMemcpyAsync(dA0, A0, sizeof(double) * N, hipMemcpyHostToDevice, s));
+      present_dt = dt;
+    }
+    SAV_HIP(hipStreamSynchronize(s));
+  }
+  double present_dt = 0.0;
+
+  void get_state(int which, double* const* a, double* scal, double* A0, double* dt) override {
+    const int b = which == SAV_PRESENT ? ip : io;
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    get_real(a[0], fphys(b, F_PHI), NP);
+    get_cplx(a[1], fhalf(b, F_PHI), NH);
+    get_real(a[2], fphys(b, F_PSI), NP);
+    get_cplx(a[3], fhalf(b, F_PSI), NH);
+    get_real(a[4], uphys(b), nd * P);
+    get_cplx(a[5], uhalf(b), nd * H);
+    get_real(a[6], pphys(b), P);
+    get_cplx(a[7], phalf(b), H);
+    get_real(a[8], fphys(b, F_MU), NP);
+    get_cplx(a[9], fhalf(b, F_MU), NH);
+    get_real(a[10], fphys(b, F_NU), NP);
+    get_cplx(a[11], fhalf(b, F_NU), NH);
+    if (scal) {
+      DStateScal st;
+      SAV_HIP(hipMemcpyAsync(&st, dstate + b, sizeof(st), hipMemcpyDeviceToHost, s));
+      SAV_HIP(hipStreamSynchronize(s));
+      scal[0] = st.Q;
+      scal[1] = st.R1;
+      scal[2] = st.R2;
+      scal[3] = st.R3;
+    }
+    if (A0) {
+      SAV_HIP(hipMemcpyAsync(A0, dA0, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+      SAV_HIP(hipStreamSynchronize(s));
+    }
+    if (dt) *dt = present_dt;
+  }
+
+  // host <-> device with precision conversion
+  void put_real(const double* h, R* d, long long n) {
+    if (!h) return;
+    if constexpr (sizeof(R) == 8) {
+      SAV_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, s));
+      SAV_HIP(hipStreamSynchronize(s));
+    } else {
+      std::vector<R> t(n);
+      for (long long i = 0; i < n; ++i) t[i] = (R)h[i];
+      SAV_HIP(hipMemcpyAsync(d, t.data(), n * sizeof(R), hipMemcpyHostToDevice, s));
+      SAV_HIP(hipStreamSynchronize(s));
+    }
+  }
+  void put_cplx(const double* h, C* d, long long n) { put_real(h, reinterpret_cast<R*>(d), 2 * n); }
+  void get_real(double* h, const R* d, long long n) {
+    if (!h) return;
+    if constexpr (sizeof(R) == 8) {
+      SAV_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      SAV_HIP(hipStreamSynchronize(s));
+    } else {
+      std::vector<R> t(n);
+      SAV_HIP(hipMemcpyAsync(t.data(), d, n * sizeof(R), hipMemcpyDeviceToHost, s));
+      SAV_HIP(hipStreamSynchronize(s));
+      for (long long i = 0; i < n; ++i) h[i] = (double)t[i];
+    }
+  }
+  void get_cplx(double* h, const C* d, long long n) { get_real(h, reinterpret_cast<const R*>(d), 2 * n); }
+
+  int check_flags() {
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    if (hs->flags & 7) return SAV_E_WC_NEGATIVE;
+    return 0;
+  }
+
+  int init_sav() override {
+    const long long NH = (long long)N * H, NP = (long long)N * P;
+    set_bdf(1.0, -1.0, 0.0, present_dt > 0 ? present_dt : c.dt);
+    functionals(fphys(ip, F_PHI), fphys(ip, F_PSI), true);
+    int rc = check_flags();
+    if (rc) return rc;
+    rfft({job(w_H, w_Hh, 4 * N)});
+    // mu_hat = L_phi phi_hat + R1 H1^ + R2 H2^ + R3 H3^ ; nu_hat = S4 psi_hat + R3 G^ (simulation.jl:35-47)
+    const C* Hh = w_Hh;
+    C* muh = fhalf(ip, F_MU);
+    C* nuh = fhalf(ip, F_NU);
+    const C* phh = fhalf(ip, F_PHI);
+    const C* psh = fhalf(ip, F_PSI);
+    const SpecC sc = spc();
+    const KIdx ki{Nxh, Ny, Nz};
+    const DScal* dsl = ds;
+    foreach_n(s, NH, [=] __device__(long long i) {
+      int ix, iy, iz, f;
+      ki(i, ix, iy, iz, f);
+      const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      const R Lp = (R)(sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3);
+      const R r1 = (R)dsl->R1s, r2 = (R)dsl->R2s, r3 = (R)dsl->R3s;
+      const C p = phh[i], h1 = Hh[i], g = Hh[NH + i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
+      muh[i] = C{Lp * p.x + r1 * h1.x + r2 * h2.x + r3 * h3.x, Lp * p.y + r1 * h1.y + r2 * h2.y + r3 * h3.y};
+      const C y = psh[i];
+      const R S4 = (R)sc.S4;
+      nuh[i] = C{S4 * y.x + r3 * g.x, S4 * y.y + r3 * g.y};
+    });
+    irfft({job(muh, fphys(ip, F_MU), N), job(nuh, fphys(ip, F_NU), N)});
+    DStateScal* stw = dstate + ip;
+    DScal* dsw = ds;
+    single(s, [=] __device__() {
+      stw->Q = 1.0;
+      stw->R1 = dsw->R1s;
+      stw->R2 = dsw->R2s;
+      stw->R3 = dsw->R3s;
+    });
+    // old = deepcopy(present)
+    const long long nphys = (4LL * N + nd + 1) * P, nhalf = (4LL * N + nd + 1) * H;
+    SAV_HIP(hipMemcpyAsync(sphys[io], sphys[ip], nphys * sizeof(R), hipMemcpyDeviceToDevice, s));
+    SAV_HIP(hipMemcpyAsync(shalf[io], shalf[ip], nhalf * sizeof(C), hipMemcpyDeviceToDevice, s));
+    SAV_HIP(hipMemcpyAsync(dstate + io, dstate + ip, sizeof(DStateScal), hipMemcpyDeviceToDevice, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    (void)NP;
+    return 0;
+  }
+
+  int step_impl(double dt, int n, sav_step_stats_t* st, bool monitor_on) {
+    const double a = n == 1 ? 1.0 : 1.5, b = n == 1 ? -1.0 : -2.0, cc = n == 1 ? 0.0 : 0.5;
+    present_dt = dt;
+    set_bdf(a, b, cc, dt);
+    step1();
+    int rc = 0;
+    step2();
+    step3();
+    step4();
+    step5();
+    step67();
+    // fatal W + C < 0 or non-finite: the step is not committed
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    if (hs->flags & 7) return SAV_E_WC_NEGATIVE;
+    if (hs->flags & 16) return SAV_E_NONFINITE;
+    // rotate: old <- present, present <- new
+    const int oldp = ip, oldo = io;
+    ip = inew;
+    io = oldp;
+    inew = oldo;
+    irfft({job(shalf[ip], sphys[ip], 4 * N + nd + 1)});
+    if (monitor_on) monitor();
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    bool notconv = false;
+    if (st) {
+      std::memset(st, 0, sizeof(*st));
+      for (int k = 0; k < 3; ++k) {
+        st->krylov_iters[k] = nd == 2 ? hks[k].iter : 0;
+        st->krylov_solved[k] = nd == 2 ? hks[k].solved : 1;
+        st->krylov_rnorm[k] = nd == 2 ? hks[k].rnorm : 0.0;
+      }
+      st->energy = hs->energy;
+      st->area_ratio = hs->area_ratio;
+      st->Q = hs->Q;
+      DStateScal sc;
+      SAV_HIP(hipMemcpy(&sc, dstate + ip, sizeof(sc), hipMemcpyDeviceToHost));
+      st->R1 = sc.R1;
+      st->R2 = sc.R2;
+      st->R3 = sc.R3;
+    }
+    if (nd == 2)
+      for (int k = 0; k < 3; ++k)
+        if (!hks[k].solved) notconv = true;
+    if (notconv) rc = SAV_W_KRYLOV_NOT_CONVERGED;
+    else if (hs->flags & 8) rc = SAV_W_THETA2_NONNEG;
+    if (st) st->status = rc;
+    return rc;
+  }
+
+  int step(double dt, int n, sav_step_stats_t* st) override {
+    SAV_HIP(hipEventRecord(ev0, s));
+    int rc = step_impl(dt, n, st, true);
+    SAV_HIP(hipEventRecord(ev1, s));
+    SAV_HIP(hipEventSynchronize(ev1));
+    SAV_HIP(hipEventElapsedTime(&last_ms, ev0, ev1));
+    return rc;
+  }
+
+  int run(const double* t, int nn, int n0, double* eh, double* ah, int32_t* kit_total) override {
+    int worst = 0;
+    long long ktot = 0;
+    SAV_HIP(hipEventRecord(ev0, s));
+    for (int k = 0; k + 1 < nn; ++k) {
+      sav_step_stats_t st;
+      const int rc = step_impl(t[k + 1] - t[k], n0 + k, &st, true);
+      if (rc < 0) return rc;
+      if (rc > worst) worst = rc;
+      if (eh) eh[k] = st.energy;
+      if (ah) ah[k] = st.area_ratio;
+      ktot += st.krylov_iters[0] + st.krylov_iters[1] + st.krylov_iters[2];
+    }
+    SAV_HIP(hipEventRecord(ev1, s));
+    SAV_HIP(hipEventSynchronize(ev1));
+    SAV_HIP(hipEventElapsedTime(&last_ms, ev0, ev1));
+    if (kit_total) *kit_total = (int32_t)ktot;
+    return worst;
+  }
+
+  double last_elapsed_ms() override { return last_ms; }
+
+  // Event timing of one psi-matvec kernel over the following steps.
+  std::vector<hipEvent_t> tev;
+  void kernel_timing(int id) override {
+    if (nd != 2) throw HipError("kernel timing covers the 2D psi matvec kernels", SAV_E_UNSUPPORTED);
+    if (tev.empty()) {
+      tev.resize(2 * 256);
+      for (auto& e : tev) SAV_HIP(hipEventCreate(&e));
+      kc.ev = tev.data();
+      kc.ev_cap = 256;
+    }
+    kc.timed = id;
+    kc.ev_used = 0;
+    kc.tk_count = 0;
+    kc.tk_ms = 0.0;
+    kc.tk_bytes = 0.0;
+  }
+  void kernel_timing_get(long long* n, double* ms, double* bytes) override {
+    if (n) *n = kc.tk_count;
+    if (ms) *ms = kc.tk_ms;
+    if (bytes) *bytes = kc.tk_count ? kc.tk_bytes / (double)kc.tk_count : 0.0;
+  }
+
+  void rfft_host(const double* in, double* out, int count) override {
+    R* dphys = dalloc_tmp_phys(count);
+    C* dhalf = dalloc_tmp_half(count);
+    put_real(in, dphys, count * P);
+    rfft({job(dphys, dhalf, count)});
+    get_cplx(out, dhalf, count * H);
+  }
+  void irfft_host(const double* in, double* out, int count) override {
+    R* dphys = dalloc_tmp_phys(count);
+    C* dhalf = dalloc_tmp_half(count);
+    put_cplx(in, dhalf, count * H);
+    irfft({job(dhalf, dphys, count)});
+    get_real(out, dphys, count * P);
+  }
+  void fft_full_host(const double* in, double* out, int count, int inverse) override {
+    if (nd != 2) throw HipError("full complex transforms are 2D only", SAV_E_UNSUPPORTED);
+    C* d = reinterpret_cast<C*>(dalloc_tmp_phys(4 * count));  // 2*count*P complex -> 4*count*P reals
+    put_cplx(in, d, count * P);
+    fft_full_2d<R>(g, d, count, inverse != 0, s);
+    get_cplx(out, d, count * P);
+  }
+  R* tphys = nullptr;
+  long long tphys_n = 0;
+  C* thalf = nullptr;
+  long long thalf_n = 0;
+  R* dalloc_tmp_phys(int count) {
+    if ((long long)count * P > tphys_n) {
+      tphys_n = (long long)count * P;
+      tphys = dalloc<R>(tphys_n);
+    }
+    return tphys;
+  }
+  C* dalloc_tmp_half(int count) {
+    if ((long long)count * H > thalf_n) {
+      thalf_n = (long long)count * H;
+      thalf = dalloc<C>(thalf_n);
+    }
+    return thalf;
+  }
+
+  int eval_functionals(const double* phi, const double* psi, double* H1, double* H2, double* H3, double* G,
+                       double* W, double* area) override {
+    const long long NP = (long long)N * P;
+    put_real(phi, w_phis, NP);
+    put_real(psi, w_psis, NP);
+    SAV_HIP(hipMemsetAsync(&ds->flags, 0, sizeof(int), s));
+    functionals(w_phis, w_psis, false);
+    int rc = check_flags();
+    get_real(H1, w_H, NP);
+    get_real(G, w_H + NP, NP);
+    get_real(H2, w_H + 2 * NP, NP);
+    get_real(H3, w_H + 3 * NP, NP);
+    if (W) {
+      W[0] = hs->W1;
+      W[1] = hs->W2;
+      W[2] = hs->W3;
+    }
+    if (area)
+      for (int n = 0; n < N; ++n) area[n] = hs->area[n];
+    return rc;
+  }
+
+  int psi_solve_host(const double* phi_star, double a, double dt, const double* rhs, double* out, int32_t* iters,
+                     int32_t* solved, double* rnorm) override {
+    if (nd != 2) return SAV_E_UNSUPPORTED;
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    put_real(phi_star, w_phis, NP);
+    set_bdf(a, 0.0, 0.0, dt);
+    R* M = w_M;
+    const R* phs = w_phis;
+    const double M0 = c.M0_psi;
+    foreach_n(s, NP, [=] __device__(long long i) {
+      const R t = phs[i] * phs[i] - R(1.0);
+      M[i] = R(0.1) * (R(1.0) - (R)M0 * (t * t));
+    });
+    max_reduce(M, NP, &ds->Mmax);
+    C* dr = w_rhs;
+    put_cplx(rhs, dr, NH);
+    C* dout = w_rhs + NH;
+    int it = psi_solve(dr, dout, 0);
+    get_cplx(out, dout, NH);
+    if (iters) *iters = it;
+    if (solved) *solved = hks[0].solved;
+    if (rnorm) *rnorm = hks[0].rnorm;
+    return hks[0].solved ? 0 : SAV_W_KRYLOV_NOT_CONVERGED;
+  }
+
+  int modified_energy(double* E, double* ar) override {
+    monitor();
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    if (E) *E = hs->energy;
+    if (ar) *ar = hs->area_ratio;
+    return 0;
+  }
+};
+
+// Global maximum of a field into a device double.
+template <typename R>
+__global__ __launch_bounds__(256) void k_max_stage(const R* __restrict__ x, long long n, double* __restrict__ partials) {
+  __shared__ double sc[4];
+  double m = -INFINITY;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    m = fmax(m, (double)x[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(sc[0], sc[1]), fmax(sc[2], sc[3]));
+}
+__global__ __launch_bounds__(256) void k_max_fin(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+  __shared__ double sc[4];
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < nb; i += 256) m = fmax(m, partials[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(sc[0], sc[1]), fmax(sc[2], sc[3]));
+}
+
+template <typename R>
+void Solver<R>::reduce_max_impl(const R* x, long long n, double* out) {
+  long long nb = (n + 1023) / 1024;
+  if (nb < 1) nb = 1;
+  if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
+  hipLaunchKernelGGL((k_max_stage<R>), dim3((unsigned)nb), dim3(256), 0, s, x, n, partials);
+  hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(256), 0, s, partials, (int)nb, out);
+  SAV_LAUNCH_CHECK();
+}
+
+std::unique_ptr<ISolver> make_solver(const HostCfg& cfg) {
+  if (cfg.c.precision == SAV_PREC_FP32) return std::unique_ptr<ISolver>(new Solver<float>(cfg));
+  return std::unique_ptr<ISolver>(new Solver<double>(cfg));
+}
+
+}  // namespace sav

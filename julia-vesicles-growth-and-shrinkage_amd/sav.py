@@ -1,0 +1,495 @@
+"""Host-side mirror of the reference's operator interface over libsav_hip.so.
+
+Names follow the reference: ``Config`` (src/Types.jl:27), ``FieldState``
+(src/Types.jl:167), ``build_operators`` (src/SpectralUtils.jl:36),
+``to_spectral!`` / ``to_physical!`` (src/SpectralUtils.jl:132-148),
+``generate_initial_condition`` (src/Init.jl:74), ``run_simulation``
+(src/simulation.jl:2).  Arrays use the Julia shapes and Fortran (column-major)
+order, so ``state.phi.shape == (Nx, Ny, N)`` and the memory handed to the C ABI
+is byte-identical to what Julia's ``ccall`` would pass.
+
+Everything numeric runs in the HIP library; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import List, Optional
+
+import numpy as np
+
+from .config import Config, bdf_coeff, set_para_base, SplitMix64
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsav_hip.so"
+
+SAV_OK = 0
+SAV_W_KRYLOV_NOT_CONVERGED = 1
+SAV_W_THETA2_NONNEG = 2
+SAV_E_ARG = -1
+SAV_E_HIP = -2
+SAV_E_WC_NEGATIVE = -3
+SAV_E_NONFINITE = -4
+SAV_E_UNSUPPORTED = -5
+SAV_PREC_FP64 = 0
+SAV_PREC_FP32 = 1
+SAV_PRECOND_DIAG = 0
+SAV_PRECOND_LEARNED = 1
+SAV_PRESENT = 0
+SAV_OLD = 1
+
+_dp = ct.POINTER(ct.c_double)
+
+
+class SavConfigC(ct.Structure):
+    _fields_ = [("ndim", ct.c_int32), ("N", ct.c_int32), ("Nx", ct.c_int32), ("Ny", ct.c_int32),
+                ("Nz", ct.c_int32), ("Lx", ct.c_double), ("Ly", ct.c_double), ("Lz", ct.c_double),
+                ("epsilon", ct.c_double), ("M_phi", ct.c_double), ("M0_psi", ct.c_double),
+                ("eta", ct.c_double), ("gamma_surf", ct.c_double), ("gamma_area", ct.c_int64),
+                ("gamma_bend", ct.c_double), ("gamma_in", ct.c_int64), ("beta_in", ct.c_int64),
+                ("gamma_out", ct.c_int64), ("beta_out", ct.c_int64), ("lamda", ct.c_int64),
+                ("psi_in_v", _dp), ("psi_out_v", _dp),
+                ("S1", ct.c_double), ("S2", ct.c_double), ("S3", ct.c_double), ("S4", ct.c_double),
+                ("C1", ct.c_double), ("C2", ct.c_double), ("C3", ct.c_double),
+                ("dt", ct.c_double), ("T", ct.c_double), ("tol", ct.c_double),
+                ("precision", ct.c_int32), ("precond", ct.c_int32), ("itmax", ct.c_int32),
+                ("atol", ct.c_double), ("device", ct.c_int32)]
+
+
+class SavStepStatsC(ct.Structure):
+    _fields_ = [("krylov_iters", ct.c_int32 * 3), ("krylov_solved", ct.c_int32 * 3),
+                ("krylov_rnorm", ct.c_double * 3), ("energy", ct.c_double), ("area_ratio", ct.c_double),
+                ("Q", ct.c_double), ("R1", ct.c_double), ("R2", ct.c_double), ("R3", ct.c_double),
+                ("status", ct.c_int32)]
+
+
+EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sav_set_state",
+           "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft",
+           "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
+           "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get")
+
+_lib_handle = None
+
+
+class SavError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class WCNegativeError(SavError):
+    """error("FATAL: W + C < 0") of src/Utils.jl:162,180,188."""
+
+
+def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
+    """Loads libsav_hip.so; raises if it has not been built (no CPU fallback)."""
+    global _lib_handle
+    if _lib_handle is not None and path is None:
+        return _lib_handle
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is missing: build it with build.py (hipcc, gfx950)")
+    lib = ct.CDLL(str(p))
+    lib.sav_abi_version.restype = ct.c_int
+    lib.sav_create.argtypes = [ct.POINTER(SavConfigC), ct.POINTER(ct.c_void_p)]
+    lib.sav_destroy.argtypes = [ct.c_void_p]
+    lib.sav_last_error.argtypes = [ct.c_void_p]
+    lib.sav_last_error.restype = ct.c_char_p
+    lib.sav_set_state.argtypes = [ct.c_void_p, ct.c_int] + [_dp] * 14 + [ct.c_double]
+    lib.sav_get_state.argtypes = [ct.c_void_p, ct.c_int] + [_dp] * 14 + [_dp]
+    lib.sav_init_sav.argtypes = [ct.c_void_p]
+    lib.sav_step.argtypes = [ct.c_void_p, ct.c_double, ct.c_int, ct.POINTER(SavStepStatsC)]
+    lib.sav_run.argtypes = [ct.c_void_p, _dp, ct.c_int, ct.c_int, _dp, _dp, ct.POINTER(ct.c_int32)]
+    lib.sav_rfft.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int]
+    lib.sav_irfft.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int]
+    lib.sav_fft_full.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int, ct.c_int]
+    lib.sav_eval_functionals.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.sav_psi_solve.argtypes = [ct.c_void_p, _dp, ct.c_double, ct.c_double, _dp, _dp,
+                                  ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), _dp]
+    lib.sav_modified_energy.argtypes = [ct.c_void_p, _dp, _dp]
+    lib.sav_last_elapsed_ms.argtypes = [ct.c_void_p, _dp]
+    lib.sav_kernel_timing.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_kernel_timing_get.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64), _dp, _dp]
+    if path is None:
+        _lib_handle = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a, dtype):
+    """Fortran-ordered contiguous array of the given dtype (no copy when possible)."""
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+# ---------------------------------------------------------------------------
+# FieldState (src/Types.jl:167-232) in Julia shapes, column-major
+# ---------------------------------------------------------------------------
+@dataclass
+class FieldState:
+    phi: np.ndarray
+    phi_hat: np.ndarray
+    psi: np.ndarray
+    psi_hat: np.ndarray
+    u: np.ndarray
+    u_hat: np.ndarray
+    p: np.ndarray
+    p_hat: np.ndarray
+    mu: np.ndarray
+    mu_hat: np.ndarray
+    nu: np.ndarray
+    nu_hat: np.ndarray
+    Q: float = 0.0
+    R1: float = 0.0
+    R2: float = 0.0
+    R3: float = 0.0
+    A0: np.ndarray = field(default_factory=lambda: np.zeros(1))
+    dt: float = 0.0
+
+    ARRAYS = ("phi", "phi_hat", "psi", "psi_hat", "u", "u_hat", "p", "p_hat", "mu", "mu_hat", "nu", "nu_hat")
+
+    @staticmethod
+    def zeros(conf: Config) -> "FieldState":
+        """FieldState(Nx, Ny, N) of src/Types.jl:204-232 (3D: Types_3d.jl with 3 velocity components)."""
+        sp = (conf.Nx, conf.Ny) + ((conf.Nz,) if conf.ndim == 3 else ())
+        hp = (conf.Nxh,) + sp[1:]
+        nd = conf.ndim
+        r = lambda *s: np.zeros(s, dtype=np.float64, order="F")
+        z = lambda *s: np.zeros(s, dtype=np.complex128, order="F")
+        return FieldState(phi=r(*sp, conf.N), phi_hat=z(*hp, conf.N), psi=r(*sp, conf.N),
+                          psi_hat=z(*hp, conf.N), u=r(*sp, nd), u_hat=z(*hp, nd), p=r(*sp),
+                          p_hat=z(*hp), mu=r(*sp, conf.N), mu_hat=z(*hp, conf.N), nu=r(*sp, conf.N),
+                          nu_hat=z(*hp, conf.N), A0=np.zeros(conf.N))
+
+    def copy(self) -> "FieldState":
+        return FieldState(**{k: (v.copy(order="K") if isinstance(v, np.ndarray) else v)
+                             for k, v in self.__dict__.items()})
+
+
+def update_state(dest: FieldState, src: FieldState) -> FieldState:
+    """update_state! (src/Types.jl:243-265): copies the 12 arrays and Q, R1..R3; not A0, not dt."""
+    for k in FieldState.ARRAYS:
+        getattr(dest, k)[...] = getattr(src, k)
+    dest.Q, dest.R1, dest.R2, dest.R3 = src.Q, src.R1, src.R2, src.R3
+    return dest
+
+
+# ---------------------------------------------------------------------------
+# Operators: the device handle (build_operators, src/SpectralUtils.jl:36-118)
+# ---------------------------------------------------------------------------
+class Operators:
+    """Owns one libsav_hip handle: plans, twiddles, multipliers, device state.
+
+    ``precision`` is SAV_PREC_FP64 (the reference's arithmetic) or SAV_PREC_FP32
+    (FP32 fields/FFTs with FP64 reductions, 3x3 solve and Krylov scalars)."""
+
+    def __init__(self, conf: Config, precision: int = SAV_PREC_FP64, itmax: int = 20, atol: float = 1e-14,
+                 device: int = 0, precond: int = SAV_PRECOND_DIAG):
+        self.lib = load_library()
+        self.conf = conf
+        self._pin = np.ascontiguousarray(conf.psi_in_v[: conf.N], dtype=np.float64)
+        self._pout = np.ascontiguousarray(conf.psi_out_v[: conf.N], dtype=np.float64)
+        c = SavConfigC()
+        c.ndim, c.N, c.Nx, c.Ny, c.Nz = conf.ndim, conf.N, conf.Nx, conf.Ny, conf.Nz
+        c.Lx, c.Ly, c.Lz = conf.Lx, conf.Ly, conf.Lz
+        c.epsilon, c.M_phi, c.M0_psi, c.eta, c.gamma_surf = (conf.epsilon, conf.M_phi, conf.M0_psi,
+                                                            conf.eta, conf.gamma_surf)
+        c.gamma_area, c.gamma_bend = conf.gamma_area, conf.gamma_bend
+        c.gamma_in, c.beta_in, c.gamma_out, c.beta_out, c.lamda = (conf.gamma_in, conf.beta_in,
+                                                                   conf.gamma_out, conf.beta_out, conf.lamda)
+        c.psi_in_v, c.psi_out_v = _ptr(self._pin), _ptr(self._pout)
+        c.S1, c.S2, c.S3, c.S4 = conf.S1, conf.S2, conf.S3, conf.S4
+        c.C1, c.C2, c.C3 = conf.C1, conf.C2, conf.C3
+        c.dt, c.T, c.tol = conf.dt, conf.T, conf.tol
+        c.precision, c.precond, c.itmax, c.atol, c.device = precision, precond, itmax, atol, device
+        self.precision = precision
+        h = ct.c_void_p()
+        rc = self.lib.sav_create(ct.byref(c), ct.byref(h))
+        if rc != 0:
+            raise SavError(rc, self.lib.sav_last_error(None).decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sav_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self.lib.sav_last_error(self.h).decode()
+            if rc == SAV_E_WC_NEGATIVE:
+                raise WCNegativeError(rc, f"{what}: {msg}")
+            raise SavError(rc, f"{what}: {msg}")
+        return rc
+
+    # -- state ----------------------------------------------------------------
+    def set_state(self, which: int, st: FieldState):
+        arr = []
+        keep = []
+        for k in FieldState.ARRAYS:
+            v = getattr(st, k)
+            a = _f(v, np.complex128 if k.endswith("_hat") else np.float64)
+            keep.append(a)
+            arr.append(_ptr(a))
+        scal = np.array([st.Q, st.R1, st.R2, st.R3], dtype=np.float64)
+        A0 = np.ascontiguousarray(st.A0, dtype=np.float64)
+        if A0.size < self.conf.N:
+            A0 = np.zeros(self.conf.N)
+        self._check(self.lib.sav_set_state(self.h, which, *arr, _ptr(scal), _ptr(A0), float(st.dt)), "set_state")
+
+    def get_state(self, which: int, st: Optional[FieldState] = None) -> FieldState:
+        st = st or FieldState.zeros(self.conf)
+        ptrs = []
+        for k in FieldState.ARRAYS:
+            a = getattr(st, k)
+            assert a.flags.f_contiguous
+            ptrs.append(_ptr(a))
+        scal = np.zeros(4)
+        A0 = np.zeros(self.conf.N)
+        dt = ct.c_double(0.0)
+        self._check(self.lib.sav_get_state(self.h, which, *ptrs, _ptr(scal), _ptr(A0), ct.byref(dt)), "get_state")
+        st.Q, st.R1, st.R2, st.R3 = (float(x) for x in scal)
+        if which == SAV_PRESENT:
+            st.A0 = A0
+        st.dt = dt.value
+        return st
+
+    def init_sav(self):
+        """simulation.jl:24-53 on the present state (old = present afterwards)."""
+        return self._check(self.lib.sav_init_sav(self.h), "init_sav")
+
+    def step(self, dt: float, n: int):
+        """One loop body (simulation.jl:73-110) + monitor; returns the step stats."""
+        st = SavStepStatsC()
+        rc = self._check(self.lib.sav_step(self.h, float(dt), int(n), ct.byref(st)), "step")
+        return rc, st
+
+    def run(self, t_nodes: np.ndarray, n0: int = 1):
+        t = np.ascontiguousarray(t_nodes, dtype=np.float64)
+        m = max(len(t) - 1, 0)
+        E = np.zeros(m)
+        A = np.zeros(m)
+        kit = ct.c_int32(0)
+        rc = self._check(self.lib.sav_run(self.h, _ptr(t), len(t), n0, _ptr(E), _ptr(A), ct.byref(kit)), "run")
+        return rc, E, A, kit.value
+
+    def elapsed_ms(self) -> float:
+        v = ct.c_double(0.0)
+        self.lib.sav_last_elapsed_ms(self.h, ct.byref(v))
+        return v.value
+
+    KERNELS = {1: "k_mv_cols_inv", 2: "k_mv_rows", 3: "k_mv_cols_fwd"}
+
+    def kernel_timing(self, kernel_id: int):
+        """Starts HIP-event timing of one psi-matvec kernel (0 stops)."""
+        self._check(self.lib.sav_kernel_timing(self.h, int(kernel_id)), "kernel_timing")
+
+    def kernel_timing_get(self):
+        n, ms, b = ct.c_int64(0), ct.c_double(0), ct.c_double(0)
+        self._check(self.lib.sav_kernel_timing_get(self.h, ct.byref(n), ct.byref(ms), ct.byref(b)),
+                    "kernel_timing_get")
+        return n.value, ms.value, b.value
+
+    # -- transforms (to_spectral!, to_physical!) --------------------------------
+    def _count(self, a, half=False):
+        sp = self.conf.Nx * self.conf.Ny * (self.conf.Nz if self.conf.ndim == 3 else 1)
+        if half:
+            sp = sp // self.conf.Nx * self.conf.Nxh
+        return a.size // sp
+
+    def rfft(self, u: np.ndarray) -> np.ndarray:
+        u = _f(u, np.float64)
+        cnt = self._count(u)
+        shp = (self.conf.Nxh,) + u.shape[1:]
+        out = np.zeros(shp, dtype=np.complex128, order="F")
+        self._check(self.lib.sav_rfft(self.h, _ptr(u), _ptr(out), cnt), "rfft")
+        return out
+
+    def irfft(self, uh: np.ndarray) -> np.ndarray:
+        uh = _f(uh, np.complex128)
+        cnt = self._count(uh, half=True)
+        shp = (self.conf.Nx,) + uh.shape[1:]
+        out = np.zeros(shp, dtype=np.float64, order="F")
+        self._check(self.lib.sav_irfft(self.h, _ptr(uh), _ptr(out), cnt), "irfft")
+        return out
+
+    def fft_full(self, x: np.ndarray, inverse: bool = False) -> np.ndarray:
+        x = _f(x, np.complex128)
+        cnt = self._count(x)
+        out = np.zeros_like(x, order="F")
+        self._check(self.lib.sav_fft_full(self.h, _ptr(x), _ptr(out), cnt,
+                                          int(inverse)), "fft_full")
+        return out
+
+    def eval_functionals(self, phi, psi):
+        phi = _f(phi, np.float64)
+        psi = _f(psi, np.float64)
+        H1, H2, H3, G = (np.zeros_like(phi, order="F") for _ in range(4))
+        W = np.zeros(3)
+        area = np.zeros(self.conf.N)
+        self._check(self.lib.sav_eval_functionals(self.h, _ptr(phi), _ptr(psi), _ptr(H1), _ptr(H2), _ptr(H3),
+                                                  _ptr(G), _ptr(W), _ptr(area)), "eval_functionals")
+        return H1, H2, H3, G, W, area
+
+    def psi_solve(self, phi_star, a, dt, rhs_hat):
+        phi_star = _f(phi_star, np.float64)
+        rhs = _f(rhs_hat, np.complex128)
+        out = np.zeros_like(rhs, order="F")
+        it, sv, rn = ct.c_int32(0), ct.c_int32(0), ct.c_double(0)
+        rc = self._check(self.lib.sav_psi_solve(self.h, _ptr(phi_star), float(a), float(dt),
+                                                _ptr(rhs), _ptr(out),
+                                                ct.byref(it), ct.byref(sv), ct.byref(rn)), "psi_solve")
+        return out, dict(iters=it.value, solved=bool(sv.value), rnorm=rn.value, rc=rc)
+
+    def modified_energy(self):
+        E, ar = ct.c_double(0), ct.c_double(0)
+        self._check(self.lib.sav_modified_energy(self.h, ct.byref(E), ct.byref(ar)), "modified_energy")
+        return E.value, ar.value
+
+
+def build_operators(conf: Config, **kw) -> Operators:
+    """build_operators(cfg) of src/SpectralUtils.jl:36 -> a device handle."""
+    return Operators(conf, **kw)
+
+
+def to_spectral(u: np.ndarray, ops: Operators) -> np.ndarray:
+    """to_spectral! (src/SpectralUtils.jl:132-135): unnormalised rfft over (x, y[, z])."""
+    return ops.rfft(u)
+
+
+def to_physical(u_hat: np.ndarray, ops: Operators) -> np.ndarray:
+    """to_physical! (src/SpectralUtils.jl:144-148): irfft scaled by 1/(Nx Ny[Nz]); input untouched."""
+    return ops.irfft(u_hat)
+
+
+# ---------------------------------------------------------------------------
+# Initial conditions (src/Init.jl:74-217), with the unseeded rand() of
+# state_type 4 replaced by a seeded SplitMix64 stream.
+# ---------------------------------------------------------------------------
+def generate_initial_condition(conf: Config, ops: Optional[Operators], state_type: int, seed: int = 1234):
+    if conf.ndim != 2:
+        raise ValueError("generate_initial_condition: 2D shapes only (the 3D branch of Init_3d.jl is broken)")
+    Nx, Ny, N = conf.Nx, conf.Ny, conf.N
+    Lx, Ly, eps = conf.Lx, conf.Ly, conf.epsilon
+    x = np.linspace(0, Lx - conf.dx, Nx)
+    y = np.linspace(0, Ly - conf.dy, Ny)
+    X, Y = np.meshgrid(x, y, indexing="ij")                 # (Nx, Ny) like Init.jl:83-84
+    phi = np.zeros((Nx, Ny, N), order="F")
+    s2e = math.sqrt(2) * eps
+    if N == 1:
+        cx, cy = [Lx / 2], [Ly / 2]
+    elif N == 2:
+        cx = [0.75, 1.25] if conf.goal == "g" else [0.65, 1.35]
+        cy = [Ly / 2, Ly / 2]
+    else:
+        cx = [Lx / 4, Lx / 2, 3 * Lx / 4]
+        cy = [Ly / 2] * 3
+    if N == 1:
+        if state_type > 6:
+            raise ValueError(f"state_type {state_type} invalid for N = 1")
+        cxn, cyn = cx[0], cy[0]
+        if state_type == 1:
+            R = 0.2 * Lx
+            dist = np.sqrt((X - cxn) ** 2 / 2 + (Y - cyn) ** 2)
+            phi[:, :, 0] = np.tanh((R - dist) / s2e)
+        elif state_type == 2:
+            Xc, Yc = X - cxn, Y - cyn
+            radius, sm = 0.65 * Lx, 40.0
+            ks, kc = math.sqrt(3) / 2, 0.5
+            d1, d2, d3 = -Yc, ks * Xc + kc * Yc, -ks * Xc + kc * Yc
+            mx = np.maximum(d1, np.maximum(d2, d3))
+            d = mx + (1 / sm) * np.log(np.exp(sm * (d1 - mx)) + np.exp(sm * (d2 - mx)) + np.exp(sm * (d3 - mx)))
+            phi[:, :, 0] = -np.tanh((d - radius / 3) / s2e)
+        elif state_type == 3:
+            R0, amp = (0.3, 0.01) if conf.goal == "s" else (0.18, 0.02)
+            r = np.sqrt((X - cxn) ** 2 + (Y - cyn) ** 2)
+            th = np.arctan2(X - cxn, Y - cyn)
+            phi[:, :, 0] = np.tanh((R0 + amp * np.cos(10 * th) - r) / s2e)
+        elif state_type == 4:
+            R0, amp = (0.3, 0.15) if conf.goal == "s" else (0.2, 0.25)
+            r = np.sqrt((X - cxn) ** 2 + (Y - cyn) ** 2)
+            th = np.arctan2(X - cxn, Y - cyn)
+            Rt = np.full(th.shape, R0)
+            rng = SplitMix64(seed)
+            for k in range(1, 11):
+                an = (rng.uniform() * 2 - 1) * amp / k
+                pn = rng.uniform() * 2 * math.pi
+                Rt = Rt + an * R0 * np.cos(k * th + pn)
+            phi[:, :, 0] = np.tanh((Rt - r) / s2e)
+        else:
+            raise ValueError(f"state_type {state_type} has no shape in the reference")
+    else:
+        if state_type <= 6:
+            raise ValueError(f"state_type {state_type} invalid for N = {N}")
+        for n in range(N):
+            R0, amp = (0.2, 0.02) if state_type == 7 else (0.3, 0.01)
+            r = np.sqrt((X - cx[n]) ** 2 + (Y - cy[n]) ** 2)
+            th = np.arctan2(X - cx[n], Y - cy[n])
+            phi[:, :, n] = np.tanh((R0 + amp * np.cos(10 * th) - r) / s2e)
+    psi = np.zeros_like(phi, order="F")
+    for n in range(N):
+        psi[:, :, n] = (-0.1 * phi[:, :, n] + 0.7) if conf.goal == "s" else (-0.35 * phi[:, :, n] + 0.45)
+    return state_from_physical(conf, ops, phi, psi)
+
+
+def state_from_physical(conf: Config, ops: Optional[Operators], phi: np.ndarray, psi: np.ndarray) -> FieldState:
+    """A FieldState whose spectra are rfft of the physical fields (Init.jl:200-216);
+    u = p = mu = nu = 0 and Q = R1 = R2 = R3 = 1.  ``phi``/``psi`` in Julia shape."""
+    st = FieldState.zeros(conf)
+    st.phi[...] = phi
+    st.psi[...] = psi
+    if ops is None:
+        raise ValueError("state_from_physical needs the device operators for the transforms")
+    st.phi_hat[...] = ops.rfft(st.phi)
+    st.psi_hat[...] = ops.rfft(st.psi)
+    st.Q = st.R1 = st.R2 = st.R3 = 1.0
+    return st
+
+
+def julia_shape(conf: Config, a: np.ndarray) -> np.ndarray:
+    """Memory-order array (N, [Nz,] Ny, Nx) -> Julia-shaped Fortran view (Nx, Ny, [Nz,] N)."""
+    return a.T
+
+
+# ---------------------------------------------------------------------------
+# run_simulation (src/simulation.jl:2-155) without plots and .mat output
+# ---------------------------------------------------------------------------
+def time_nodes(dt: float, T: float) -> np.ndarray:
+    """Two-phase schedule of simulation.jl:15-19: 0:1e-7:1e-5, then 1e-5:dt:T (first dropped)."""
+    t1 = np.arange(0, 101) * 1e-7
+    n2 = int(math.floor((T - 1e-5) / dt + 1e-9)) + 1
+    t2 = 1e-5 + np.arange(n2) * dt
+    return np.concatenate([t1, t2[1:]])
+
+
+def run_simulation(dt_val: float, T_val: float, state_type: int, save_path: Optional[str] = None,
+                   save_interval: float = 0.001, conf: Optional[Config] = None,
+                   state: Optional[FieldState] = None, t_nodes: Optional[np.ndarray] = None,
+                   precision: int = SAV_PREC_FP64):
+    """Returns (present, energy_history, area_ratio_history, Dt) like simulation.jl:154.
+
+    With ``conf``/``state``/``t_nodes`` left out it runs the reference's own set-up
+    (set_para_base, generate_initial_condition, two-phase Dt schedule)."""
+    conf = conf or set_para_base(dt_val, T_val)
+    ops = build_operators(conf, precision=precision)
+    Dt = time_nodes(conf.dt, T_val) if t_nodes is None else np.asarray(t_nodes, dtype=np.float64)
+    present = state if state is not None else generate_initial_condition(conf, ops, state_type)
+    ops.set_state(SAV_PRESENT, present)
+    ops.init_sav()
+    rc, E, A, kit = ops.run(Dt, 1)
+    present = ops.get_state(SAV_PRESENT)
+    ops.close()
+    return present, list(E), list(A), Dt
