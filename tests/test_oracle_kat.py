@@ -349,3 +349,44 @@ def test_first_step_is_bdf1_with_old_equal_present(jv, so):
     orc = so.Oracle(wl.conf)
     pres, old = orc.init_sav(so.state_from_physical(phi, psi, orc.ops, 2))
     assert np.array_equal(2.0 * pres.phi - old.phi, pres.phi)
+
+
+# --- learned-preconditioner network reference --------------------------------
+def test_conv_oracle_gradients_by_finite_differences():
+    from oracle import conv_oracle as co
+    rng = np.random.default_rng(21)
+    params = co.init_params(3)
+    params = [(W, rng.standard_normal(b.shape) * 0.1) for W, b in params]
+    x = rng.standard_normal((2, 2, 8, 6))
+    t = rng.standard_normal((2, 1, 8, 6))
+    loss, grads = co.mse_loss(params, x, t)
+    for l in range(4):
+        W, b = params[l]
+        for idx in [(0, 0, 0, 0), (W.shape[0] - 1, W.shape[1] - 1, 2, 1), (0, W.shape[1] - 1, 1, 2)]:
+            h = 1e-6
+            Wp = W.copy(); Wp[idx] += h
+            Wm = W.copy(); Wm[idx] -= h
+            pp = list(params); pp[l] = (Wp, b)
+            pm = list(params); pm[l] = (Wm, b)
+            fd = (co.mse_loss(pp, x, t)[0] - co.mse_loss(pm, x, t)[0]) / (2 * h)
+            assert grads[l][0][idx] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+        h = 1e-6
+        bp = b.copy(); bp[0] += h
+        bm = b.copy(); bm[0] -= h
+        pp = list(params); pp[l] = (W, bp)
+        pm = list(params); pm[l] = (W, bm)
+        fd = (co.mse_loss(pp, x, t)[0] - co.mse_loss(pm, x, t)[0]) / (2 * h)
+        assert grads[l][1][0] == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def test_conv_oracle_conv_is_a_periodic_correlation():
+    from oracle import conv_oracle as co
+    rng = np.random.default_rng(22)
+    a = rng.standard_normal((1, 1, 5, 4))
+    W = rng.standard_normal((1, 1, 3, 3))
+    out = co.conv3(W, a)
+    i, j = 0, 3
+    ref = sum(W[0, 0, di, dj] * a[0, 0, (i + di - 1) % 5, (j + dj - 1) % 4] for di in range(3) for dj in range(3))
+    assert out[0, 0, i, j] == pytest.approx(ref, rel=1e-14)
+    assert co.gelu(np.array([0.0]))[0] == 0.0
+    assert co.gelu(np.array([30.0]))[0] == pytest.approx(30.0)
