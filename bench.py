@@ -38,8 +38,8 @@ def _setup(jv, wl, precision):
     st = jv.FieldState.zeros(conf)
     st.phi[...] = phi.T
     st.psi[...] = psi.T
-    st.phi_hat[...] = ops.rfft(st.phi)
-    st.psi_hat[...] = ops.rfft(st.psi)
+    st.phi_hat[...] = ops.rfft_accurate(st.phi)
+    st.psi_hat[...] = ops.rfft_accurate(st.psi)
     st.Q = st.R1 = st.R2 = st.R3 = 1.0
     ops.set_state(jv.SAV_PRESENT, st)
     ops.init_sav()

@@ -15,6 +15,10 @@ namespace sav {
 template <typename R>
 struct alignas(2 * sizeof(R)) cplx {
   R x, y;
+  cplx() = default;
+  __host__ __device__ constexpr cplx(R a, R b) : x(a), y(b) {}
+  template <typename S>
+  __host__ __device__ constexpr cplx(const cplx<S>& o) : x((R)o.x), y((R)o.y) {}
 };
 
 template <typename R> __host__ __device__ __forceinline__ cplx<R> mk(R a, R b) { return cplx<R>{a, b}; }

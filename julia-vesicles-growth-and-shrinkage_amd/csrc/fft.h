@@ -12,6 +12,9 @@
 //   butterfly j in [0, L/R): inputs x[j + k*L/R], twiddles W_{Ls*R}^{(j mod Ls)*k},
 //   outputs at (j / Ls)*Ls*R + (j mod Ls) + m*Ls.
 #pragma once
+#include <cmath>
+#include <vector>
+
 #include "common.h"
 
 namespace sav {
@@ -22,7 +25,10 @@ struct FftPlanData {
   int E;
 };
 
-constexpr FftPlanData make_fft_plan(int L) {
+// ttmax: largest team (lanes per transform).  FP64 uses 128-lane teams (E = 8 for
+// L = 1024: low VGPR count, 4 waves/SIMD); FP32 uses 64-lane teams so that a
+// 512-thread column workgroup still spans 8 adjacent columns (64-byte segments).
+constexpr FftPlanData make_fft_plan(int L, int ttmax = 64) {
   FftPlanData p{};
   int n = L, np = 0, has3 = 0;
   while (n % 3 == 0) { p.rad[np++] = 3; n /= 3; has3 = 1; }
@@ -37,7 +43,7 @@ constexpr FftPlanData make_fft_plan(int L) {
   int lcm = has3 ? 3 * maxr : maxr;
   if (lcm < 1) lcm = 1;
   int E = lcm;
-  while (L / E > 64 && (L % (2 * E)) == 0 && 2 * E <= 32) E *= 2;
+  while (L / E > ttmax && (L % (2 * E)) == 0 && 2 * E <= 32) E *= 2;
   p.E = E;
   return p;
 }
@@ -122,18 +128,57 @@ __device__ __forceinline__ void bfly(cplx<R>* a) {
 // lds[p * ls] (ls = 1 for private row storage, = team count for interleaved
 // column storage).  `tw` = exp(-2 pi i n / L), n in [0, L).
 // ---------------------------------------------------------------------------
-template <typename R, int L, bool INV>
+// Pass-ordered twiddle table of a length-L plan: for pass p >= 1 (stride LS_p,
+// radix R_p) the row of butterfly residue jm holds W_{LS R}^{jm k}, k = 1..R-1, at
+// offset twoff(p) + jm*8 + (k-1): one butterfly reads 8 consecutive entries.
+constexpr int twiddle_table_len(int L) {
+  FftPlanData P = make_fft_plan(L);
+  int ls = 1, n = 0;
+  for (int p = 0; p < P.np; ++p) {
+    if (p > 0) n += ls * 8;
+    ls *= P.rad[p];
+  }
+  return n > 0 ? n : 1;
+}
+
+// TTMAX: largest team; PADS: LDS padding, one slot every 2^PADS positions (0 = none),
+// which spreads the stride-R Stockham writes of the early passes over the banks.
+// TWM: 0 = plain table exp(-2 pi i n / L), 1 = pass-ordered table (twiddle_table_len).
+// WSYNC: the team is one wavefront (TT == 64, one team per wave): LDS exchanges between
+// passes need only a wavefront-level barrier instead of a workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Default twiddle layout per precision: FP64 reads the compact plain table (a smaller
+// cache footprint wins), FP32 the pass-ordered one (fewer scattered loads per butterfly).
+template <typename R>
+constexpr int default_twm() {
+  return sizeof(R) == 8 ? 0 : 1;
+}
+
+template <typename R, int L, bool INV, int TTMAX = (sizeof(R) == 8 ? 128 : 64), int PADS = 0,
+          int TWM = default_twm<R>(), int WSYNC = 0>
 struct TeamFFT {
-  static constexpr FftPlanData P = make_fft_plan(L);
+  static constexpr FftPlanData P = make_fft_plan(L, TTMAX);
   static constexpr int E = P.E;
   static constexpr int TT = L / E;
   static constexpr int R0 = P.rad[0];
   static constexpr int RL = P.rad[P.np - 1];
+  static constexpr int LDS_LEN = PADS ? L + (L >> PADS) + 1 : L;   // LDS slots per transform
+  __device__ static __forceinline__ int lidx(int p) { return PADS ? p + (p >> PADS) : p; }
+  static_assert(!WSYNC || L / P.E == 64, "wavefront-synchronised teams must be exactly one wavefront");
+  __device__ static __forceinline__ void sync() {
+    if constexpr (WSYNC) wave_lds_sync();
+    else __syncthreads();
+  }
 
   __device__ static __forceinline__ int in_pos(int t, int e) { return t + (e / R0) * TT + (e % R0) * (L / R0); }
   __device__ static __forceinline__ int out_pos(int t, int e) { return t + (e / RL) * TT + (e % RL) * (L / RL); }
 
-  template <int PI, int LS>
+  template <int PI, int LS, int TWO>
   __device__ static __forceinline__ void passes(cplx<R> (&v)[E], cplx<R>* lds, int ls, int t,
                                                 const cplx<R>* __restrict__ tw) {
     if constexpr (PI < P.np) {
@@ -146,7 +191,7 @@ struct TeamFFT {
         for (int q = 0; q < NB; ++q) {
           const int j = t + q * TT;
 #pragma unroll
-          for (int k = 0; k < RAD; ++k) v[q * RAD + k] = lds[(j + k * LR) * ls];
+          for (int k = 0; k < RAD; ++k) v[q * RAD + k] = lds[lidx(j + k * LR) * ls];
         }
       }
 #pragma unroll
@@ -154,9 +199,10 @@ struct TeamFFT {
         const int j = t + q * TT;
         if constexpr (LS > 1) {
           const int jm = j % LS;
+          const cplx<R>* twr = tw + TWO + jm * 8;
 #pragma unroll
           for (int k = 1; k < RAD; ++k) {
-            cplx<R> w = tw[jm * k * TWS];
+            cplx<R> w = TWM ? twr[k - 1] : tw[jm * k * TWS];
             if (INV) w.y = -w.y;
             v[q * RAD + k] = v[q * RAD + k] * w;
           }
@@ -164,24 +210,82 @@ struct TeamFFT {
         bfly<R, INV, RAD>(&v[q * RAD]);
       }
       if constexpr (PI + 1 < P.np) {
-        __syncthreads();
+        sync();
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
           const int j = t + q * TT;
           const int base = (j / LS) * LS * RAD + (j % LS);
 #pragma unroll
-          for (int m = 0; m < RAD; ++m) lds[(base + m * LS) * ls] = v[q * RAD + m];
+          for (int m = 0; m < RAD; ++m) lds[lidx(base + m * LS) * ls] = v[q * RAD + m];
         }
-        __syncthreads();
-        passes<PI + 1, LS * RAD>(v, lds, ls, t, tw);
+        sync();
+        passes<PI + 1, LS * RAD, (PI > 0 ? TWO + LS * 8 : TWO)>(v, lds, ls, t, tw);
       }
     }
   }
 
   __device__ static __forceinline__ void run(cplx<R> (&v)[E], cplx<R>* lds, int ls, int t,
                                              const cplx<R>* __restrict__ tw) {
-    passes<0, 1>(v, lds, ls, t, tw);
+    passes<0, 1, 0>(v, lds, ls, t, tw);
   }
+};
+
+// Host: pass-ordered twiddle table (see twiddle_table_len), computed in long double.
+template <typename R>
+inline void build_pass_twiddles(int L, cplx<R>* out) {
+  const FftPlanData P = make_fft_plan(L);
+  int ls = 1, off = 0;
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int p = 0; p < P.np; ++p) {
+    const int rad = P.rad[p];
+    if (p > 0) {
+      for (int jm = 0; jm < ls; ++jm)
+        for (int k = 0; k < 8; ++k) {
+          cplx<R> w{R(0), R(0)};
+          if (k + 1 < rad) {
+            const long double a = -two_pi * (long double)(jm * (k + 1)) / (long double)(ls * rad);
+            w = cplx<R>{(R)cosl(a), (R)sinl(a)};
+          }
+          out[off + jm * 8 + k] = w;
+        }
+      off += ls * 8;
+    }
+    ls *= rad;
+  }
+  if (off == 0) out[0] = cplx<R>{R(1), R(0)};
+}
+
+// Host: the table the default TeamFFT<R, L, ...> expects for length L.
+template <typename R>
+inline std::vector<cplx<R>> default_twiddles(int L) {
+  if (default_twm<R>() == 0) {
+    std::vector<cplx<R>> w(L);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int n = 0; n < L; ++n) {
+      const long double a = -two_pi * (long double)n / (long double)L;
+      w[n] = cplx<R>{(R)cosl(a), (R)sinl(a)};
+    }
+    return w;
+  }
+  std::vector<cplx<R>> w(twiddle_table_len(L));
+  build_pass_twiddles<R>(L, w.data());
+  return w;
+}
+
+// Columns per workgroup of a strided (column) transform: 8 adjacent columns (128-byte
+// row segments in FP64, 64-byte in FP32), at least 256 threads per workgroup, at most
+// 1024 threads and the 160 KB of LDS of a gfx950 CU.
+template <typename R, int L>
+struct ColTile {
+  using F = TeamFFT<R, L, false>;
+  static constexpr int TT = F::TT;
+  static constexpr int ES = (int)sizeof(cplx<R>);
+  static constexpr int want = 8 > (256 / TT) ? 8 : (256 / TT);
+  static constexpr int cap0 = 163840 / (L * ES);
+  static constexpr int capp2 = cap0 >= 256 ? 256 : cap0 >= 128 ? 128 : cap0 >= 64 ? 64 : cap0 >= 32 ? 32 : cap0 >= 16 ? 16 : cap0 >= 8 ? 8 : cap0 >= 4 ? 4 : cap0 >= 2 ? 2 : 1;
+  static constexpr int C0 = want < capp2 ? want : capp2;
+  static constexpr int C = (C0 * TT > 1024) ? (1024 / TT) : C0;
+  static constexpr int NT = C * TT;
 };
 
 // Multipliers applied while loading the first inverse pass.

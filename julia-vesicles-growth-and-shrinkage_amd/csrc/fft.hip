@@ -6,17 +6,7 @@ namespace sav {
 
 // Columns handled per workgroup for a column (strided) transform of length L.
 template <typename R, int L>
-struct ColCfg {
-  using F = TeamFFT<R, L, false>;
-  static constexpr int TT = F::TT;
-  static constexpr int ES = (int)sizeof(cplx<R>);
-  static constexpr int want = (64 / ES) > (256 / TT) ? (64 / ES) : (256 / TT);
-  static constexpr int cap0 = 65536 / (L * ES);
-  static constexpr int capp2 = cap0 >= 256 ? 256 : cap0 >= 128 ? 128 : cap0 >= 64 ? 64 : cap0 >= 32 ? 32 : cap0 >= 16 ? 16 : cap0 >= 8 ? 8 : cap0 >= 4 ? 4 : cap0 >= 2 ? 2 : 1;
-  static constexpr int C0 = want < capp2 ? want : capp2;
-  static constexpr int C = (C0 * TT > 512) ? (512 / TT) : C0;
-  static constexpr int NT = C * TT;
-};
+using ColCfg = ColTile<R, L>;
 
 // Rows handled per workgroup for a contiguous transform of length L.
 template <typename R, int L>

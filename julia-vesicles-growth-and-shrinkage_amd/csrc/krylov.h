@@ -1,5 +1,7 @@
 // psi BiCGSTAB of src/Solvers.jl:39-81 on full complex spectra (2D).
 #pragma once
+#include <functional>
+
 #include "fft.h"
 #include "solver.h"
 
@@ -32,6 +34,12 @@ struct KrylovCtx {
   double ev_bytes[512];
   long long tk_count = 0;
   double tk_ms = 0.0, tk_bytes = 0.0;
+  // iterate snapshots for logging training pairs (nullptr: off)
+  cplx<R>* snap = nullptr;
+  int snap_cap = 0, snap_n = 0;
+  // preconditioned directions of the flexible solver
+  cplx<R>* Bp = nullptr;
+  cplx<R>* Bs = nullptr;
 };
 
 // Solves A x = b with b already in ctx.b; x in ctx.x on return.  Returns the
@@ -43,5 +51,13 @@ int bicgstab_2d(KrylovCtx<R>& k, int slot);
 // (one matvec + preconditioner, used by tests through the solver).
 template <typename R>
 void matvec_prec_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot);
+
+// out = A y without the preconditioner (queued, no synchronisation).
+template <typename R>
+void matvec_plain_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot);
+
+// Flexible right-preconditioned BiCGSTAB with preconditioner `prec` (see krylov.hip).
+template <typename R>
+int fbicgstab_2d(KrylovCtx<R>& k, int slot, const std::function<void(const cplx<R>*, cplx<R>*)>& prec);
 
 }  // namespace sav

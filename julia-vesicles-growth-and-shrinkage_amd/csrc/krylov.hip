@@ -15,30 +15,27 @@
 // host's convergence check.
 #include <cmath>
 
+#include <hip/hip_ext.h>
+
 #include "krylov.h"
 #include "launch.h"
 
 namespace sav {
 
 template <typename R, int L>
-struct KColCfg {
-  using F = TeamFFT<R, L, false>;
-  static constexpr int TT = F::TT;
-  static constexpr int ES = (int)sizeof(cplx<R>);
-  static constexpr int want = (64 / ES) > (256 / TT) ? (64 / ES) : (256 / TT);
-  static constexpr int cap0 = 65536 / (L * ES);
-  static constexpr int capp2 = cap0 >= 256 ? 256 : cap0 >= 128 ? 128 : cap0 >= 64 ? 64 : cap0 >= 32 ? 32 : cap0 >= 16 ? 16 : cap0 >= 8 ? 8 : cap0 >= 4 ? 4 : cap0 >= 2 ? 2 : 1;
-  static constexpr int C0 = want < capp2 ? want : capp2;
-  static constexpr int C = (C0 * TT > 512) ? (512 / TT) : C0;
-  static constexpr int NT = C * TT;
-};
+using KColCfg = ColTile<R, L>;
 
+// Row kernel: one team per row, 256 threads per workgroup; the LDS of each row is
+// padded by one slot every 16 (FP64) or 32 (FP32) positions against bank conflicts.
 template <typename R, int L>
 struct KRowCfg {
-  using F = TeamFFT<R, L, false>;
-  static constexpr int TT = F::TT;
+  static constexpr int PADS = sizeof(R) == 8 ? 4 : 5;
+  using FI = TeamFFT<R, L, true, (sizeof(R) == 8 ? 128 : 64), PADS>;
+  using FF = TeamFFT<R, L, false, (sizeof(R) == 8 ? 128 : 64), PADS>;
+  static constexpr int TT = FI::TT;
   static constexpr int ROWS = TT >= 256 ? 1 : 256 / TT;
   static constexpr int NT = ROWS * TT;
+  static constexpr int RS = FI::LDS_LEN | 1;   // odd row stride
 };
 
 __device__ __forceinline__ double pinv_diag(double a_dt, double Mmax, double S4, double kx0, double ky) {
@@ -50,7 +47,7 @@ __device__ __forceinline__ double pinv_diag(double a_dt, double Mmax, double S4,
 // K1: column inverse FFTs of D1f_x*(S4 x) and D1f_y*(S4 x)
 // ---------------------------------------------------------------------------
 template <typename R, int L>
-__global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
+__global__ __launch_bounds__((KColCfg<R, L>::NT), 4) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
                                                                     cplx<R>* __restrict__ Bx,
                                                                     cplx<R>* __restrict__ By, int Nx, double S4,
                                                                     const double* __restrict__ kx0,
@@ -68,25 +65,29 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_inv(const cplx<
   const long long base = (long long)blockIdx.y * Nx * L + col;
   const R kxv = act ? (R)kx0[col] : R(0);
   const R s4 = (R)S4;
-  cplx<R> vx[E], vy[E];
+  // b1 = S4 x is kept; the two derivative transforms run one after the other
+  cplx<R> b1[E], v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int pos = F::in_pos(t, e);
     cplx<R> a = act ? X[base + (long long)pos * Nx] : cplx<R>{R(0), R(0)};
-    const cplx<R> b1 = cplx<R>{s4 * a.x, s4 * a.y};
-    const R kyv = (R)ky[pos];
-    vx[e] = cplx<R>{-kxv * b1.y, kxv * b1.x};
-    vy[e] = cplx<R>{-kyv * b1.y, kyv * b1.x};
+    b1[e] = cplx<R>{s4 * a.x, s4 * a.y};
+    v[e] = cplx<R>{-kxv * b1[e].y, kxv * b1[e].x};
   }
-  F::run(vx, lds + c, C, t, tw);
+  F::run(v, lds + c, C, t, tw);
   if (act) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) Bx[base + (long long)F::out_pos(t, e) * Nx] = vx[e];
+    for (int e = 0; e < E; ++e) Bx[base + (long long)F::out_pos(t, e) * Nx] = v[e];
   }
-  F::run(vy, lds + c, C, t, tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const R kyv = (R)ky[F::in_pos(t, e)];
+    v[e] = cplx<R>{-kyv * b1[e].y, kyv * b1[e].x};
+  }
+  F::run(v, lds + c, C, t, tw);
   if (act) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) By[base + (long long)F::out_pos(t, e) * Nx] = vy[e];
+    for (int e = 0; e < E; ++e) By[base + (long long)F::out_pos(t, e) * Nx] = v[e];
   }
 }
 
@@ -99,17 +100,17 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
                                                                 double scale, const cplx<R>* __restrict__ tw,
                                                                 const KScal* __restrict__ ks) {
   if (ks->done) return;
-  using FI = TeamFFT<R, L, true>;
-  using FF = TeamFFT<R, L, false>;
-  constexpr int TT = FI::TT, E = FI::E, ROWS = KRowCfg<R, L>::ROWS;
-  __shared__ cplx<R> lds[ROWS * (L + 1)];
+  using FI = typename KRowCfg<R, L>::FI;
+  using FF = typename KRowCfg<R, L>::FF;
+  constexpr int TT = FI::TT, E = FI::E, ROWS = KRowCfg<R, L>::ROWS, RS = KRowCfg<R, L>::RS;
+  __shared__ cplx<R> lds[ROWS * RS];
   const int rloc = threadIdx.x / TT, t = threadIdx.x % TT;
   const long long row = (long long)blockIdx.x * ROWS + rloc;
   const bool act = row < 2 * nrows_half;
   const long long rr = act ? (row % nrows_half) : 0;
   cplx<R>* data = (row < nrows_half ? Bx : By) + rr * L;
   const R* m = M + rr * L;
-  cplx<R>* my = lds + rloc * (L + 1);
+  cplx<R>* my = lds + rloc * RS;
   cplx<R> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = act ? data[FI::in_pos(t, e)] : cplx<R>{R(0), R(0)};
@@ -121,11 +122,11 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
     const int pos = FI::out_pos(t, e);
     const R mv = act ? m[pos] : R(0);
     const cplx<R> z = cplx<R>{sc * v[e].x, sc * v[e].y};
-    my[pos] = cplx<R>{mv * z.x, mv * z.y};
+    my[FI::lidx(pos)] = cplx<R>{mv * z.x, mv * z.y};
   }
   __syncthreads();
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = my[FF::in_pos(t, e)];
+  for (int e = 0; e < E; ++e) v[e] = my[FF::lidx(FF::in_pos(t, e))];
   FF::run(v, my, 1, t, tw);
   if (act) {
 #pragma unroll
@@ -155,20 +156,39 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_fwd(
   const int col = blockIdx.x * C + c;
   const bool act = col < Nx;
   const long long base = (long long)blockIdx.y * Nx * L + col;
-  cplx<R> vx[E], vy[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const long long off = base + (long long)F::in_pos(t, e) * Nx;
-    vx[e] = act ? Bx[off] : cplx<R>{R(0), R(0)};
-    vy[e] = act ? By[off] : cplx<R>{R(0), R(0)};
-  }
-  F::run(vx, lds + c, C, t, tw);
-  F::run(vy, lds + c, C, t, tw);
   const double a_dt = ds->a / ds->dt;
   const double Mmax = ds->Mmax;
   const double kxd = act ? kx0[col] : 0.0;
   const R kxv = (R)kxd;
   const R adt = (R)a_dt;
+  // FP64: the two transforms run one after the other and the x-derivative term is
+  // parked in `out` (same lane, same address), halving the live registers.
+  constexpr bool TWO_PHASE = sizeof(R) == 8;
+  cplx<R> vx[E], vy[TWO_PHASE ? 1 : E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const long long off = base + (long long)F::in_pos(t, e) * Nx;
+    vx[e] = act ? Bx[off] : cplx<R>{R(0), R(0)};
+    if constexpr (!TWO_PHASE) vy[e] = act ? By[off] : cplx<R>{R(0), R(0)};
+  }
+  F::run(vx, lds + c, C, t, tw);
+  if constexpr (TWO_PHASE) {
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const long long off = base + (long long)F::out_pos(t, e) * Nx;
+        out[off] = cplx<R>{-kxv * vx[e].y, kxv * vx[e].x};
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const long long off = base + (long long)F::in_pos(t, e) * Nx;
+      vx[e] = act ? By[off] : cplx<R>{R(0), R(0)};
+    }
+    F::run(vx, lds + c, C, t, tw);
+  } else {
+    F::run(vy, lds + c, C, t, tw);
+  }
   double acc[3] = {0.0, 0.0, 0.0};
   if (act) {
 #pragma unroll
@@ -178,25 +198,35 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_fwd(
       const double kyd = ky[pos];
       const R kyv = (R)kyd;
       const cplx<R> y = Y[off];
-      const cplx<R> dx = cplx<R>{-kxv * vx[e].y, kxv * vx[e].x};
-      const cplx<R> dy = cplx<R>{-kyv * vy[e].y, kyv * vy[e].x};
+      cplx<R> dx, dy;
+      if constexpr (TWO_PHASE) {
+        dx = out[off];
+        dy = cplx<R>{-kyv * vx[e].y, kyv * vx[e].x};
+      } else {
+        dx = cplx<R>{-kxv * vx[e].y, kxv * vx[e].x};
+        dy = cplx<R>{-kyv * vy[e].y, kyv * vy[e].x};
+      }
       const cplx<R> sum = dx + dy;
       const cplx<R> q = cplx<R>{adt * y.x - sum.x, adt * y.y - sum.y};
-      const R pv = (R)pinv_diag(a_dt, Mmax, S4, kxd, kyd);
-      const cplx<R> o = cplx<R>{q.x * pv, q.y * pv};
+      cplx<R> o = q;
+      if constexpr ((MODE & 4) == 0) {
+        const R pv = (R)pinv_diag(a_dt, Mmax, S4, kxd, kyd);
+        o = cplx<R>{q.x * pv, q.y * pv};
+      }
       out[off] = o;
-      if constexpr (MODE == 1) {
+      if constexpr ((MODE & 3) == 1) {
         const cplx<R> cc = Cv[off];
         acc[0] += (double)cc.x * (double)o.x + (double)cc.y * (double)o.y;
         acc[1] += (double)cc.x * (double)o.y - (double)cc.y * (double)o.x;
-      } else if constexpr (MODE == 2) {
-        acc[0] += (double)o.x * (double)y.x + (double)o.y * (double)y.y;
-        acc[1] += (double)o.x * (double)y.y - (double)o.y * (double)y.x;
+      } else if constexpr ((MODE & 3) == 2) {
+        const cplx<R> dd = Cv[off];
+        acc[0] += (double)o.x * (double)dd.x + (double)o.y * (double)dd.y;
+        acc[1] += (double)o.x * (double)dd.y - (double)o.y * (double)dd.x;
         acc[2] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
       }
     }
   }
-  if constexpr (MODE != 0) {
+  if constexpr ((MODE & 3) != 0) {
     block_sum<3>(acc, scratch);
     if (threadIdx.x == 0) {
       const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
@@ -395,14 +425,20 @@ static int vec_blocks(long long n) {
 
 #define SAV_KRY_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
+// Start/stop events of the next launch of kernel `id` when it is the timed one (null
+// otherwise). They are passed to hipExtLaunchKernel, which stamps them from the
+// dispatch itself, so their interval is the kernel's own duration.
 template <typename R>
-static void tev_pre(KrylovCtx<R>& k, int id) {
-  if (k.timed == id && k.ev_used < k.ev_cap) SAV_HIP(hipEventRecord(k.ev[2 * k.ev_used], k.st));
+static void tev_pre(KrylovCtx<R>& k, int id, hipEvent_t& e0, hipEvent_t& e1) {
+  e0 = e1 = nullptr;
+  if (k.timed == id && k.ev_used < k.ev_cap) {
+    e0 = k.ev[2 * k.ev_used];
+    e1 = k.ev[2 * k.ev_used + 1];
+  }
 }
 template <typename R>
 static void tev_post(KrylovCtx<R>& k, int id, double bytes) {
   if (k.timed == id && k.ev_used < k.ev_cap) {
-    SAV_HIP(hipEventRecord(k.ev[2 * k.ev_used + 1], k.st));
     k.ev_iter[k.ev_used] = k.cur_iter;
     k.ev_bytes[k.ev_used] = bytes;
     k.ev_used++;
@@ -423,19 +459,22 @@ static void tev_collect(KrylovCtx<R>& k, int iters) {
   k.ev_used = 0;
 }
 
+// mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
 template <typename R>
-static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot) {
+static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
+                         const cplx<R>* partner = nullptr) {
   const KScal* ks = k.ks + slot;
   int nb3 = 0;
   const double sn = (double)sizeof(R) * (double)k.nfull;
-  tev_pre(k, 1);
+  hipEvent_t e0, e1;
+  tev_pre(k, 1, e0, e1);
   switch (k.Ny) {
 #define X(L)                                                                                                  \
   case L: {                                                                                                   \
     using CC = KColCfg<R, L>;                                                                                 \
     dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);                                                               \
-    hipLaunchKernelGGL((k_mv_cols_inv<R, L>), grid, dim3(CC::NT), 0, k.st, y, k.Bx, k.By, k.Nx, k.S4, k.kx0,  \
-                       k.ky, k.tw_y, ks);                                                                     \
+    hipExtLaunchKernelGGL((k_mv_cols_inv<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx, \
+                          k.S4, k.kx0, k.ky, k.tw_y, ks);                                                     \
     break;                                                                                                    \
   }
     SAV_KRY_SIZES(X)
@@ -444,7 +483,7 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   }
   SAV_LAUNCH_CHECK();
   tev_post(k, 1, 6.0 * sn);
-  tev_pre(k, 2);
+  tev_pre(k, 2, e0, e1);
   const long long nrh = (long long)k.N * k.Ny;
   const double scale = 1.0 / ((double)k.Nx * k.Ny);
   switch (k.Nx) {
@@ -452,8 +491,8 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   case L: {                                                                                              \
     using CC = KRowCfg<R, L>;                                                                            \
     dim3 grid((unsigned)((2 * nrh + CC::ROWS - 1) / CC::ROWS));                                          \
-    hipLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, k.M, nrh, scale, k.tw_x, \
-                       ks);                                                                              \
+    hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, k.M, nrh,  \
+                          scale, k.tw_x, ks);                                                            \
     break;                                                                                               \
   }
     SAV_KRY_SIZES(X)
@@ -462,30 +501,33 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   }
   SAV_LAUNCH_CHECK();
   tev_post(k, 2, 9.0 * sn);
-  tev_pre(k, 3);
+  tev_pre(k, 3, e0, e1);
+#define KL(L, M)                                                                                              \
+  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, y, partner, \
+                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials)
   switch (k.Ny) {
-#define X(L)                                                                                                   \
-  case L: {                                                                                                    \
-    using CC = KColCfg<R, L>;                                                                                  \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);                                                                \
-    nb3 = (int)(grid.x * grid.y);                                                                              \
-    if (mode == 1)                                                                                             \
-      hipLaunchKernelGGL((k_mv_cols_fwd<R, L, 1>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, y, k.b, out, k.Nx, \
-                         k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials);                                     \
-    else if (mode == 2)                                                                                        \
-      hipLaunchKernelGGL((k_mv_cols_fwd<R, L, 2>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, y, k.b, out, k.Nx, \
-                         k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials);                                     \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_mv_cols_fwd<R, L, 0>), grid, dim3(CC::NT), 0, k.st, k.Bx, k.By, y, k.b, out, k.Nx, \
-                         k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials);                                     \
-    break;                                                                                                     \
+#define X(L)                                      \
+  case L: {                                       \
+    using CC = KColCfg<R, L>;                     \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);   \
+    nb3 = (int)(grid.x * grid.y);                 \
+    switch (mode) {                               \
+      case 1: KL(L, 1); break;                    \
+      case 2: KL(L, 2); break;                    \
+      case 4: KL(L, 4); break;                    \
+      case 5: KL(L, 5); break;                    \
+      case 6: KL(L, 6); break;                    \
+      default: KL(L, 0); break;                   \
+    }                                             \
+    break;                                        \
   }
     SAV_KRY_SIZES(X)
 #undef X
     default: break;
   }
+#undef KL
   SAV_LAUNCH_CHECK();
-  tev_post(k, 3, (mode == 1 ? 10.0 : mode == 2 ? 8.0 : 6.0) * sn);
+  tev_post(k, 3, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn);
   return nb3;
 }
 
@@ -499,6 +541,13 @@ void matvec_prec_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot) {
 }
 
 template <typename R>
+void matvec_plain_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot) {
+  KScal z{};
+  SAV_HIP(hipMemcpyAsync(k.ks + slot, &z, sizeof(KScal), hipMemcpyHostToDevice, k.st));
+  matvec_launch(k, y, out, 4, slot);
+}
+
+template <typename R>
 int bicgstab_2d(KrylovCtx<R>& k, int slot) {
   const long long n = k.nfull;
   const int nbv = vec_blocks(n);
@@ -509,18 +558,20 @@ int bicgstab_2d(KrylovCtx<R>& k, int slot) {
   hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 0, k.atol, k.rtol, k.itmax);
   SAV_LAUNCH_CHECK();
   int queued = 0;
-  int batch = k.guess[slot] > 0 ? k.guess[slot] : 8;
+  const bool logging = k.snap != nullptr;
+  int batch = logging ? 1 : (k.guess[slot] > 0 ? k.guess[slot] : 8);
+  k.snap_n = 0;
   while (true) {
     const int nb = batch < k.itmax - queued ? batch : k.itmax - queued;
     for (int it = 0; it < nb; ++it) {
       k.cur_iter = queued + it + 1;
       // v = P^{-1} A p ; alpha = rho / <c, v>
-      int nb3 = matvec_launch(k, k.p, k.v, 1, slot);
+      int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b);
       hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 1, k.atol, k.rtol, k.itmax);
       // s = r - alpha v ; x += alpha p
       hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks);
       // t = P^{-1} A s ; omega = <t, s> / <t, t>
-      nb3 = matvec_launch(k, k.s, k.t, 2, slot);
+      nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s);
       hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 2, k.atol, k.rtol, k.itmax);
       // x += omega s ; r = s - omega t ; <c, r>, ||r||
       hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks, k.partials);
@@ -532,8 +583,14 @@ int bicgstab_2d(KrylovCtx<R>& k, int slot) {
     queued += nb;
     SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
     SAV_HIP(hipStreamSynchronize(k.st));
+    if (logging && k.snap_n < k.snap_cap && k.hks[slot].iter > k.snap_n) {
+      // iterate x_k after iteration k (only iterations that ran)
+      SAV_HIP(hipMemcpyAsync(k.snap + (long long)k.snap_n * n, k.x, n * sizeof(cplx<R>), hipMemcpyDeviceToDevice,
+                             k.st));
+      k.snap_n++;
+    }
     if (k.hks[slot].done || queued >= k.itmax) break;
-    batch = 2;
+    if (!logging) batch = 2;
   }
   const int iters = k.hks[slot].iter;
   k.guess[slot] = iters > 0 ? iters : 1;
@@ -541,6 +598,108 @@ int bicgstab_2d(KrylovCtx<R>& k, int slot) {
   return iters;
 }
 
+// Flexible (right-preconditioned) BiCGSTAB with a possibly nonlinear preconditioner
+// z = prec(r).  x0 = 0, shadow vector c = b.  Stops when the diagonally
+// preconditioned TRUE residual ||P^{-1}(b - A x)|| <= atol + rtol ||P^{-1} b||,
+// the quantity the reference's left-preconditioned solve tests.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fkry_init(const cplx<R>* __restrict__ b, cplx<R>* __restrict__ r,
+                                                   cplx<R>* __restrict__ p, cplx<R>* __restrict__ x, long long n,
+                                                   int Nx, int Ny, double S4, const double* __restrict__ kx0,
+                                                   const double* __restrict__ ky, const DScal* __restrict__ ds,
+                                                   double* __restrict__ partials) {
+  __shared__ double scratch[3 * 4];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const int ix = (int)(i % Nx);
+    const int iy = (int)((i / Nx) % Ny);
+    const double pv = pinv_diag(a_dt, Mmax, S4, kx0[ix], ky[iy]);
+    const cplx<R> bb = b[i];
+    r[i] = bb;
+    p[i] = bb;
+    x[i] = cplx<R>{R(0), R(0)};
+    acc[0] += (double)bb.x * (double)bb.x + (double)bb.y * (double)bb.y;
+    acc[2] += pv * pv * ((double)bb.x * (double)bb.x + (double)bb.y * (double)bb.y);
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 3 + 0] = acc[0];
+    partials[blockIdx.x * 3 + 1] = acc[1];
+    partials[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+// x += omega s_hat ; r = s - omega t ; partials <c, r>, ||P^{-1} r||^2
+template <typename R>
+__global__ __launch_bounds__(256) void k_fkry_upd2(const cplx<R>* __restrict__ s, const cplx<R>* __restrict__ shat,
+                                                   const cplx<R>* __restrict__ t, const cplx<R>* __restrict__ c,
+                                                   cplx<R>* __restrict__ x, cplx<R>* __restrict__ r, long long n,
+                                                   int Nx, int Ny, double S4, const double* __restrict__ kx0,
+                                                   const double* __restrict__ ky, const DScal* __restrict__ ds,
+                                                   const KScal* __restrict__ ks, double* __restrict__ partials) {
+  __shared__ double scratch[3 * 4];
+  if (ks->done) return;
+  const cplx<R> om = cplx<R>{(R)ks->wr, (R)ks->wi};
+  const cplx<R> mom = cplx<R>{(R)-ks->wr, (R)-ks->wi};
+  const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    x[i] = x[i] + om * shat[i];
+    const cplx<R> ri = s[i] + mom * t[i];
+    r[i] = ri;
+    const cplx<R> ci = c[i];
+    const int ix = (int)(i % Nx);
+    const int iy = (int)((i / Nx) % Ny);
+    const double pv = pinv_diag(a_dt, Mmax, S4, kx0[ix], ky[iy]);
+    acc[0] += (double)ci.x * (double)ri.x + (double)ci.y * (double)ri.y;
+    acc[1] += (double)ci.x * (double)ri.y - (double)ci.y * (double)ri.x;
+    acc[2] += pv * pv * ((double)ri.x * (double)ri.x + (double)ri.y * (double)ri.y);
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 3 + 0] = acc[0];
+    partials[blockIdx.x * 3 + 1] = acc[1];
+    partials[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+template <typename R>
+int fbicgstab_2d(KrylovCtx<R>& k, int slot, const std::function<void(const cplx<R>*, cplx<R>*)>& prec) {
+  const long long n = k.nfull;
+  const int nbv = vec_blocks(n);
+  KScal* ks = k.ks + slot;
+  hipLaunchKernelGGL((k_fkry_init<R>), dim3(nbv), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4,
+                     k.kx0, k.ky, k.ds, k.partials);
+  hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 0, k.atol, k.rtol, k.itmax);
+  SAV_LAUNCH_CHECK();
+  SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
+  SAV_HIP(hipStreamSynchronize(k.st));
+  for (int it = 0; it < k.itmax && !k.hks[slot].done; ++it) {
+    k.cur_iter = it + 1;
+    prec(k.p, k.Bp);                                        // p_hat
+    int nb3 = matvec_launch(k, k.Bp, k.v, 5, slot, k.b);    // v = A p_hat, <c, v>
+    hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 1, k.atol, k.rtol, k.itmax);
+    hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.Bp, k.s, k.x, n, ks);
+    prec(k.s, k.Bs);                                        // s_hat
+    nb3 = matvec_launch(k, k.Bs, k.t, 6, slot, k.s);        // t = A s_hat, <t, s>, <t, t>
+    hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 2, k.atol, k.rtol, k.itmax);
+    hipLaunchKernelGGL((k_fkry_upd2<R>), dim3(nbv), dim3(256), 0, k.st, k.s, k.Bs, k.t, k.b, k.x, k.r, n, k.Nx,
+                       k.Ny, k.S4, k.kx0, k.ky, k.ds, ks, k.partials);
+    hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 3, k.atol, k.rtol, k.itmax);
+    hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks);
+    SAV_LAUNCH_CHECK();
+    SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
+    SAV_HIP(hipStreamSynchronize(k.st));
+  }
+  return k.hks[slot].iter;
+}
+
+template int fbicgstab_2d<double>(KrylovCtx<double>&, int,
+                                  const std::function<void(const cplx<double>*, cplx<double>*)>&);
+template int fbicgstab_2d<float>(KrylovCtx<float>&, int, const std::function<void(const cplx<float>*, cplx<float>*)>&);
+template void matvec_plain_2d<double>(KrylovCtx<double>&, const cplx<double>*, cplx<double>*, int);
+template void matvec_plain_2d<float>(KrylovCtx<float>&, const cplx<float>*, cplx<float>*, int);
 template int bicgstab_2d<double>(KrylovCtx<double>&, int);
 template int bicgstab_2d<float>(KrylovCtx<float>&, int);
 template void matvec_prec_2d<double>(KrylovCtx<double>&, const cplx<double>*, cplx<double>*, int);

@@ -319,8 +319,9 @@ Net::Net(hipStream_t s) : st(s) {
 }
 
 Net::~Net() {
-  if (st) hipStreamSynchronize(st);
-  for (void* p : allocs) hipFree(p);
+  // a destructor cannot report: errors here are dropped on purpose
+  if (st) (void)hipStreamSynchronize(st);
+  for (void* p : allocs) (void)hipFree(p);
 }
 
 void Net::ensure(long long npix) {

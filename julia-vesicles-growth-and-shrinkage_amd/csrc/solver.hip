@@ -79,6 +79,9 @@ template <typename R>
 class Solver final : public ISolver {
  public:
   using C = cplx<R>;
+  // pointwise and spectral arithmetic runs in FP64 registers; storage is R
+  using T = double;
+  using CD = cplx<double>;
   HostCfg hc;
   sav_config_t c;
   int nd, N, Nx, Ny, Nz, Nxh;
@@ -148,11 +151,13 @@ class Solver final : public ISolver {
     g.Ny = Ny;
     g.Nz = Nz;
     g.Nxh = Nxh;
-    g.tw_x = upload(twiddles<R>(Nx));
-    g.tw_x2 = g.tw_x;
-    g.tw_xh = upload(twiddles<R>(Nx / 2));
-    g.tw_y = upload(twiddles<R>(Ny));
-    g.tw_z = nd == 3 ? upload(twiddles<R>(Nz)) : g.tw_y;
+    // FFT pass tables in the layout TeamFFT<R> reads; tw_x2 is the plain length-Nx
+    // table of the r2c/c2r split (X[k] = E[k] + W^k O[k]).
+    g.tw_x = upload(default_twiddles<R>(Nx));
+    g.tw_x2 = upload(twiddles<R>(Nx));
+    g.tw_xh = upload(default_twiddles<R>(Nx / 2));
+    g.tw_y = upload(default_twiddles<R>(Ny));
+    g.tw_z = nd == 3 ? upload(default_twiddles<R>(Nz)) : g.tw_y;
     g.k = KTables{d_kx, d_ky, d_kz, d_kx0};
     // states
     const long long nphys = (4LL * N + nd + 1) * P, nhalf = (4LL * N + nd + 1) * H;
@@ -331,18 +336,29 @@ class Solver final : public ISolver {
   // scalars W1..W3, R1s..R3s, area[n] (Utils.jl:36-249).  When set_A0 is true
   // (SAV initialisation) A0 := area first, as simulation.jl:25-26 does.
   // =========================================================================
-  void functionals(const R* phi, const R* psi, bool set_A0) {
+  void functionals(const R* phi, const R* psi, bool set_A0, const C* spec = nullptr, const C* spec_old = nullptr) {
     const long long NP = (long long)N * P;
     const PhysC q = pc();
     const int ndl = nd;
-    // derivatives of rfft(phi): gx, gy, lap, bih
-    rfft({job(phi, w_phat, N)});
+    // derivatives gx, gy, lap, bih of phi.  FP64: from rfft(phi) as the reference does
+    // (Utils.jl:37, 149, 167, 209).  FP32 (`spec` given): from the state spectra
+    // 2 phi_hat^n - phi_hat^{n-1}, because the FFT roundoff floor of rfft(phi) in FP32
+    // (~1e-7 of the largest coefficient at every k) would be multiplied by S1 k^4 in
+    // the biharmonic term; spectra stored in FP32 keep their own relative precision.
+    const C* src = w_phat;
+    const C* src2 = nullptr;
+    if (spec) {
+      src = spec;
+      src2 = spec_old;
+    } else {
+      rfft({job(phi, w_phat, N)});
+    }
     R* gx = w_d4;
     R* gy = w_d4 + NP;
     R* lap = w_d4 + 2 * NP;
     R* bih = w_d4 + 3 * NP;
-    irfft({job(w_phat, gx, N, MUL_DX), job(w_phat, gy, N, MUL_DY), job(w_phat, lap, N, MUL_LAP),
-           job(w_phat, bih, N, MUL_BIH)});
+    irfft({job(src, gx, N, MUL_DX, src2), job(src, gy, N, MUL_DY, src2), job(src, lap, N, MUL_LAP, src2),
+           job(src, bih, N, MUL_BIH, src2)});
     // per-vesicle sums: f_surf, f_bend, E2 integrand, f_osm, psi^2 ; omega -> w_om
     R* om = w_om;
     const double* pin = d_psi_in;
@@ -351,26 +367,26 @@ class Solver final : public ISolver {
     reduce_n<5>(s, P, N,
                 [=] __device__(long long i, int n, double(&acc)[5]) {
                   const long long k = (long long)n * Pl + i;
-                  const R ph = phi[k], ps = psi[k], gxx = gx[k], gyy = gy[k], lp = lap[k];
-                  const R eps = (R)q.eps;
+                  const T ph = phi[k], ps = psi[k], gxx = gx[k], gyy = gy[k], lp = lap[k];
+                  const T eps = (T)q.eps;
                   // f_surf (Utils.jl:41)
-                  const R gg = R(0.25) * (ph * ph - R(1)) * (ph * ph - R(1));
-                  const R c1 = (R)(3.0 * SQ2 / 4.0);
-                  const R fs = c1 * (gg / eps) + eps * (gxx * gxx + gyy * gyy) / R(2);
+                  const T gg = T(0.25) * (ph * ph - T(1)) * (ph * ph - T(1));
+                  const T c1 = (T)(3.0 * SQ2 / 4.0);
+                  const T fs = c1 * (gg / eps) + eps * (gxx * gxx + gyy * gyy) / T(2);
                   // f_bend (Utils.jl:49)
-                  const R gp = ph * (ph * ph - R(1));
-                  const R c2 = (R)(3.0 * SQ2 / (16.0 * q.eps));
-                  const R tb = gp / eps - eps * lp;
-                  const R fb = c2 * (tb * tb);
+                  const T gp = ph * (ph * ph - T(1));
+                  const T c2 = (T)(3.0 * SQ2 / (16.0 * q.eps));
+                  const T tb = gp / eps - eps * lp;
+                  const T fb = c2 * (tb * tb);
                   // E2 integrand (Utils.jl:175-177)
-                  const R e2 = R(0.5) * (R)q.S1 * (lp * lp) + R(0.5) * (R)q.S2 * (gxx * gxx + gyy * gyy) +
-                               R(0.5) * (R)q.S3 * (ph * ph);
+                  const T e2 = T(0.5) * (T)q.S1 * (lp * lp) + T(0.5) * (T)q.S2 * (gxx * gxx + gyy * gyy) +
+                               T(0.5) * (T)q.S3 * (ph * ph);
                   // f_osmotic (Utils.jl:53-60)
-                  const R P_ = R(-0.5) * (ph * ph * ph) + R(1.5) * ph;
-                  const R di = ps - (R)pin[n], dou = ps - (R)pout[n];
-                  const R fin = R(0.5) * (R)q.gin * (di * di) + (R)q.bin;
-                  const R fou = R(0.5) * (R)q.gout * (dou * dou) + (R)q.bout;
-                  const R fo = ((R(1) + P_) / R(2)) * fin + ((R(1) - P_) / R(2)) * fou;
+                  const T P_ = T(-0.5) * (ph * ph * ph) + T(1.5) * ph;
+                  const T di = ps - (T)pin[n], dou = ps - (T)pout[n];
+                  const T fin = T(0.5) * (T)q.gin * (di * di) + (T)q.bin;
+                  const T fou = T(0.5) * (T)q.gout * (dou * dou) + (T)q.bout;
+                  const T fo = ((T(1) + P_) / T(2)) * fin + ((T(1) - P_) / T(2)) * fou;
                   // omega (Utils.jl:152)
                   om[k] = (ph * ph * ph - ph) / eps - eps * lp;
                   acc[0] += (double)fs;
@@ -434,23 +450,23 @@ class Solver final : public ISolver {
     const R* lom = w_lom;
     foreach_n(s, NP, [=] __device__(long long k) {
       const int n = (int)(k / Pl);
-      const R ph = phi[k], ps = psi[k], w = om[k], lw = lom[k], lp = lap[k], bh = bih[k];
-      const R eps = (R)q.eps;
+      const T ph = phi[k], ps = psi[k], w = om[k], lw = lom[k], lp = lap[k], bh = bih[k];
+      const T eps = (T)q.eps;
       const double fac = q.garea * (dsl->area[n] - A0[n]) * (3.0 * SQ2 / 4.0);
-      H1[k] = ((R)fac * w) / (R)dsl->R1s;
-      const R sb = (R)q.gs * (R)(3.0 * SQ2 / 4.0) * w +
-                   (R)q.gb * (R)(3.0 * SQ2 / (8.0 * q.eps)) * (w * (R(3) * (ph * ph) - R(1)) / eps - eps * lw);
-      const R lin = (R)q.S1 * bh - (R)q.S2 * lp + (R)q.S3 * ph;
-      H2[k] = (sb - lin) / (R)dsl->R2s;
-      const R pp = R(-1.5) * (ph * ph) + R(1.5);
-      const R di = ps - (R)pin[n], dou = ps - (R)pout[n];
-      const R fin = R(0.5) * (R)q.gin * (di * di) + (R)q.bin;
-      const R fou = R(0.5) * (R)q.gout * (dou * dou) + (R)q.bout;
-      H3[k] = (R(0.5) * pp * (fin - fou)) / (R)dsl->R3s;
-      const R P_ = R(-0.5) * (ph * ph * ph) + R(1.5) * ph;
-      const R G1 = R(0.5) * (R(1) + P_) * ((R)q.gin * di) + R(0.5) * (R(1) - P_) * ((R)q.gout * dou);
-      const R G2 = (R)q.S4 * ps;
-      G[k] = (G1 - G2) / (R)dsl->R3s;
+      H1[k] = ((T)fac * w) / (T)dsl->R1s;
+      const T sb = (T)q.gs * (T)(3.0 * SQ2 / 4.0) * w +
+                   (T)q.gb * (T)(3.0 * SQ2 / (8.0 * q.eps)) * (w * (T(3) * (ph * ph) - T(1)) / eps - eps * lw);
+      const T lin = (T)q.S1 * bh - (T)q.S2 * lp + (T)q.S3 * ph;
+      H2[k] = (sb - lin) / (T)dsl->R2s;
+      const T pp = T(-1.5) * (ph * ph) + T(1.5);
+      const T di = ps - (T)pin[n], dou = ps - (T)pout[n];
+      const T fin = T(0.5) * (T)q.gin * (di * di) + (T)q.bin;
+      const T fou = T(0.5) * (T)q.gout * (dou * dou) + (T)q.bout;
+      H3[k] = (T(0.5) * pp * (fin - fou)) / (T)dsl->R3s;
+      const T P_ = T(-0.5) * (ph * ph * ph) + T(1.5) * ph;
+      const T G1 = T(0.5) * (T(1) + P_) * ((T)q.gin * di) + T(0.5) * (T(1) - P_) * ((T)q.gout * dou);
+      const T G2 = (T)q.S4 * ps;
+      G[k] = (G1 - G2) / (T)dsl->R3s;
     });
   }
 
@@ -469,7 +485,7 @@ class Solver final : public ISolver {
     R* rphys = w_d4 + 4 * NP;  // scratch (d4 slots 4,5 are free during solves)
     irfft({job(rhs, rphys, N)});
     C* b = kc.b;
-    foreach_n(s, NP, [=] __device__(long long i) { b[i] = C{rphys[i], R(0)}; });
+    foreach_n(s, NP, [=] __device__(long long i) { b[i] = CD{rphys[i], T(0)}; });
     fft_full_2d<R>(g, b, N, false, s);
     const int it = bicgstab_2d<R>(kc, slot);
     const C* x = kc.x;
@@ -505,18 +521,21 @@ class Solver final : public ISolver {
     R *phs = w_phis, *pss = w_psis, *M = w_M, *us = w_us;
     const double M0 = c.M0_psi;
     foreach_n(s, NP, [=] __device__(long long i) {
-      const R a = R(2.0) * phN[i] - phO[i];
+      const T a = T(2.0) * phN[i] - phO[i];
       phs[i] = a;
-      pss[i] = R(2.0) * psN[i] - psO[i];
-      const R t = a * a - R(1.0);
-      M[i] = R(0.1) * (R(1.0) - (R)M0 * (t * t));
+      pss[i] = T(2.0) * psN[i] - psO[i];
+      const T t = a * a - T(1.0);
+      M[i] = T(0.1) * (T(1.0) - (T)M0 * (t * t));
     });
-    foreach_n(s, (long long)nd * P, [=] __device__(long long i) { us[i] = R(2.0) * uN[i] - uO[i]; });
+    foreach_n(s, (long long)nd * P, [=] __device__(long long i) { us[i] = T(2.0) * uN[i] - uO[i]; });
     if (nd == 2) {
       // M_max = maximum(M_psi) (Solvers.jl:27)
       max_reduce(M, NP, &ds->Mmax);
     }
-    functionals(phs, pss, false);
+    if (sizeof(R) == 8)
+      functionals(phs, pss, false);
+    else
+      functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));
     // rfft of H1, G, H2, H3 -> w_Hh (same order)
     rfft({job(w_H, w_Hh, 4 * N)});
     // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115)
@@ -535,26 +554,26 @@ class Solver final : public ISolver {
       const double lapd = -k2, bihd = k2 * k2;
       const double Lphi = sc.S1 * bihd - sc.S2 * lapd + sc.S3;
       const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
-      const R Lp = (R)Lphi, lh = (R)lhs;
-      const R b = (R)dsl->b, cc = (R)dsl->c, dt = (R)dsl->dt;
-      const C xN = phhN[i], xO = phhO[i];
-      C t = C{-(b * xN.x + cc * xO.x) / dt, -(b * xN.y + cc * xO.y) / dt};
-      const C p11 = C{t.x / lh, t.y / lh};
+      const T Lp = (T)Lphi, lh = (T)lhs;
+      const T b = (T)dsl->b, cc = (T)dsl->c, dt = (T)dsl->dt;
+      const CD xN = phhN[i], xO = phhO[i];
+      CD t = CD{-(b * xN.x + cc * xO.x) / dt, -(b * xN.y + cc * xO.y) / dt};
+      const CD p11 = CD{t.x / lh, t.y / lh};
       cch[PHI11 * NH + i] = p11;
-      cch[MU11 * NH + i] = C{Lp * p11.x, Lp * p11.y};
-      const C yN = pshN[i], yO = pshO[i];
-      rhs[i] = C{-(b * yN.x + cc * yO.x) / dt, -(b * yN.y + cc * yO.y) / dt};
-      const R mM = -(R)sc.Mphi;
-      const C h1 = Hh[i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
-      const C p12 = C{(mM * h1.x) / lh, (mM * h1.y) / lh};
-      const C p13 = C{(mM * h2.x) / lh, (mM * h2.y) / lh};
-      const C p14 = C{(mM * h3.x) / lh, (mM * h3.y) / lh};
+      cch[MU11 * NH + i] = CD{Lp * p11.x, Lp * p11.y};
+      const CD yN = pshN[i], yO = pshO[i];
+      rhs[i] = CD{-(b * yN.x + cc * yO.x) / dt, -(b * yN.y + cc * yO.y) / dt};
+      const T mM = -(T)sc.Mphi;
+      const CD h1 = Hh[i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
+      const CD p12 = CD{(mM * h1.x) / lh, (mM * h1.y) / lh};
+      const CD p13 = CD{(mM * h2.x) / lh, (mM * h2.y) / lh};
+      const CD p14 = CD{(mM * h3.x) / lh, (mM * h3.y) / lh};
       cch[PHI12 * NH + i] = p12;
-      cch[MU12 * NH + i] = C{Lp * p12.x + h1.x, Lp * p12.y + h1.y};
+      cch[MU12 * NH + i] = CD{Lp * p12.x + h1.x, Lp * p12.y + h1.y};
       cch[PHI13 * NH + i] = p13;
       cch[PHI14 * NH + i] = p14;
-      cch[MU13 * NH + i] = C{Lp * p13.x + h2.x, Lp * p13.y + h2.y};
-      cch[MU14 * NH + i] = C{Lp * p14.x + h3.x, Lp * p14.y + h3.y};
+      cch[MU13 * NH + i] = CD{Lp * p13.x + h2.x, Lp * p13.y + h2.y};
+      cch[MU14 * NH + i] = CD{Lp * p14.x + h3.x, Lp * p14.y + h3.y};
     });
     // grad G -> M * grad G -> rfft -> rhs12 = sum_d D1_d * (.)   (Solvers.jl:99-105)
     const C* Gh = w_Hh + NH;
@@ -566,9 +585,9 @@ class Solver final : public ISolver {
       const R* gG = w_d4;
       const int ndl = nd;
       const bool var = nd == 2;
-      const R m0 = (R)c.M0_psi;
+      const T m0 = (T)c.M0_psi;
       foreach_n(s, (long long)nd * NP, [=] __device__(long long i) {
-        const R m = var ? M[i % NP] : m0;
+        const T m = var ? M[i % NP] : m0;
         w[i] = m * gG[i];
       });
       (void)ndl;
@@ -579,11 +598,11 @@ class Solver final : public ISolver {
       foreach_n(s, NH, [=] __device__(long long i) {
         int ix, iy, iz, f;
         ki(i, ix, iy, iz, f);
-        const R kk[3] = {(R)sc.kx[ix], (R)sc.ky[iy], (R)sc.kz[iz]};
-        C acc = C{R(0), R(0)};
+        const T kk[3] = {(T)sc.kx[ix], (T)sc.ky[iy], (T)sc.kz[iz]};
+        CD acc = CD{T(0), T(0)};
         for (int d = 0; d < ndd; ++d) {
-          const C a = wh[d * NH + i];
-          const C t = C{-kk[d] * a.y, kk[d] * a.x};
+          const CD a = wh[d * NH + i];
+          const CD t = CD{-kk[d] * a.y, kk[d] * a.x};
           acc = d == 0 ? t : acc + t;
         }
         rhs12[i] = acc;
@@ -620,13 +639,13 @@ class Solver final : public ISolver {
         const double k2 = kx * kx + ky * ky + kz * kz;
         const double Lphi = sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3;
         const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
-        const R Lp = (R)Lphi, lh = (R)lhs;
-        const C t = ah[i];
-        const C p21 = C{(-t.x) / lh, (-t.y) / lh};
+        const T Lp = (T)Lphi, lh = (T)lhs;
+        const CD t = ah[i];
+        const CD p21 = CD{(-t.x) / lh, (-t.y) / lh};
         cch[PHI21 * NH + i] = p21;
-        cch[MU21 * NH + i] = C{Lp * p21.x, Lp * p21.y};
-        const C t2 = ah[NH + i];
-        rhs21[i] = C{-t2.x, -t2.y};
+        cch[MU21 * NH + i] = CD{Lp * p21.x, Lp * p21.y};
+        const CD t2 = ah[NH + i];
+        rhs21[i] = CD{-t2.x, -t2.y};
       });
     }
     // psi solves
@@ -645,24 +664,24 @@ class Solver final : public ISolver {
         ki(i, ix, iy, iz, f);
         const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
         const double lapd = -(kx * kx + ky * ky + kz * kz);
-        const R lh = (R)((dsl->a / dsl->dt) - sc.M0 * sc.S4 * lapd);
-        const C a = r[i], b = r[NH + i], cc = r[2 * NH + i];
-        p11[i] = C{a.x / lh, a.y / lh};
-        p12[i] = C{b.x / lh, b.y / lh};
-        p21[i] = C{cc.x / lh, cc.y / lh};
+        const T lh = (T)((dsl->a / dsl->dt) - sc.M0 * sc.S4 * lapd);
+        const CD a = r[i], b = r[NH + i], cc = r[2 * NH + i];
+        p11[i] = CD{a.x / lh, a.y / lh};
+        p12[i] = CD{b.x / lh, b.y / lh};
+        p21[i] = CD{cc.x / lh, cc.y / lh};
       });
       kit[0] = kit[1] = kit[2] = 0;
     }
     // nu = S4 psi (+ G_hat for nu_12)
     {
       C* cch2 = cache;
-      const R S4 = (R)c.S4;
+      const T S4 = (T)c.S4;
       foreach_n(s, NH, [=] __device__(long long i) {
-        const C a = cch2[PSI11 * NH + i], b = cch2[PSI12 * NH + i], d = cch2[PSI21 * NH + i];
-        const C gh = Gh[i];
-        cch2[NU11 * NH + i] = C{S4 * a.x, S4 * a.y};
-        cch2[NU12 * NH + i] = C{S4 * b.x + gh.x, S4 * b.y + gh.y};
-        cch2[NU21 * NH + i] = C{S4 * d.x, S4 * d.y};
+        const CD a = cch2[PSI11 * NH + i], b = cch2[PSI12 * NH + i], d = cch2[PSI21 * NH + i];
+        const CD gh = Gh[i];
+        cch2[NU11 * NH + i] = CD{S4 * a.x, S4 * a.y};
+        cch2[NU12 * NH + i] = CD{S4 * b.x + gh.x, S4 * b.y + gh.y};
+        cch2[NU21 * NH + i] = CD{S4 * d.x, S4 * d.y};
       });
     }
   }
@@ -690,7 +709,7 @@ class Solver final : public ISolver {
     const DScal* dsl = ds;
     reduce_n<22>(s, NP, 1,
                  [=] __device__(long long i, int, double(&acc)[22]) {
-                   const R b = (R)dsl->b, cc = (R)dsl->c;
+                   const T b = (T)dsl->b, cc = (T)dsl->c;
                    const double tph = (double)(b * phN[i] + cc * phO[i]);
                    const double tps = (double)(b * psN[i] + cc * psO[i]);
                    const double h1 = H1[i], h2 = H2[i], h3 = H3[i], gg = G[i];
@@ -817,10 +836,10 @@ class Solver final : public ISolver {
     C* o = s3;
     const DScal* dsl = ds;
     foreach_n(s, NH, [=] __device__(long long i) {
-      const R R11 = (R)dsl->Rs[0], R21 = (R)dsl->Rs[1], R31 = (R)dsl->Rs[2];
-      const R R12 = (R)dsl->Rs[3], R22 = (R)dsl->Rs[4], R32 = (R)dsl->Rs[5];
+      const T R11 = (T)dsl->Rs[0], R21 = (T)dsl->Rs[1], R31 = (T)dsl->Rs[2];
+      const T R12 = (T)dsl->Rs[3], R22 = (T)dsl->Rs[4], R32 = (T)dsl->Rs[5];
       auto at = [&](int f) { return k[f * NH + i]; };
-      auto ax = [](C a, R r, C b) { return C{a.x + r * b.x, a.y + r * b.y}; };
+      auto ax = [](C a, R r, C b) { return CD{a.x + r * b.x, a.y + r * b.y}; };
       o[0 * NH + i] = ax(ax(ax(at(PHI11), R11, at(PHI12)), R21, at(PHI13)), R31, at(PHI14));
       o[1 * NH + i] = ax(ax(ax(at(MU11), R11, at(MU12)), R21, at(MU13)), R31, at(MU14));
       o[2 * NH + i] = ax(at(PSI11), R31, at(PSI12));
@@ -853,7 +872,7 @@ class Solver final : public ISolver {
     R *conv = w_conv, *coupl = w_coupl, *rhs2 = w_rhs2;
     const int ndd = nd, Nl = N;
     const long long Pl = P;
-    const R lam = (R)c.lamda;
+    const T lam = (T)c.lamda;
     foreach_n(s, P, [=] __device__(long long i) {
       for (int cc = 0; cc < ndd; ++cc) {
         R a = us[i] * du[(cc * ndd) * Pl + i];
@@ -861,10 +880,10 @@ class Solver final : public ISolver {
         conv[cc * Pl + i] = a;
       }
       for (int d = 0; d < ndd; ++d) {
-        R sum = R(0);
+        R sum = T(0);
         for (int n = 0; n < Nl; ++n) {
           const long long k = (long long)n * Pl + i;
-          const R t = phs[k] * dmn[d * NP + k] + pss[k] * dmn[(ndd + d) * NP + k];
+          const T t = phs[k] * dmn[d * NP + k] + pss[k] * dmn[(ndd + d) * NP + k];
           sum = n == 0 ? t : sum + t;
         }
         coupl[d * Pl + i] = sum;
@@ -885,16 +904,16 @@ class Solver final : public ISolver {
       const long long sp = i % Hl;
       const double kk[3] = {sc.kx[ix], sc.ky[iy], sc.kz[iz]};
       const double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
-      const R lh = (R)((dsl->a / dsl->dt) - sc.eta * (-k2));
-      const R b = (R)dsl->b, cc = (R)dsl->c, dt = (R)dsl->dt;
-      const C xN = uhN[i], xO = uhO[i];
-      const C pp = phat[sp];
-      const R kd = (R)kk[d];
-      const C dp = C{-kd * pp.y, kd * pp.x};
-      const C r1 = C{-(b * xN.x + cc * xO.x) / dt - dp.x, -(b * xN.y + cc * xO.y) / dt - dp.y};
-      uth[i] = C{r1.x / lh, r1.y / lh};
-      const C r2 = r2h[i];
-      uth[ndd * Hl + i] = C{r2.x / lh, r2.y / lh};
+      const T lh = (T)((dsl->a / dsl->dt) - sc.eta * (-k2));
+      const T b = (T)dsl->b, cc = (T)dsl->c, dt = (T)dsl->dt;
+      const CD xN = uhN[i], xO = uhO[i];
+      const CD pp = phat[sp];
+      const T kd = (T)kk[d];
+      const CD dp = CD{-kd * pp.y, kd * pp.x};
+      const CD r1 = CD{-(b * xN.x + cc * xO.x) / dt - dp.x, -(b * xN.y + cc * xO.y) / dt - dp.y};
+      uth[i] = CD{r1.x / lh, r1.y / lh};
+      const CD r2 = r2h[i];
+      uth[ndd * Hl + i] = CD{r2.x / lh, r2.y / lh};
     });
     irfft({job(w_uth, w_ut12, 2 * nd)});
   }
@@ -973,12 +992,12 @@ class Solver final : public ISolver {
     const int ndd = nd, Nl = N;
     const long long Hl = H;
     foreach_n(s, H, [=] __device__(long long i) {
-      const R Q = (R)dsl->Q;
+      const T Q = (T)dsl->Q;
       for (int n = 0; n < Nl; ++n) {
         const long long k = (long long)n * Hl + i;
         auto cmb = [&](int f) {
-          const C a = o[f * NH + k], b = o[(f + 4) * NH + k];
-          return C{a.x + Q * b.x, a.y + Q * b.y};
+          const CD a = o[f * NH + k], b = o[(f + 4) * NH + k];
+          return CD{a.x + Q * b.x, a.y + Q * b.y};
         };
         phN[k] = cmb(0);
         muN[k] = cmb(1);
@@ -989,27 +1008,27 @@ class Solver final : public ISolver {
       ki(i, ix, iy, iz, f);
       const double kk[3] = {sc.kx[ix], sc.ky[iy], sc.kz[iz]};
       C ut[3];
-      C div = C{R(0), R(0)};
+      CD div = CD{T(0), T(0)};
       for (int d = 0; d < ndd; ++d) {
-        const C a = uth[d * Hl + i], b = uth[(ndd + d) * Hl + i];
-        ut[d] = C{a.x + Q * b.x, a.y + Q * b.y};
-        const R kd = (R)kk[d];
-        const C t = C{-kd * ut[d].y, kd * ut[d].x};
+        const CD a = uth[d * Hl + i], b = uth[(ndd + d) * Hl + i];
+        ut[d] = CD{a.x + Q * b.x, a.y + Q * b.y};
+        const T kd = (T)kk[d];
+        const CD t = CD{-kd * ut[d].y, kd * ut[d].x};
         div = d == 0 ? t : div + t;
       }
       const double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
-      const R lapv = (R)(-k2);
-      const R fac = (R)(dsl->a / dsl->dt);
-      C dp = C{(fac * div.x) / lapv, (fac * div.y) / lapv};
-      if (i == 0) dp = C{R(0), R(0)};
+      const T lapv = (T)(-k2);
+      const T fac = (T)(dsl->a / dsl->dt);
+      CD dp = CD{(fac * div.x) / lapv, (fac * div.y) / lapv};
+      if (i == 0) dp = CD{T(0), T(0)};
       const double dta = dsl->dt / dsl->a;
       for (int d = 0; d < ndd; ++d) {
-        const R kd = (R)(dta * kk[d]);
-        uN[d * Hl + i] = C{ut[d].x - (-kd * dp.y), ut[d].y - kd * dp.x};
+        const T kd = (T)(dta * kk[d]);
+        uN[d * Hl + i] = CD{ut[d].x - (-kd * dp.y), ut[d].y - kd * dp.x};
       }
-      const R eta = (R)sc.eta;
-      const C po = pOld[i];
-      pN[i] = C{(po.x + dp.x) - eta * div.x, (po.y + dp.y) - eta * div.y};
+      const T eta = (T)sc.eta;
+      const CD po = pOld[i];
+      pN[i] = CD{(po.x + dp.x) - eta * div.x, (po.y + dp.y) - eta * div.y};
     });
     DStateScal* stw = dstate + inew;
     DScal* dsw = ds;
@@ -1053,10 +1072,10 @@ class Solver final : public ISolver {
     reduce_n<1>(s, P, N,
                 [=] __device__(long long i, int n, double(&acc)[1]) {
                   const long long k = (long long)n * Pl + i;
-                  const R p = phi[k], gx = gxy[k], gy = gxy[NP + k];
-                  const R eps = (R)q.eps;
-                  const R gg = R(0.25) * (p * p - R(1)) * (p * p - R(1));
-                  const R fs = (R)(3.0 * SQ2 / 4.0) * (gg / eps) + eps * (gx * gx + gy * gy) / R(2);
+                  const T p = phi[k], gx = gxy[k], gy = gxy[NP + k];
+                  const T eps = (T)q.eps;
+                  const T gg = T(0.25) * (p * p - T(1)) * (p * p - T(1));
+                  const T fs = (T)(3.0 * SQ2 / 4.0) * (gg / eps) + eps * (gx * gx + gy * gy) / T(2);
                   acc[0] += (double)fs;
                 },
                 partials, redA);
@@ -1066,7 +1085,7 @@ class Solver final : public ISolver {
                  [=] __device__(long long i, int, double(&acc)[14]) {
                    if (i < nu) {
                      const double a = u[i], b = uo[i];
-                     const double st = (double)(R(2) * u[i] - uo[i]);
+                     const double st = (double)(T(2) * u[i] - uo[i]);
                      acc[0] += a * a;
                      acc[1] += st * st;
                      (void)b;
@@ -1087,8 +1106,8 @@ class Solver final : public ISolver {
                        acc[8] += cc * cc;
                        acc[9] += d * d;
                      }
-                     const double p = phi[i], pst = (double)(R(2) * phi[i] - phO[i]);
-                     const double y = ps[i], yst = (double)(R(2) * ps[i] - psO[i]);
+                     const double p = phi[i], pst = (double)(T(2) * phi[i] - phO[i]);
+                     const double y = ps[i], yst = (double)(T(2) * ps[i] - psO[i]);
                      acc[10] += p * p;
                      acc[11] += pst * pst;
                      acc[12] += y * y;
@@ -1226,7 +1245,7 @@ class Solver final : public ISolver {
   int init_sav() override {
     const long long NH = (long long)N * H, NP = (long long)N * P;
     set_bdf(1.0, -1.0, 0.0, present_dt > 0 ? present_dt : c.dt);
-    functionals(fphys(ip, F_PHI), fphys(ip, F_PSI), true);
+    functionals(fphys(ip, F_PHI), fphys(ip, F_PSI), true, sizeof(R) == 8 ? nullptr : fhalf(ip, F_PHI));
     int rc = check_flags();
     if (rc) return rc;
     rfft({job(w_H, w_Hh, 4 * N)});
@@ -1244,13 +1263,13 @@ class Solver final : public ISolver {
       ki(i, ix, iy, iz, f);
       const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
       const double k2 = kx * kx + ky * ky + kz * kz;
-      const R Lp = (R)(sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3);
-      const R r1 = (R)dsl->R1s, r2 = (R)dsl->R2s, r3 = (R)dsl->R3s;
-      const C p = phh[i], h1 = Hh[i], g = Hh[NH + i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
-      muh[i] = C{Lp * p.x + r1 * h1.x + r2 * h2.x + r3 * h3.x, Lp * p.y + r1 * h1.y + r2 * h2.y + r3 * h3.y};
-      const C y = psh[i];
-      const R S4 = (R)sc.S4;
-      nuh[i] = C{S4 * y.x + r3 * g.x, S4 * y.y + r3 * g.y};
+      const T Lp = (T)(sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3);
+      const T r1 = (T)dsl->R1s, r2 = (T)dsl->R2s, r3 = (T)dsl->R3s;
+      const CD p = phh[i], h1 = Hh[i], g = Hh[NH + i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
+      muh[i] = CD{Lp * p.x + r1 * h1.x + r2 * h2.x + r3 * h3.x, Lp * p.y + r1 * h1.y + r2 * h2.y + r3 * h3.y};
+      const CD y = psh[i];
+      const T S4 = (T)sc.S4;
+      nuh[i] = CD{S4 * y.x + r3 * g.x, S4 * y.y + r3 * g.y};
     });
     irfft({job(muh, fphys(ip, F_MU), N), job(nuh, fphys(ip, F_NU), N)});
     DStateScal* stw = dstate + ip;
@@ -1447,8 +1466,8 @@ class Solver final : public ISolver {
     const R* phs = w_phis;
     const double M0 = c.M0_psi;
     foreach_n(s, NP, [=] __device__(long long i) {
-      const R t = phs[i] * phs[i] - R(1.0);
-      M[i] = R(0.1) * (R(1.0) - (R)M0 * (t * t));
+      const T t = phs[i] * phs[i] - T(1.0);
+      M[i] = T(0.1) * (T(1.0) - (T)M0 * (t * t));
     });
     max_reduce(M, NP, &ds->Mmax);
     C* dr = w_rhs;

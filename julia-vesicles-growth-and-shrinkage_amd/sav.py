@@ -318,6 +318,18 @@ class Operators:
         self._check(self.lib.sav_rfft(self.h, _ptr(u), _ptr(out), cnt), "rfft")
         return out
 
+    def rfft_accurate(self, u: np.ndarray) -> np.ndarray:
+        """Spectra of initial data: the device transform for an FP64 handle; for an
+        FP32 handle the FP64 transform is taken on the host (numpy), because the
+        FP32 FFT roundoff floor at high k is amplified by the S1 k^4 stabiliser.
+        Used to prepare inputs only, never inside a time step."""
+        if self.precision == SAV_PREC_FP64:
+            return self.rfft(u)
+        u = np.asarray(u, dtype=np.float64)
+        nd = self.conf.ndim
+        axes = tuple(range(nd - 1, -1, -1))      # Julia (x, y[, z], ...): x is the halved axis
+        return np.asfortranarray(np.fft.rfftn(u, axes=axes))
+
     def irfft(self, uh: np.ndarray) -> np.ndarray:
         uh = _f(uh, np.complex128)
         cnt = self._count(uh, half=True)
@@ -453,8 +465,8 @@ def state_from_physical(conf: Config, ops: Optional[Operators], phi: np.ndarray,
     st.psi[...] = psi
     if ops is None:
         raise ValueError("state_from_physical needs the device operators for the transforms")
-    st.phi_hat[...] = ops.rfft(st.phi)
-    st.psi_hat[...] = ops.rfft(st.psi)
+    st.phi_hat[...] = ops.rfft_accurate(st.phi)
+    st.psi_hat[...] = ops.rfft_accurate(st.psi)
     st.Q = st.R1 = st.R2 = st.R3 = 1.0
     return st
 

@@ -15,8 +15,8 @@ def device_state(jv, ops, conf, phi, psi):
     st = jv.FieldState.zeros(conf)
     st.phi[...] = phi.T
     st.psi[...] = psi.T
-    st.phi_hat[...] = ops.rfft(st.phi)
-    st.psi_hat[...] = ops.rfft(st.psi)
+    st.phi_hat[...] = ops.rfft_accurate(st.phi)
+    st.psi_hat[...] = ops.rfft_accurate(st.psi)
     st.Q = st.R1 = st.R2 = st.R3 = 1.0
     return st
 
