@@ -71,6 +71,47 @@ def run_case(jv, wl, precision, steps, warmup, timed_kernel=0):
     return res, st
 
 
+def run_case_learned(jv, wl, precision, steps, warmup, collect_steps=4, train_iters=1000):
+    """Learned psi preconditioner: windows logged from `collect_steps` diagonal steps,
+    Adam training on the GPU, then `warmup` + `steps` steps with the network (timed)."""
+    conf = wl.conf
+    ops = _setup(jv, wl, precision)
+    ops.precond_collect(True, patches_per_snapshot=32, seed=17)
+    t = np.arange(collect_steps + warmup + steps + 1) * conf.dt
+    rc, E0, A0, kit_diag = ops.run(t[: collect_steps + 1], 1)
+    ops.precond_collect(False)
+    nwin = ops.precond_samples()
+    _sync()
+    t0 = time.perf_counter()
+    l0, l1 = ops.precond_train(iters=train_iters, batch=32, lr=2e-3, seed=5)
+    _sync()
+    train_s = time.perf_counter() - t0
+    ops.set_precond(jv.SAV_PRECOND_LEARNED)
+    if warmup:
+        ops.run(t[collect_steps: collect_steps + warmup + 1], collect_steps + 1)
+    _sync()
+    t0 = time.perf_counter()
+    fb0 = ops.precond_fallbacks()
+    rc, E, A, kit = ops.run(t[collect_steps + warmup:], collect_steps + warmup + 1)
+    _sync()
+    t1 = time.perf_counter()
+    fallbacks = ops.precond_fallbacks() - fb0
+    # the same form with the analytic first-order stencil instead of the network
+    ops.set_precond(jv.SAV_PRECOND_STENCIL)
+    n1 = collect_steps + warmup + steps + 1
+    t2 = time.perf_counter()
+    _, _, _, kit_s = ops.run(np.arange(n1 - 1, n1 + steps) * conf.dt, n1)
+    _sync()
+    t3 = time.perf_counter()
+    ops.close()
+    return dict(rc=rc, steps_per_s=steps / (t1 - t0), ms_per_step=1e3 * (t1 - t0) / steps,
+                krylov_iters=kit, psi_iters_per_step=kit / steps, diag_fallback_solves=fallbacks,
+                stencil_psi_iters_per_step=kit_s / steps, stencil_steps_per_s=steps / (t3 - t2),
+                diag_psi_iters_per_step_during_collection=kit_diag / collect_steps,
+                windows=nwin, train_iters=train_iters, train_seconds=round(train_s, 3),
+                loss_first=l0, loss_last=l1, energy_monotone=bool(np.all(np.diff(E) <= 0)))
+
+
 def cpu_baseline(jv, wl):
     """The FP64 CPU oracle on the host cores: one BDF1 step of the same workload
     after the SAV initialisation (a bounded sample of the timed work)."""
@@ -96,6 +137,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--only", choices=["all", "2d", "3d"], default="all")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -114,6 +156,8 @@ def main():
                for k in ("phi", "psi", "mu", "nu")}
         err["u"] = float(np.max(np.abs(s32.u - s64.u)) / max(np.max(np.abs(s64.u)), 1e-300))
         out["2d"] = dict(fp32=r32, fp64=r64, fp32_vs_fp64_max_rel_err=err)
+        if not a.no_learned:
+            out["2d"]["fp32_learned"] = run_case_learned(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup)
     if a.only in ("all", "3d"):
         wl4 = jv.workload_c4(n=128, steps=a.steps)
         q32, t32 = run_case(jv, wl4, jv.SAV_PREC_FP32, a.steps, a.warmup)
@@ -151,6 +195,15 @@ def main():
         line["psi_iters_per_step"] = {"fp32": out["2d"]["fp32"]["krylov_iters"] / a.steps,
                                       "fp64": out["2d"]["fp64"]["krylov_iters"] / a.steps}
         line["fp64_steps_per_s_1024"] = round(out["2d"]["fp64"]["steps_per_s"], 3)
+        if "fp32_learned" in out["2d"]:
+            L = out["2d"]["fp32_learned"]
+            line["learned_precond_1024"] = {
+                "steps_per_s": round(L["steps_per_s"], 3), "psi_iters_per_step": L["psi_iters_per_step"],
+                "analytic_stencil_steps_per_s": round(L["stencil_steps_per_s"], 3),
+                "analytic_stencil_psi_iters_per_step": L["stencil_psi_iters_per_step"],
+                "diag_psi_iters_per_step": out["2d"]["fp32"]["krylov_iters"] / a.steps,
+                "diag_fallback_solves": L["diag_fallback_solves"], "windows": L["windows"], "train_iters": L["train_iters"], "train_seconds": L["train_seconds"],
+                "loss_first": L["loss_first"], "loss_last": L["loss_last"]}
         line["fp32_vs_fp64_max_rel_err_1024"] = out["2d"]["fp32_vs_fp64_max_rel_err"]
         line["energy_monotone_1024"] = out["2d"]["fp64"]["energy_monotone"]
     if "3d" in out:

@@ -41,6 +41,7 @@ extern "C" {
 
 #define SAV_PRECOND_DIAG 0
 #define SAV_PRECOND_LEARNED 1
+#define SAV_PRECOND_STENCIL 2 /* first-order Neumann stencil in place of the network (see below) */
 
 #define SAV_PRESENT 0
 #define SAV_OLD 1
@@ -156,6 +157,61 @@ int sav_last_elapsed_ms(sav_handle_t* h, double* ms);
  * bytes per launch (reads + writes of whole vectors). */
 int sav_kernel_timing(sav_handle_t* h, int kernel_id);
 int sav_kernel_timing_get(sav_handle_t* h, int64_t* launches, double* total_ms, double* bytes_per_launch);
+
+/*
+ * Learned psi preconditioner (no reference counterpart; it replaces the
+ * `prec_func!`/`P_inv_diag` plug-in of src/Solvers.jl:57-65 in the 2D psi solve).
+ *   z = P ( r + F( g ) ),   g(x) = kappa sum_tap w_tap(x) Re F^{-1}(P r)(x + d(tap))
+ * P the diagonal preconditioner, F the full 2D FFT, kappa = S4 max(M_psi) / (dx dy),
+ * and w(x) = (1 - M_psi(x) / max M_psi) y(x), y(x) the nine weights of a 3x3 stencil at
+ * pixel x (tap = di*3 + dj, offset (di-1, dj-1), periodic), produced once per mobility
+ * field by a periodic 3x3
+ * convolutional network 2 -> 16 -> 16 -> 16 -> 9 with GELU on the inputs
+ * (4 (M_psi / max M_psi - 0.75), 1): BF16 weights and activations on the MFMA units,
+ * FP32 accumulation.  The preconditioner is linear in r, as BiCGSTAB needs.  The
+ * psi solve is then a flexible right-preconditioned BiCGSTAB that stops on the same
+ * diagonally preconditioned residual as the reference's left-preconditioned solve
+ * (||P r|| <= atol + rtol ||P b||).  2D only; needs Nx % 32 == 0 and Ny % 16 == 0
+ * (else SAV_E_UNSUPPORTED).
+ *
+ * Parameters: a flat float vector of SAV_NET_NPARAM values, for each layer
+ * W[cout][cin][3][3] (cross-correlation, periodic) then b[cout].
+ */
+#define SAV_NET_NPARAM 6249
+
+/* Selects the psi preconditioner of the following solves: SAV_PRECOND_DIAG,
+ * SAV_PRECOND_LEARNED, or SAV_PRECOND_STENCIL, the same form with the analytic
+ * first-order stencil g = -S4 div((max M - M) grad v) (5-point, face-averaged
+ * coefficients) in place of the network: z = P (2 r - A P r) up to discretisation. */
+int sav_set_precond(sav_handle_t* h, int precond);
+/* He-uniform hidden layers from a SplitMix64 seed, zero output layer (the network then adds nothing). */
+int sav_precond_init(sav_handle_t* h, uint64_t seed);
+/* Copies the network parameters in (load) or out (save); n must be SAV_NET_NPARAM. */
+int sav_precond_load(sav_handle_t* h, const float* params, int n);
+int sav_precond_save(sav_handle_t* h, float* params, int n);
+/* on != 0: the following psi solves run with the diagonal preconditioner and log training
+ * pairs: for every iterate x_k of a solve, e = x* - x_k, r = A e, and patches_per_snapshot
+ * random 64 x 64 windows of the network input, v = Re F^{-1}(P r) / s and the target
+ * Re F^{-1}(P^{-1} e - r) / (kappa s), s = rms of F^{-1}(P r).
+ * The store is cleared when collection starts and holds up to 24576 windows.
+ * on == 0 stops logging (the store is kept for sav_precond_train). */
+int sav_precond_collect(sav_handle_t* h, int on, int patches_per_snapshot, uint64_t seed);
+/* Windows in the store, and the number of solves with the learned (or stencil)
+ * preconditioner that did not reach the tolerance within itmax and were therefore
+ * repeated from x0 = 0 with the diagonal preconditioner (either pointer may be NULL). */
+int sav_precond_samples(sav_handle_t* h, int64_t* n_windows, int64_t* fallbacks);
+/* Adam (beta 0.9/0.999) on the masked MSE (4-pixel border excluded) of the stencil
+ * prediction sum_tap w_tap v(x + d(tap)) against the target, over random batches of the
+ * store, learning rate decaying from lr to lr/10 (cosine).  loss_first/loss_last are the
+ * mean losses of the first and last 10 iterations. */
+int sav_precond_train(sav_handle_t* h, int iters, int batch, double lr, uint64_t seed, double* loss_first,
+                      double* loss_last);
+/* Test hooks: the network on host images x[nimg][H][W][2] (channel fastest, rounded to
+ * BF16) -> y[nimg][H][W][9]; and the masked MSE of y against t[nimg][H][W][9] with the
+ * parameter gradient (flat layout).  H % 16 == 0, W % 32 == 0. */
+int sav_net_forward(sav_handle_t* h, const float* x, int nimg, int H, int W, float* y);
+int sav_net_grad(sav_handle_t* h, const float* x, const float* t, int nimg, int H, int W, int border,
+                 float* grads, double* loss);
 
 #ifdef __cplusplus
 }

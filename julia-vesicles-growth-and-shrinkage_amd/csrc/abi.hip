@@ -41,7 +41,10 @@ int sav_create(const sav_config_t* cfg, sav_handle_t** out) {
     if (!cfg->psi_in_v || !cfg->psi_out_v) throw std::invalid_argument("psi_in_v/psi_out_v required");
     if (cfg->precision != SAV_PREC_FP64 && cfg->precision != SAV_PREC_FP32)
       throw std::invalid_argument("unknown precision");
-    if (cfg->precond != SAV_PRECOND_DIAG) throw std::invalid_argument("only the diagonal preconditioner is built");
+    if (cfg->precond != SAV_PRECOND_DIAG && cfg->precond != SAV_PRECOND_LEARNED)
+      throw std::invalid_argument("unknown preconditioner");
+    if (cfg->precond == SAV_PRECOND_LEARNED && (cfg->ndim != 2 || cfg->Nx % 32 != 0 || cfg->Ny % 16 != 0))
+      throw std::invalid_argument("the learned preconditioner needs 2D, Nx % 32 == 0 and Ny % 16 == 0");
     sav::HostCfg hc;
     hc.c = *cfg;
     hc.psi_in.assign(cfg->psi_in_v, cfg->psi_in_v + cfg->N);
@@ -50,6 +53,7 @@ int sav_create(const sav_config_t* cfg, sav_handle_t** out) {
     auto h = new sav_handle();
     try {
       h->s = sav::make_solver(hc);
+      if (cfg->precond == SAV_PRECOND_LEARNED) h->s->set_precond(SAV_PRECOND_LEARNED);
     } catch (...) {
       delete h;
       throw;
@@ -192,6 +196,57 @@ int sav_last_elapsed_ms(sav_handle_t* h, double* ms) {
   if (!h || !ms) return SAV_E_ARG;
   *ms = h->s->last_elapsed_ms();
   return SAV_OK;
+}
+
+
+int sav_set_precond(sav_handle_t* h, int precond) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->set_precond(precond));
+}
+
+int sav_precond_init(sav_handle_t* h, uint64_t seed) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->precond_init(seed));
+}
+
+int sav_precond_load(sav_handle_t* h, const float* params, int n) {
+  if (!h || !params) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->precond_params(true, const_cast<float*>(params), n));
+}
+
+int sav_precond_save(sav_handle_t* h, float* params, int n) {
+  if (!h || !params) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->precond_params(false, params, n));
+}
+
+int sav_precond_collect(sav_handle_t* h, int on, int patches_per_snapshot, uint64_t seed) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->precond_collect(on, patches_per_snapshot, seed));
+}
+
+int sav_precond_samples(sav_handle_t* h, int64_t* n_windows, int64_t* fallbacks) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, if (n_windows) *n_windows = h->s->precond_samples();
+            if (fallbacks) *fallbacks = h->s->precond_fallbacks(); return SAV_OK);
+}
+
+int sav_precond_train(sav_handle_t* h, int iters, int batch, double lr, uint64_t seed, double* loss_first,
+                      double* loss_last) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->precond_train(iters, batch, lr, seed, loss_first, loss_last));
+}
+
+int sav_net_forward(sav_handle_t* h, const float* x, int nimg, int H, int W, float* y) {
+  if (!h || !x || !y || nimg < 1 || H < 16 || W < 32 || H % 16 || W % 32) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->net_forward(x, nimg, H, W, y));
+}
+
+int sav_net_grad(sav_handle_t* h, const float* x, const float* t, int nimg, int H, int W, int border, float* grads,
+                 double* loss) {
+  if (!h || !x || !t || !grads || nimg < 1 || H < 16 || W < 32 || H % 16 || W % 32 || border < 0 ||
+      2 * border >= H || 2 * border >= W)
+    return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->net_grad(x, t, nimg, H, W, border, grads, loss));
 }
 
 }  // extern "C"

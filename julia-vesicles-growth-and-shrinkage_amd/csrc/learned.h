@@ -10,6 +10,25 @@ namespace sav {
 
 typedef uint16_t bf16_t;
 
+// Training windows of the preconditioner are PATCH x PATCH pixels; the loss skips
+// PATCH_BORDER pixels at each edge (the receptive-field radius of four 3x3 layers).
+constexpr int PATCH = 64;
+constexpr int PATCH_BORDER = 4;
+
+__host__ __device__ __forceinline__ float bf2f(bf16_t h) {
+  uint32_t u = ((uint32_t)h) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+// FP32 -> BF16, round to nearest even (finite inputs)
+__host__ __device__ __forceinline__ bf16_t f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
 struct NetLayerDev {
   int cin, cout, kp, kpt;   // channels, padded K of the forward and of the transposed operand
   float* W;                 // master weights [cout][9*cin]  (k = tap*cin + c, tap = di*3 + dj)
@@ -24,8 +43,10 @@ struct NetLayerDev {
 class Net {
  public:
   static constexpr int NL = 4;
-  static constexpr int CH[NL + 1] = {2, 16, 16, 16, 1};
-  static constexpr int NPARAM = 2 * 16 * 9 + 16 + 16 * 16 * 9 + 16 + 16 * 16 * 9 + 16 + 16 * 9 + 1;
+  // inputs: mobility channel and a constant channel; outputs: the 9 weights of a 3x3 stencil
+  static constexpr int CH[NL + 1] = {2, 16, 16, 16, 9};
+  static constexpr int COUT = 9;
+  static constexpr int NPARAM = 2 * 16 * 9 + 16 + 16 * 16 * 9 + 16 + 16 * 16 * 9 + 16 + 9 * 16 * 9 + 9;
 
   explicit Net(hipStream_t s);
   ~Net();
@@ -33,13 +54,21 @@ class Net {
   // flat parameter vector: for each layer W[cout][cin][3][3] then b[cout]
   void set_params(const float* host);
   void get_params(float* host);
+  // He-uniform hidden layers from a SplitMix64 stream, zero output layer (so the
+  // untrained network adds nothing to the diagonal preconditioner), zero biases.
+  void init_random(uint64_t seed);
 
-  // y[img][H][W] (fp32) = net(x), x = bf16 NHWC [img][H][W][2].  Activations are
-  // kept for a following backward() when `keep` is set.
+  // y[img][H][W][COUT] (fp32) = net(x), x = bf16 NHWC [img][H][W][2].  Activations are
+  // kept for a following backward pass when `keep` is set.
   void forward(const bf16_t* x, int nimg, int H, int W, float* y, bool keep);
-  // Masked MSE against target (interior: border pixels excluded), gradients into gW/gb,
-  // returns the loss (one device->host copy).
+  // Masked MSE of the raw outputs against target[img][H][W][COUT] (test hook); gradients
+  // into gW/gb; returns the loss (one device->host copy).
   double loss_backward(const bf16_t* x, const float* target, int nimg, int H, int W, int border);
+  // Masked MSE of the stencil prediction D[p] sum_tap y[p][tap] * v[p + d(tap)] (periodic
+  // in each image, D = 0.25 - x[p][0] / 4) against target[img][H][W]; gradients into
+  // gW/gb; returns the loss.
+  double loss_backward_stencil(const bf16_t* x, const float* v, const float* target, int nimg, int H, int W,
+                               int border);
   void adam_step(float lr, float beta1, float beta2, float eps);
   // gradient vector in the flat parameter layout (test hook)
   void get_grads(float* host);
@@ -51,15 +80,17 @@ class Net {
  private:
   void ensure(long long npix);
   void pack_operands();
+  double backward(const bf16_t* x, int nimg, int H, int W);
   long long cap = 0;
   bf16_t* act[NL] = {};   // activations a_1..a_3 (16 ch), index 0 unused
   bf16_t* zb[NL] = {};    // pre-activations z_1..z_3
   bf16_t* g16a = nullptr; // gradient buffers (16 ch)
   bf16_t* g16b = nullptr;
-  bf16_t* g1 = nullptr;   // last-layer gradient (1 ch)
+  bf16_t* glast = nullptr; // output-layer gradient (COUT ch)
   float* yb = nullptr;
   float* partials = nullptr;
   double* dloss = nullptr;
+  int nb_loss = 0;
   std::vector<void*> allocs;
   template <class T>
   T* alloc(size_t n);

@@ -20,12 +20,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CTX = 32, CTY = 16;  // output tile of one workgroup (4 waves)
 
-__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
-__device__ __forceinline__ bf16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return (bf16_t)(u >> 16);
-}
 __device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_d(float z) {
   return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * expf(-0.5f * z * z) * 0.39894228040143268f;
@@ -125,7 +119,9 @@ __global__ __launch_bounds__(256) void k_conv(const bf16_t* __restrict__ in, int
       if (zout) *reinterpret_cast<uint2*>(zout + pix * 16 + 4 * g) = zv;
       *reinterpret_cast<uint2*>(aout + pix * 16 + 4 * g) = av;
     } else if constexpr (EPI == 1) {
-      if (g == 0) yout[pix] = acc[0] + bo[0];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (4 * g + i < COUT) yout[pix * COUT + 4 * g + i] = acc[i] + bo[i];
     } else {
       const uint2 zp = *reinterpret_cast<const uint2*>(zprev + pix * 16 + 4 * g);
       const uint16_t zs[4] = {(uint16_t)(zp.x & 0xFFFF), (uint16_t)(zp.x >> 16), (uint16_t)(zp.y & 0xFFFF),
@@ -226,19 +222,60 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ 
   }
 }
 
-// Masked MSE: loss = sum_interior (y - t)^2 / n_int ; g = 2 (y - t) / n_int on the interior, 0 outside.
+// Masked MSE over C channels: loss = sum_interior (y - t)^2 / n_int, n_int = interior
+// pixels * C; g = 2 (y - t) / n_int on the interior, 0 outside.
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t, int nimg,
-                                             int H, int W, int border, bf16_t* __restrict__ g, double* __restrict__ lp) {
+                                             int H, int W, int C, int border, bf16_t* __restrict__ g,
+                                             double* __restrict__ lp) {
   __shared__ double sc[4];
-  const long long n = (long long)nimg * H * W;
-  const double nint = (double)nimg * (H - 2 * border) * (W - 2 * border);
+  const long long n = (long long)nimg * H * W * C;
+  const double nint = (double)nimg * (H - 2 * border) * (W - 2 * border) * C;
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-    const int x = (int)(i % W), yy = (int)((i / W) % H);
+    const long long p = i / C;
+    const int x = (int)(p % W), yy = (int)((p / W) % H);
     const bool in = x >= border && x < W - border && yy >= border && yy < H - border;
     const float d = y[i] - t[i];
     acc += in ? (double)d * (double)d : 0.0;
     g[i] = f2bf(in ? (float)(2.0 * d / nint) : 0.f);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) lp[blockIdx.x] = (sc[0] + sc[1] + sc[2] + sc[3]) / nint;
+}
+
+// Stencil loss: pred[p] = D[p] sum_tap w[p][tap] v[p + d(tap)] (tap = di*3 + dj, offset
+// (di - 1, dj - 1), periodic in each image), D = 0.25 - x[p][0] / 4 = 1 - M / M_max from
+// the mobility input channel; loss = sum_interior (pred - t)^2 / n_int;
+// g[p][tap] = 2 (pred - t) D[p] v[p + d(tap)] / n_int on the interior.
+__global__ __launch_bounds__(256) void k_stencil_mse(const bf16_t* __restrict__ x, const float* __restrict__ w,
+                                                     const float* __restrict__ v, const float* __restrict__ t,
+                                                     int nimg, int H, int W, int border, bf16_t* __restrict__ g,
+                                                     double* __restrict__ lp) {
+  __shared__ double sc[4];
+  const long long n = (long long)nimg * H * W;
+  const double nint = (double)nimg * (H - 2 * border) * (W - 2 * border);
+  double acc = 0.0;
+  for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
+    const int xc = (int)(p % W), yy = (int)((p / W) % H);
+    const long long img = p / ((long long)H * W);
+    const bool in = xc >= border && xc < W - border && yy >= border && yy < H - border;
+    float vn[9];
+    double pred = 0.0;
+    const double Dp = 0.25 - 0.25 * (double)bf2f(x[2 * p]);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int sy = (yy + tap / 3 - 1 + H) % H, sx = (xc + tap % 3 - 1 + W) % W;
+      vn[tap] = v[(img * H + sy) * W + sx];
+      pred += (double)w[p * 9 + tap] * (double)vn[tap];
+    }
+    pred *= Dp;
+    const double d = pred - (double)t[p];
+    acc += in ? d * d : 0.0;
+    const double gs = in ? 2.0 * d * Dp / nint : 0.0;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) g[p * 9 + tap] = f2bf((float)(gs * vn[tap]));
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = acc;
@@ -334,8 +371,8 @@ void Net::ensure(long long npix) {
   }
   g16a = alloc<bf16_t>(npix * 16);
   g16b = alloc<bf16_t>(npix * 16);
-  g1 = alloc<bf16_t>(npix);
-  yb = alloc<float>(npix);
+  glast = alloc<bf16_t>(npix * COUT);
+  yb = alloc<float>(npix * COUT);
   const long long nblk = (npix + CTX * CTY - 1) / (CTX * CTY);
   partials = alloc<float>(nblk * (16 * 144 + 16));
 }
@@ -371,6 +408,29 @@ void Net::set_params(const float* h) {
   t_adam = 0;
   pack_operands();
   SAV_HIP(hipStreamSynchronize(st));
+}
+
+void Net::init_random(uint64_t seed) {
+  uint64_t z = seed;
+  auto next = [&z]() {  // SplitMix64
+    z += 0x9E3779B97F4A7C15ULL;
+    uint64_t x = z;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+  };
+  std::vector<float> h(NPARAM, 0.f);
+  size_t off = 0;
+  for (int l = 0; l < NL; ++l) {
+    const int cin = CH[l], cout = CH[l + 1], nw = cout * cin * 9;
+    const double a = (l == NL - 1) ? 0.0 : std::sqrt(6.0 / (9.0 * cin));
+    for (int i = 0; i < nw; ++i) {
+      const double u = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+      h[off + i] = (float)(a * (2.0 * u - 1.0));
+    }
+    off += nw + cout;
+  }
+  set_params(h.data());
 }
 
 static void to_host_layout(const NetLayerDev& d, const float* dev, float* h) {
@@ -421,18 +481,36 @@ void Net::forward(const bf16_t* x, int nimg, int H, int W, float* y, bool keep) 
                      keep ? zb[2] : nullptr, act[2], nullptr, nullptr);
   hipLaunchKernelGGL((k_conv<16, 16, 0>), grid, dim3(256), 0, st, act[2], H, W, L[2].Wm, L[2].b,
                      keep ? zb[3] : nullptr, act[3], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv<16, 1, 1>), grid, dim3(256), 0, st, act[3], H, W, L[3].Wm, L[3].b, nullptr, nullptr,
-                     y ? y : yb, nullptr);
+  hipLaunchKernelGGL((k_conv<16, COUT, 1>), grid, dim3(256), 0, st, act[3], H, W, L[3].Wm, L[3].b, nullptr,
+                     nullptr, y ? y : yb, nullptr);
   SAV_LAUNCH_CHECK();
 }
 
 double Net::loss_backward(const bf16_t* x, const float* target, int nimg, int H, int W, int border) {
   forward(x, nimg, H, W, nullptr, true);
-  const long long npix = (long long)nimg * H * W;
-  int nb = (int)((npix + 1023) / 1024);
+  const long long n = (long long)nimg * H * W * COUT;
+  int nb = (int)((n + 1023) / 1024);
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, st, yb, target, nimg, H, W, border, g1, dloss);
-  hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(64), 0, st, dloss, nb, dloss + 4096);
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, st, yb, target, nimg, H, W, COUT, border, glast, dloss);
+  nb_loss = nb;
+  return backward(x, nimg, H, W);
+}
+
+double Net::loss_backward_stencil(const bf16_t* x, const float* v, const float* target, int nimg, int H, int W,
+                                  int border) {
+  forward(x, nimg, H, W, nullptr, true);
+  const long long n = (long long)nimg * H * W;
+  int nb = (int)((n + 1023) / 1024);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_stencil_mse, dim3(nb), dim3(256), 0, st, x, yb, v, target, nimg, H, W, border, glast,
+                     dloss);
+  nb_loss = nb;
+  return backward(x, nimg, H, W);
+}
+
+// Back-propagates the output gradient in `glast` (loss partials in dloss[0 .. nb_loss)).
+double Net::backward(const bf16_t* x, int nimg, int H, int W) {
+  hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(64), 0, st, dloss, nb_loss, dloss + 4096);
   dim3 grid(W / CTX, H / CTY, nimg);
   const int nblk = (int)(grid.x * grid.y * grid.z);
   auto reduce = [&](int l) {
@@ -443,10 +521,10 @@ double Net::loss_backward(const bf16_t* x, const float* target, int nimg, int H,
                        nkp, d.gW, d.gb);
   };
   // layer 4
-  hipLaunchKernelGGL((k_wgrad<16, 1>), grid, dim3(256), 0, st, act[3], g1, H, W, partials);
+  hipLaunchKernelGGL((k_wgrad<16, COUT>), grid, dim3(256), 0, st, act[3], glast, H, W, partials);
   reduce(3);
-  hipLaunchKernelGGL((k_conv<1, 16, 2>), grid, dim3(256), 0, st, g1, H, W, L[3].Wt, nullptr, nullptr, g16a, nullptr,
-                     zb[3]);
+  hipLaunchKernelGGL((k_conv<COUT, 16, 2>), grid, dim3(256), 0, st, glast, H, W, L[3].Wt, nullptr, nullptr, g16a,
+                     nullptr, zb[3]);
   // layer 3
   hipLaunchKernelGGL((k_wgrad<16, 16>), grid, dim3(256), 0, st, act[2], g16a, H, W, partials);
   reduce(2);

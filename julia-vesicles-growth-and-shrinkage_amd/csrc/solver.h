@@ -67,6 +67,18 @@ class ISolver {
   virtual double last_elapsed_ms() = 0;
   virtual void kernel_timing(int id) = 0;
   virtual void kernel_timing_get(long long* n, double* ms, double* bytes) = 0;
+  // learned psi preconditioner (2D)
+  virtual int set_precond(int mode) = 0;
+  virtual int precond_init(uint64_t seed) = 0;
+  virtual int precond_params(bool set, float* p, int n) = 0;
+  virtual int precond_collect(int on, int patches_per_snapshot, uint64_t seed) = 0;
+  virtual long long precond_samples() = 0;
+  virtual long long precond_fallbacks() = 0;
+  virtual int precond_train(int iters, int batch, double lr, uint64_t seed, double* loss_first,
+                            double* loss_last) = 0;
+  virtual int net_forward(const float* x, int nimg, int Hh, int Ww, float* y) = 0;
+  virtual int net_grad(const float* x, const float* t, int nimg, int Hh, int Ww, int border, float* grads,
+                       double* loss) = 0;
 };
 
 std::unique_ptr<ISolver> make_solver(const HostCfg& cfg);

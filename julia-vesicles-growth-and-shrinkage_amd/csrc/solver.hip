@@ -8,11 +8,13 @@
 //   spectral: [phi_hat | psi_hat | mu_hat | nu_hat | u_hat | p_hat]
 // Three such buffers rotate (present, old, next) so update_state! is a pointer
 // swap and the back-transform of a new state is a single batched irfft.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "krylov.h"
 #include "launch.h"
+#include "learned.h"
 #include "solver.h"
 
 namespace sav {
@@ -487,7 +489,25 @@ class Solver final : public ISolver {
     C* b = kc.b;
     foreach_n(s, NP, [=] __device__(long long i) { b[i] = CD{rphys[i], T(0)}; });
     fft_full_2d<R>(g, b, N, false, s);
-    const int it = bicgstab_2d<R>(kc, slot);
+    int it;
+    if (use_learned && !collecting) {
+      if (stencil_stale) refresh_stencil();
+      it = fbicgstab_2d<R>(kc, slot, [this](const C* in, C* o) { learned_apply(in, o); });
+      if (!hks[slot].solved) {
+        // a preconditioner that failed to reach the tolerance is not trusted: the solve
+        // is repeated from x0 = 0 with the reference's diagonal preconditioner; the
+        // iteration count reports both attempts
+        ++learned_fallbacks;
+        it += bicgstab_2d<R>(kc, slot);
+      }
+    } else {
+      if (collecting) {
+        kc.snap = snapbuf;
+        kc.snap_cap = c.itmax;
+      }
+      it = bicgstab_2d<R>(kc, slot);
+      kc.snap = nullptr;
+    }
     const C* x = kc.x;
     const int Nxl = Nx, Nxhl = Nxh;
     foreach_n(s, (long long)N * H, [=] __device__(long long i) {
@@ -495,7 +515,398 @@ class Solver final : public ISolver {
       const int ix = (int)(i % Nxhl);
       out[i] = x[row * Nxl + ix];
     });
+    if (collecting) harvest(kc.snap_n);
     return it;
+  }
+
+  // ==========================================================================
+  // Learned psi preconditioner (no reference counterpart).
+  //   z = P ( r + F( g ) ),  g(x) = kappa * sum_tap w_tap(x) * Re F^{-1}(P r)(x + d(tap))
+  // P = the diagonal preconditioner of Solvers.jl:57, F = the full 2D FFT,
+  // kappa = S4 * M_max / (dx dy), and w(x) = (1 - M_psi / M_max) net(4 (M_psi / M_max -
+  // 0.75), 1) the nine weights of a 3x3 stencil per pixel (zero where M_psi = M_max, as
+  // the exact correction is), computed once per M_psi (once per step).  The
+  // preconditioner is therefore a fixed LINEAR operator during a solve, which BiCGSTAB
+  // needs.  Its target: A^{-1} = P (I + (P^{-1} - A) A^{-1}) and
+  // (P^{-1} - A) e = S4 div((M_max - M_psi) grad e), a local operator of e ~ P r; the
+  // network is trained on pairs from diagonal solves, e = x* - x_k (error of iterate k),
+  // r = A e, target Re F^{-1}(P^{-1} e - r) / kappa.  The outer P damps the BF16 noise
+  // of the stencil at high k.  The flexible BiCGSTAB (fbicgstab_2d) that uses it stops on
+  // the same diagonally preconditioned residual as the reference's solve.
+  // ==========================================================================
+  std::unique_ptr<Net> net;
+  bool use_learned = false, collecting = false, stencil_stale = true;
+  int precond_kind = SAV_PRECOND_DIAG;
+  long long learned_fallbacks = 0;  // learned/stencil solves repeated with the diagonal preconditioner
+  bf16_t* lp_in = nullptr;   // [N][Ny][Nx][2]: mobility channel, constant channel
+  float* lp_W = nullptr;     // [N][Ny][Nx][9]: stencil weights
+  C* lp_tmp = nullptr;       // full complex work vectors
+  C* lp_g = nullptr;
+  C* snapbuf = nullptr;      // itmax iterates of one diagonal solve
+  int ppsnap = 0;
+  uint64_t rng = 0;
+  long long store_cap = 0, store_n = 0;
+  bf16_t* st_in = nullptr;   // [store_cap][PATCH][PATCH][2]
+  float* st_v = nullptr;     // [store_cap][PATCH][PATCH]
+  float* st_tg = nullptr;    // [store_cap][PATCH][PATCH]
+  int* d_pos = nullptr;      // window origins (image, y0, x0)
+  std::vector<int> h_pos;
+  bf16_t* bt_in = nullptr;   // training batch
+  float* bt_v = nullptr;
+  float* bt_tg = nullptr;
+  int* d_idx = nullptr;
+  int bcap = 0;
+  std::vector<int> h_idx;
+
+  uint64_t next_rand() {  // SplitMix64
+    rng += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = rng;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+  }
+
+  void require_learned_grid() const {
+    if (nd != 2) throw HipError("the learned preconditioner is built for the 2D psi solve", SAV_E_UNSUPPORTED);
+    if (Nx % 32 != 0 || Ny % 16 != 0)
+      throw HipError("the learned preconditioner needs Nx % 32 == 0 and Ny % 16 == 0", SAV_E_UNSUPPORTED);
+  }
+
+  void make_net() {
+    if (net) return;
+    net.reset(new Net(s));
+    net->init_random(0x5EED5EEDULL);
+  }
+
+  void ensure_net() {
+    require_learned_grid();
+    make_net();
+    if (lp_in) return;
+    const long long nf = kc.nfull;
+    lp_in = dalloc<bf16_t>(2 * nf);
+    lp_W = dalloc<float>(9 * nf);
+    lp_tmp = dalloc<C>(nf);
+    lp_g = dalloc<C>(nf);
+    kc.Bp = dalloc<C>(nf);
+    kc.Bs = dalloc<C>(nf);
+  }
+
+  struct PD {  // diagonal preconditioner of Solvers.jl:57 on the full grid
+    const double *kx0, *ky;
+    int Nx, Ny;
+    double S4;
+    const DScal* ds;
+    __device__ __forceinline__ double operator()(long long i) const {
+      const int ix = (int)(i % Nx), iy = (int)((i / Nx) % Ny);
+      const double lapf = -(kx0[ix] * kx0[ix] + ky[iy] * ky[iy]);
+      return 1.0 / (ds->a / ds->dt - ds->Mmax * lapf * S4);
+    }
+  };
+  PD pdiag() const { return PD{d_kx0, d_ky, Nx, Ny, c.S4, ds}; }
+  double kappa_over_mmax() const { return c.S4 * (double)Nx * Ny / (c.Lx * c.Ly); }
+
+  // network input of every pixel: (4 (M / M_max - 0.75), 1)
+  void mobility_input(bf16_t* xin) {
+    const R* M = w_M;
+    const DScal* dsl = ds;
+    foreach_n(s, kc.nfull, [=] __device__(long long i) {
+      xin[2 * i] = f2bf((float)(4.0 * ((double)M[i] / dsl->Mmax - 0.75)));
+      xin[2 * i + 1] = f2bf(1.0f);
+    });
+  }
+
+  void refresh_stencil() {
+    if (precond_kind == SAV_PRECOND_STENCIL) {
+      // first-order Neumann stencil: g = -S4 div((M_max - M) grad v), 5-point form with
+      // face averages, in units of kappa = S4 M_max / (dx dy)
+      const R* M = w_M;
+      const DScal* dsl = ds;
+      float* w = lp_W;
+      const int Nxl = Nx, Nyl = Ny;
+      const double rx = (c.Ly / Ny) / (c.Lx / Nx), ry = 1.0 / rx;  // dy/dx, dx/dy
+      foreach_n(s, kc.nfull, [=] __device__(long long i) {
+        const int ix = (int)(i % Nxl);
+        const long long row = i / Nxl;
+        const int iy = (int)(row % Nyl);
+        const long long base = (row - iy) * Nxl;
+        const double mm = dsl->Mmax;
+        auto D = [&](int y, int x) { return 1.0 - (double)M[base + (long long)((y + Nyl) % Nyl) * Nxl + (x + Nxl) % Nxl] / mm; };
+        const double dc = D(iy, ix);
+        const double de = 0.5 * (dc + D(iy, ix + 1)) * rx, dw = 0.5 * (dc + D(iy, ix - 1)) * rx;
+        const double dn = 0.5 * (dc + D(iy + 1, ix)) * ry, dsv = 0.5 * (dc + D(iy - 1, ix)) * ry;
+        float* wi = w + i * 9;
+        wi[0] = 0.f;
+        wi[1] = (float)(-dsv);
+        wi[2] = 0.f;
+        wi[3] = (float)(-dw);
+        wi[4] = (float)(de + dw + dn + dsv);
+        wi[5] = (float)(-de);
+        wi[6] = 0.f;
+        wi[7] = (float)(-dn);
+        wi[8] = 0.f;
+      });
+    } else {
+      mobility_input(lp_in);
+      net->forward(lp_in, N, Ny, Nx, lp_W, false);
+      const R* M = w_M;
+      const DScal* dsl = ds;
+      float* w = lp_W;
+      foreach_n(s, kc.nfull, [=] __device__(long long i) {
+        const float D = (float)(1.0 - (double)M[i] / dsl->Mmax);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) w[i * 9 + tap] *= D;
+      });
+    }
+    stencil_stale = false;
+  }
+
+  void learned_apply(const C* r, C* z) {
+    const long long nf = kc.nfull;
+    const PD pd = pdiag();
+    C* y = lp_tmp;
+    foreach_n(s, nf, [=] __device__(long long i) {
+      const double pv = pd(i);
+      const C ri = r[i];
+      y[i] = CD{pv * (double)ri.x, pv * (double)ri.y};
+    });
+    fft_full_2d<R>(g, lp_tmp, N, true, s);
+    const float* w = lp_W;
+    const C* yc = lp_tmp;
+    C* gg = lp_g;
+    const int Nxl = Nx, Nyl = Ny;
+    const double kom = kappa_over_mmax();
+    const DScal* dsl = ds;
+    foreach_n(s, nf, [=] __device__(long long i) {
+      const int ix = (int)(i % Nxl);
+      const long long row = i / Nxl;
+      const int iy = (int)(row % Nyl);
+      const long long base = (row - iy) * Nxl;  // first pixel of this field
+      double acc = 0.0;
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap) {
+        const int sy = (iy + tap / 3 - 1 + Nyl) % Nyl, sx = (ix + tap % 3 - 1 + Nxl) % Nxl;
+        acc += (double)w[i * 9 + tap] * (double)yc[base + (long long)sy * Nxl + sx].x;
+      }
+      gg[i] = CD{kom * dsl->Mmax * acc, 0.0};
+    });
+    fft_full_2d<R>(g, lp_g, N, false, s);
+    const C* gc = lp_g;
+    foreach_n(s, nf, [=] __device__(long long i) {
+      const double pv = pd(i);
+      const C ri = r[i], gi = gc[i];
+      z[i] = CD{pv * ((double)ri.x + (double)gi.x), pv * ((double)ri.y + (double)gi.y)};
+    });
+  }
+
+  // Training pairs from the iterates of the diagonal solve that just finished
+  // (kc.x = x*, snapbuf[k-1] = x_k): e_k = x* - x_k for k = 0 .. nsnap-1 (x_0 = 0).
+  void harvest(int nsnap) {
+    const long long nf = kc.nfull;
+    const double norm = (double)Nx * Ny * (double)nf;  // sum|x|^2 = sum|X|^2 / (Nx Ny)
+    mobility_input(lp_in);
+    const PD pd = pdiag();
+    const double kom = kappa_over_mmax();
+    for (int k = 0; k < nsnap && store_n < store_cap; ++k) {
+      C* e = kc.s;
+      C* rr = kc.t;
+      C* dl = kc.v;
+      const C* xs = kc.x;
+      const C* xk = k == 0 ? nullptr : snapbuf + (long long)(k - 1) * nf;
+      foreach_n(s, nf, [=] __device__(long long i) {
+        const C a = xs[i];
+        e[i] = xk ? CD{(double)a.x - (double)xk[i].x, (double)a.y - (double)xk[i].y} : CD{(double)a.x, (double)a.y};
+      });
+      matvec_plain_2d<R>(kc, e, rr, 3);
+      C* y = lp_tmp;
+      const C* rc = rr;
+      foreach_n(s, nf, [=] __device__(long long i) {
+        const double pv = pd(i);
+        const C ri = rc[i], ei = e[i];
+        y[i] = CD{pv * (double)ri.x, pv * (double)ri.y};
+        dl[i] = CD{(double)ei.x / pv - (double)ri.x, (double)ei.y / pv - (double)ri.y};
+      });
+      const C* yc = lp_tmp;
+      reduce_n<1>(s, nf, 1, [=] __device__(long long i, int, double(&acc)[1]) {
+        const C v = yc[i];
+        acc[0] += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+      }, partials, red);
+      fft_full_2d<R>(g, lp_tmp, N, true, s);
+      fft_full_2d<R>(g, dl, N, true, s);
+      const long long np = std::min<long long>(ppsnap, store_cap - store_n);
+      h_pos.resize(3 * np);
+      for (long long p = 0; p < np; ++p) {
+        h_pos[3 * p] = (int)(next_rand() % (uint64_t)N);
+        h_pos[3 * p + 1] = (int)(next_rand() % (uint64_t)Ny);
+        h_pos[3 * p + 2] = (int)(next_rand() % (uint64_t)Nx);
+      }
+      SAV_HIP(hipMemcpyAsync(d_pos, h_pos.data(), 3 * np * sizeof(int), hipMemcpyHostToDevice, s));
+      const double* rd = red;
+      const C* d = dl;
+      const DScal* dsl = ds;
+      const bf16_t* xin = lp_in;
+      const int* pos = d_pos;
+      bf16_t* tin = st_in + store_n * (long long)(2 * PATCH * PATCH);
+      float* tv = st_v + store_n * (long long)(PATCH * PATCH);
+      float* ttg = st_tg + store_n * (long long)(PATCH * PATCH);
+      const int Nxl = Nx, Nyl = Ny;
+      foreach_n(s, np * PATCH * PATCH, [=] __device__(long long i) {
+        const long long p = i / (PATCH * PATCH);
+        const int py = (int)((i / PATCH) % PATCH), px = (int)(i % PATCH);
+        const int img = pos[3 * p], yy = (pos[3 * p + 1] + py) % Nyl, xx = (pos[3 * p + 2] + px) % Nxl;
+        const long long src = ((long long)img * Nyl + yy) * Nxl + xx;
+        const double sy = sqrt(rd[0] / norm);
+        const double iy = sy > 0.0 ? 1.0 / sy : 0.0;
+        tin[2 * i] = xin[2 * src];
+        tin[2 * i + 1] = xin[2 * src + 1];
+        tv[i] = (float)((double)yc[src].x * iy);
+        ttg[i] = (float)((double)d[src].x * iy / (kom * dsl->Mmax));
+      });
+      store_n += np;
+    }
+  }
+
+  int set_precond(int mode) override {
+    if (mode != SAV_PRECOND_DIAG && mode != SAV_PRECOND_LEARNED && mode != SAV_PRECOND_STENCIL) return SAV_E_ARG;
+    if (mode != SAV_PRECOND_DIAG) ensure_net();
+    use_learned = mode != SAV_PRECOND_DIAG;
+    precond_kind = mode;
+    stencil_stale = true;
+    return SAV_OK;
+  }
+
+  int precond_init(uint64_t seed) override {
+    make_net();
+    net->init_random(seed);
+    stencil_stale = true;
+    return SAV_OK;
+  }
+
+  int precond_params(bool set, float* p, int n) override {
+    make_net();
+    if (n != Net::NPARAM) return SAV_E_ARG;
+    if (set) {
+      net->set_params(p);
+      stencil_stale = true;
+    } else {
+      net->get_params(p);
+    }
+    return SAV_OK;
+  }
+
+  int precond_collect(int on, int per_snapshot, uint64_t seed) override {
+    ensure_net();
+    if (!on) {
+      collecting = false;
+      return SAV_OK;
+    }
+    if (per_snapshot <= 0) return SAV_E_ARG;
+    const long long nf = kc.nfull;
+    if (!snapbuf) snapbuf = dalloc<C>((long long)c.itmax * nf);
+    if (!st_in) {
+      store_cap = 24576;  // windows of 64 x 64: 805 MB
+      st_in = dalloc<bf16_t>(store_cap * 2 * PATCH * PATCH);
+      st_v = dalloc<float>(store_cap * PATCH * PATCH);
+      st_tg = dalloc<float>(store_cap * PATCH * PATCH);
+    }
+    if (!d_pos || per_snapshot > ppsnap) d_pos = dalloc<int>(3LL * per_snapshot);
+    ppsnap = per_snapshot;
+    rng = seed;
+    store_n = 0;
+    collecting = true;
+    return SAV_OK;
+  }
+
+  long long precond_samples() override { return store_n; }
+  long long precond_fallbacks() override { return learned_fallbacks; }
+
+  int precond_train(int iters, int batch, double lr, uint64_t seed, double* l0, double* l1) override {
+    ensure_net();
+    if (store_n < 1 || iters < 1 || batch < 1) return SAV_E_ARG;
+    if (batch > bcap) {
+      bt_in = dalloc<bf16_t>((long long)batch * 2 * PATCH * PATCH);
+      bt_v = dalloc<float>((long long)batch * PATCH * PATCH);
+      bt_tg = dalloc<float>((long long)batch * PATCH * PATCH);
+      d_idx = dalloc<int>(batch);
+      bcap = batch;
+    }
+    uint64_t save = rng;
+    rng = seed;
+    double s0 = 0.0, s1 = 0.0;
+    int n0 = 0, n1 = 0;
+    h_idx.resize(batch);
+    for (int it = 0; it < iters; ++it) {
+      for (int b = 0; b < batch; ++b) h_idx[b] = (int)(next_rand() % (uint64_t)store_n);
+      SAV_HIP(hipMemcpyAsync(d_idx, h_idx.data(), batch * sizeof(int), hipMemcpyHostToDevice, s));
+      const int* idx = d_idx;
+      const bf16_t* sin = st_in;
+      const float *sv = st_v, *stg = st_tg;
+      bf16_t* bin = bt_in;
+      float *bv = bt_v, *btg = bt_tg;
+      foreach_n(s, (long long)batch * PATCH * PATCH, [=] __device__(long long i) {
+        const long long b = i / (PATCH * PATCH), q = i % (PATCH * PATCH);
+        const long long src = (long long)idx[b] * PATCH * PATCH + q;
+        bin[2 * i] = sin[2 * src];
+        bin[2 * i + 1] = sin[2 * src + 1];
+        bv[i] = sv[src];
+        btg[i] = stg[src];
+      });
+      const double loss = net->loss_backward_stencil(bt_in, bt_v, bt_tg, batch, PATCH, PATCH, PATCH_BORDER);
+      if (it < 10) {
+        s0 += loss;
+        ++n0;
+      }
+      if (it >= iters - 10) {
+        s1 += loss;
+        ++n1;
+      }
+      // cosine decay from lr to lr / 10
+      const double lrt = lr * (0.1 + 0.45 * (1.0 + std::cos(M_PI * (double)it / (double)iters)));
+      net->adam_step((float)lrt, 0.9f, 0.999f, 1e-8f);
+    }
+    SAV_HIP(hipStreamSynchronize(s));
+    rng = save;
+    stencil_stale = true;
+    if (l0) *l0 = s0 / (n0 > 0 ? n0 : 1);
+    if (l1) *l1 = s1 / (n1 > 0 ? n1 : 1);
+    return SAV_OK;
+  }
+
+  int net_forward(const float* x, int nimg, int Hh, int Ww, float* y) override {
+    make_net();
+    const long long np = (long long)nimg * Hh * Ww;
+    std::vector<bf16_t> xb(2 * np);
+    for (long long i = 0; i < 2 * np; ++i) xb[i] = f2bf(x[i]);
+    bf16_t* dx = nullptr;
+    float* dy = nullptr;
+    SAV_HIP(hipMalloc(&dx, xb.size() * sizeof(bf16_t) + 64));
+    SAV_HIP(hipMalloc(&dy, np * Net::COUT * sizeof(float) + 64));
+    SAV_HIP(hipMemcpyAsync(dx, xb.data(), xb.size() * sizeof(bf16_t), hipMemcpyHostToDevice, s));
+    net->forward(dx, nimg, Hh, Ww, dy, false);
+    SAV_HIP(hipMemcpyAsync(y, dy, np * Net::COUT * sizeof(float), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    SAV_HIP(hipFree(dx));
+    SAV_HIP(hipFree(dy));
+    return SAV_OK;
+  }
+
+  int net_grad(const float* x, const float* t, int nimg, int Hh, int Ww, int border, float* grads,
+               double* loss) override {
+    make_net();
+    const long long np = (long long)nimg * Hh * Ww;
+    std::vector<bf16_t> xb(2 * np);
+    for (long long i = 0; i < 2 * np; ++i) xb[i] = f2bf(x[i]);
+    bf16_t* dx = nullptr;
+    float* dt = nullptr;
+    SAV_HIP(hipMalloc(&dx, xb.size() * sizeof(bf16_t) + 64));
+    SAV_HIP(hipMalloc(&dt, np * Net::COUT * sizeof(float) + 64));
+    SAV_HIP(hipMemcpyAsync(dx, xb.data(), xb.size() * sizeof(bf16_t), hipMemcpyHostToDevice, s));
+    SAV_HIP(hipMemcpyAsync(dt, t, np * Net::COUT * sizeof(float), hipMemcpyHostToDevice, s));
+    const double l = net->loss_backward(dx, dt, nimg, Hh, Ww, border);
+    net->get_grads(grads);
+    SAV_HIP(hipFree(dx));
+    SAV_HIP(hipFree(dt));
+    if (loss) *loss = l;
+    return SAV_OK;
   }
 
   void set_bdf(double a, double b, double cc, double dt) {
@@ -531,6 +942,7 @@ class Solver final : public ISolver {
     if (nd == 2) {
       // M_max = maximum(M_psi) (Solvers.jl:27)
       max_reduce(M, NP, &ds->Mmax);
+      stencil_stale = true;
     }
     if (sizeof(R) == 8)
       functionals(phs, pss, false);
@@ -1319,7 +1731,7 @@ class Solver final : public ISolver {
     if (st) {
       std::memset(st, 0, sizeof(*st));
       for (int k = 0; k < 3; ++k) {
-        st->krylov_iters[k] = nd == 2 ? hks[k].iter : 0;
+        st->krylov_iters[k] = nd == 2 ? kit[k] : 0;  // all iterations spent (a fallback solve included)
         st->krylov_solved[k] = nd == 2 ? hks[k].solved : 1;
         st->krylov_rnorm[k] = nd == 2 ? hks[k].rnorm : 0.0;
       }
@@ -1470,6 +1882,7 @@ class Solver final : public ISolver {
       M[i] = T(0.1) * (T(1.0) - (T)M0 * (t * t));
     });
     max_reduce(M, NP, &ds->Mmax);
+    stencil_stale = true;
     C* dr = w_rhs;
     put_cplx(rhs, dr, NH);
     C* dout = w_rhs + NH;

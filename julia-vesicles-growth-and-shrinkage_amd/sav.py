@@ -38,8 +38,10 @@ SAV_PREC_FP64 = 0
 SAV_PREC_FP32 = 1
 SAV_PRECOND_DIAG = 0
 SAV_PRECOND_LEARNED = 1
+SAV_PRECOND_STENCIL = 2
 SAV_PRESENT = 0
 SAV_OLD = 1
+SAV_NET_NPARAM = 6249
 
 _dp = ct.POINTER(ct.c_double)
 
@@ -69,7 +71,9 @@ class SavStepStatsC(ct.Structure):
 EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sav_set_state",
            "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
-           "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get")
+           "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
+           "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
+           "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad")
 
 _lib_handle = None
 
@@ -113,6 +117,16 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_last_elapsed_ms.argtypes = [ct.c_void_p, _dp]
     lib.sav_kernel_timing.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_kernel_timing_get.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64), _dp, _dp]
+    _fp = ct.c_void_p
+    lib.sav_set_precond.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_precond_init.argtypes = [ct.c_void_p, ct.c_uint64]
+    lib.sav_precond_load.argtypes = [ct.c_void_p, _fp, ct.c_int]
+    lib.sav_precond_save.argtypes = [ct.c_void_p, _fp, ct.c_int]
+    lib.sav_precond_collect.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_uint64]
+    lib.sav_precond_samples.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]
+    lib.sav_precond_train.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, _dp, _dp]
+    lib.sav_net_forward.argtypes = [ct.c_void_p, _fp, ct.c_int, ct.c_int, ct.c_int, _fp]
+    lib.sav_net_grad.argtypes = [ct.c_void_p, _fp, _fp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _fp, _dp]
     if path is None:
         _lib_handle = lib
     return lib
@@ -370,6 +384,66 @@ class Operators:
         E, ar = ct.c_double(0), ct.c_double(0)
         self._check(self.lib.sav_modified_energy(self.h, ct.byref(E), ct.byref(ar)), "modified_energy")
         return E.value, ar.value
+
+    # -- learned psi preconditioner (no reference counterpart) -----------------
+    def set_precond(self, precond: int):
+        """SAV_PRECOND_DIAG (Solvers.jl:57) or SAV_PRECOND_LEARNED for the following psi solves."""
+        self._check(self.lib.sav_set_precond(self.h, int(precond)), "set_precond")
+
+    def precond_init(self, seed: int):
+        self._check(self.lib.sav_precond_init(self.h, int(seed) & (2 ** 64 - 1)), "precond_init")
+
+    def precond_load(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != SAV_NET_NPARAM:
+            raise ValueError(f"expected {SAV_NET_NPARAM} parameters, got {p.size}")
+        self._check(self.lib.sav_precond_load(self.h, p.ctypes.data, p.size), "precond_load")
+
+    def precond_save(self) -> np.ndarray:
+        p = np.zeros(SAV_NET_NPARAM, dtype=np.float32)
+        self._check(self.lib.sav_precond_save(self.h, p.ctypes.data, p.size), "precond_save")
+        return p
+
+    def precond_collect(self, on: bool, patches_per_snapshot: int = 16, seed: int = 1):
+        """Log training windows from the following (diagonal) psi solves."""
+        self._check(self.lib.sav_precond_collect(self.h, int(bool(on)), int(patches_per_snapshot),
+                                                 int(seed) & (2 ** 64 - 1)), "precond_collect")
+
+    def precond_samples(self) -> int:
+        n = ct.c_int64(0)
+        self._check(self.lib.sav_precond_samples(self.h, ct.byref(n), None), "precond_samples")
+        return n.value
+
+    def precond_fallbacks(self) -> int:
+        """Learned/stencil solves repeated with the diagonal preconditioner (no convergence)."""
+        n = ct.c_int64(0)
+        self._check(self.lib.sav_precond_samples(self.h, None, ct.byref(n)), "precond_samples")
+        return n.value
+
+    def precond_train(self, iters: int = 1000, batch: int = 32, lr: float = 2e-3, seed: int = 7):
+        l0, l1 = ct.c_double(0), ct.c_double(0)
+        self._check(self.lib.sav_precond_train(self.h, int(iters), int(batch), float(lr), int(seed) & (2 ** 64 - 1),
+                                               ct.byref(l0), ct.byref(l1)), "precond_train")
+        return l0.value, l1.value
+
+    def net_forward(self, x: np.ndarray) -> np.ndarray:
+        """x[nimg, H, W, 2] (float, rounded to BF16 on the way in) -> stencil weights y[nimg, H, W, 9]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        nimg, H, W, two = x.shape
+        assert two == 2
+        y = np.zeros((nimg, H, W, 9), dtype=np.float32)
+        self._check(self.lib.sav_net_forward(self.h, x.ctypes.data, nimg, H, W, y.ctypes.data), "net_forward")
+        return y
+
+    def net_grad(self, x: np.ndarray, target: np.ndarray, border: int):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        nimg, H, W, _ = x.shape
+        g = np.zeros(SAV_NET_NPARAM, dtype=np.float32)
+        loss = ct.c_double(0)
+        self._check(self.lib.sav_net_grad(self.h, x.ctypes.data, t.ctypes.data, nimg, H, W, int(border),
+                                          g.ctypes.data, ct.byref(loss)), "net_grad")
+        return loss.value, g
 
 
 def build_operators(conf: Config, **kw) -> Operators:

@@ -6,11 +6,11 @@ an addition of this build.  Forward and backward are written from the
 definitions below; the backward is pinned by finite differences in
 tests/test_oracle_kat.py.
 
-Network (periodic 3x3 convolutions, channels 2 -> 16 -> 16 -> 16 -> 1):
+Network (periodic 3x3 convolutions, channels 2 -> 16 -> 16 -> 16 -> 9):
     a0 = x                                  (B, 2, H, W)
     z_l = conv3(W_l, a_{l-1}) + b_l
     a_l = gelu(z_l)            l = 1, 2, 3
-    y   = z_4                               (B, 1, H, W)
+    y   = z_4                               (B, 9, H, W)   (3x3 stencil weights per pixel)
 conv3(W, a)[b, o, i, j] = sum_{c, di, dj} W[o, c, di, dj] a[b, c, i+di-1, j+dj-1]  (indices mod H, W)
 gelu(z) = z/2 (1 + erf(z / sqrt 2)).
 """
@@ -21,7 +21,7 @@ import math
 import numpy as np
 from scipy.special import erf
 
-CHANNELS = (2, 16, 16, 16, 1)
+CHANNELS = (2, 16, 16, 16, 9)
 
 
 def gelu(z):

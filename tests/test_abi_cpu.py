@@ -26,6 +26,14 @@ def test_library_exports_every_declared_symbol(jv):
     assert sorted(jv.EXPORTS) == names
 
 
+def test_net_parameter_count_matches_header_and_oracle(jv):
+    from oracle import conv_oracle as co
+    txt = (ROOT / "include" / "sav_abi.h").read_text()
+    n = int(re.search(r"#define SAV_NET_NPARAM (\d+)", txt).group(1))
+    count = sum(W.size + b.size for W, b in co.init_params(0))
+    assert n == jv.SAV_NET_NPARAM == count
+
+
 def test_abi_version_without_device(jv):
     assert jv.load_library().sav_abi_version() == 1
 
