@@ -112,6 +112,42 @@ def run_case_learned(jv, wl, precision, steps, warmup, collect_steps=4, train_it
                 loss_first=l0, loss_last=l1, energy_monotone=bool(np.all(np.diff(E) <= 0)))
 
 
+def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16)):
+    """The 64-instance ensemble (16 each of 128^2, 256^2, 384^2, 512^2, 1-3 vesicles):
+    aggregate steps/s for each host-thread count (1 = the sequential per-instance loop),
+    after one warm-up step of every member."""
+    wls = jv.workload_c5(steps=steps)
+    ens = jv.Ensemble([w.conf for w in wls], precision=precision)
+    for m, wl in zip(ens.members, wls):
+        phi, psi = wl.initial_fields()
+        st = jv.FieldState.zeros(wl.conf)
+        st.phi[...] = phi.T
+        st.psi[...] = psi.T
+        st.phi_hat[...] = m.rfft_accurate(st.phi)
+        st.psi_hat[...] = m.rfft_accurate(st.psi)
+        st.Q = st.R1 = st.R2 = st.R3 = 1.0
+        m.set_state(jv.SAV_PRESENT, st)
+        m.init_sav()
+    dt = wls[0].conf.dt
+    n0 = 1
+    ens.run(np.arange(2) * dt, n0, threads=16)
+    n0 += 1
+    res = {}
+    for th in threads_list:
+        t = np.arange(n0 - 1, n0 + steps) * dt
+        _sync()
+        t0 = time.perf_counter()
+        rc, E, A, kit, st = ens.run(t, n0, threads=th)
+        _sync()
+        el = time.perf_counter() - t0
+        n0 += steps
+        res[th] = dict(seconds=el, steps_per_s=len(wls) * steps / el, rc=int(rc), psi_iters=int(kit.sum()))
+    ens.close()
+    best = max(res, key=lambda k: res[k]["steps_per_s"])
+    return dict(instances=len(wls), steps=steps, by_threads=res, best_threads=best,
+                speedup_vs_sequential=res[best]["steps_per_s"] / res[1]["steps_per_s"])
+
+
 def cpu_baseline(jv, wl):
     """The FP64 CPU oracle on the host cores: one BDF1 step of the same workload
     after the SAV initialisation (a bounded sample of the timed work)."""
@@ -136,7 +172,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--only", choices=["all", "2d", "3d"], default="all")
+    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble"], default="all")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -147,6 +184,9 @@ def main():
     jv = ge.load_package()
     import torch  # noqa: F401  (device-wide synchronisation around the timed region)
 
+    if a.only == "ensemble":
+        print(json.dumps(run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32)), flush=True)
+        return
     out = {}
     wl3 = jv.workload_c3(n=1024, steps=a.steps)
     if a.only in ("all", "2d"):
@@ -210,6 +250,14 @@ def main():
         line["fp32_steps_per_s_128cubed"] = round(out["3d"]["fp32"]["steps_per_s"], 3)
         line["fp64_steps_per_s_128cubed"] = round(out["3d"]["fp64"]["steps_per_s"], 3)
         line["fp32_vs_fp64_max_rel_err_128cubed"] = out["3d"]["fp32_vs_fp64_max_rel_err"]
+    if a.only == "all":
+        ens = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32)
+        line["ensemble_64_fp32"] = {"instances": ens["instances"], "steps": ens["steps"],
+                                    "sequential_steps_per_s": round(ens["by_threads"][1]["steps_per_s"], 2),
+                                    "best_steps_per_s": round(ens["by_threads"][ens["best_threads"]]["steps_per_s"], 2),
+                                    "best_threads": ens["best_threads"],
+                                    "by_threads": {k: round(v["steps_per_s"], 2) for k, v in ens["by_threads"].items()},
+                                    "speedup_vs_sequential": round(ens["speedup_vs_sequential"], 3)}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(jv, wl3 if "2d" in out else jv.workload_c4(n=128, steps=1))
     print(json.dumps(line), flush=True)

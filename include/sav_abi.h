@@ -213,6 +213,25 @@ int sav_net_forward(sav_handle_t* h, const float* x, int nimg, int H, int W, flo
 int sav_net_grad(sav_handle_t* h, const float* x, const float* t, int nimg, int H, int W, int border,
                  float* grads, double* loss);
 
+/*
+ * Ensemble of independent instances (no reference counterpart: the reference runs
+ * one simulation per process, scripts/Main.jl:86-114).  Each member is an ordinary
+ * handle with its own stream; sav_ensemble_run drives all members over the same time
+ * nodes from n_threads host threads, so one member's Krylov convergence waits overlap
+ * the others' kernels.  Members are set up through sav_ensemble_member (borrowed
+ * handles: sav_set_state, sav_init_sav, sav_get_state ...; never sav_destroy them).
+ * energy_hist / area_hist: [n][n_nodes-1] (may be NULL); krylov_iters, status: [n]
+ * (may be NULL).  Returns the most severe member status.
+ */
+typedef struct sav_ensemble sav_ensemble_t;
+int sav_ensemble_create(const sav_config_t* cfgs, int n, sav_ensemble_t** out);
+int sav_ensemble_destroy(sav_ensemble_t* e);
+const char* sav_ensemble_last_error(const sav_ensemble_t* e);
+int sav_ensemble_size(const sav_ensemble_t* e);
+sav_handle_t* sav_ensemble_member(sav_ensemble_t* e, int i);
+int sav_ensemble_run(sav_ensemble_t* e, const double* t_nodes, int n_nodes, int n0, int n_threads,
+                     double* energy_hist, double* area_hist, int32_t* krylov_iters, int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,7 +5,7 @@ Config / FieldState / Operators / run_simulation interface."""
 from .config import (Config, BDFCoeff, bdf_coeff, set_para_base, stabilisers, SplitMix64, Vesicle, Workload,
                      initial_fields, workload_c1, workload_c2, workload_c3, workload_c4, workload_c5, WORKLOADS,
                      InexactError)
-from .sav import (FieldState, Operators, build_operators, to_spectral, to_physical, update_state,
+from .sav import (FieldState, Operators, Ensemble, config_c, build_operators, to_spectral, to_physical, update_state,
                   generate_initial_condition, state_from_physical, run_simulation, time_nodes, load_library,
                   SavError, WCNegativeError, EXPORTS, LIB_PATH,
                   SAV_OK, SAV_W_KRYLOV_NOT_CONVERGED, SAV_W_THETA2_NONNEG, SAV_E_ARG, SAV_E_HIP,

@@ -73,7 +73,9 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
            "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
-           "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad")
+           "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad",
+           "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
+           "sav_ensemble_member", "sav_ensemble_run")
 
 _lib_handle = None
 
@@ -127,6 +129,14 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_precond_train.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, _dp, _dp]
     lib.sav_net_forward.argtypes = [ct.c_void_p, _fp, ct.c_int, ct.c_int, ct.c_int, _fp]
     lib.sav_net_grad.argtypes = [ct.c_void_p, _fp, _fp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _fp, _dp]
+    lib.sav_ensemble_create.argtypes = [ct.POINTER(SavConfigC), ct.c_int, ct.POINTER(ct.c_void_p)]
+    lib.sav_ensemble_destroy.argtypes = [ct.c_void_p]
+    lib.sav_ensemble_last_error.argtypes = [ct.c_void_p]
+    lib.sav_ensemble_last_error.restype = ct.c_char_p
+    lib.sav_ensemble_size.argtypes = [ct.c_void_p]
+    lib.sav_ensemble_member.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_ensemble_member.restype = ct.c_void_p
+    lib.sav_ensemble_run.argtypes = [ct.c_void_p, _dp, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _fp, _fp]
     if path is None:
         _lib_handle = lib
     return lib
@@ -195,6 +205,27 @@ def update_state(dest: FieldState, src: FieldState) -> FieldState:
     return dest
 
 
+def config_c(conf: Config, precision: int, itmax: int = 20, atol: float = 1e-14, device: int = 0,
+             precond: int = SAV_PRECOND_DIAG):
+    """sav_config_t of a Config (src/Types.jl:27-74); returns it with the arrays it points to."""
+    pin = np.ascontiguousarray(conf.psi_in_v[: conf.N], dtype=np.float64)
+    pout = np.ascontiguousarray(conf.psi_out_v[: conf.N], dtype=np.float64)
+    c = SavConfigC()
+    c.ndim, c.N, c.Nx, c.Ny, c.Nz = conf.ndim, conf.N, conf.Nx, conf.Ny, conf.Nz
+    c.Lx, c.Ly, c.Lz = conf.Lx, conf.Ly, conf.Lz
+    c.epsilon, c.M_phi, c.M0_psi, c.eta, c.gamma_surf = (conf.epsilon, conf.M_phi, conf.M0_psi,
+                                                        conf.eta, conf.gamma_surf)
+    c.gamma_area, c.gamma_bend = conf.gamma_area, conf.gamma_bend
+    c.gamma_in, c.beta_in, c.gamma_out, c.beta_out, c.lamda = (conf.gamma_in, conf.beta_in,
+                                                               conf.gamma_out, conf.beta_out, conf.lamda)
+    c.psi_in_v, c.psi_out_v = _ptr(pin), _ptr(pout)
+    c.S1, c.S2, c.S3, c.S4 = conf.S1, conf.S2, conf.S3, conf.S4
+    c.C1, c.C2, c.C3 = conf.C1, conf.C2, conf.C3
+    c.dt, c.T, c.tol = conf.dt, conf.T, conf.tol
+    c.precision, c.precond, c.itmax, c.atol, c.device = precision, precond, itmax, atol, device
+    return c, (pin, pout)
+
+
 # ---------------------------------------------------------------------------
 # Operators: the device handle (build_operators, src/SpectralUtils.jl:36-118)
 # ---------------------------------------------------------------------------
@@ -205,25 +236,15 @@ class Operators:
     (FP32 fields/FFTs with FP64 reductions, 3x3 solve and Krylov scalars)."""
 
     def __init__(self, conf: Config, precision: int = SAV_PREC_FP64, itmax: int = 20, atol: float = 1e-14,
-                 device: int = 0, precond: int = SAV_PRECOND_DIAG):
+                 device: int = 0, precond: int = SAV_PRECOND_DIAG, _borrow=None):
         self.lib = load_library()
         self.conf = conf
-        self._pin = np.ascontiguousarray(conf.psi_in_v[: conf.N], dtype=np.float64)
-        self._pout = np.ascontiguousarray(conf.psi_out_v[: conf.N], dtype=np.float64)
-        c = SavConfigC()
-        c.ndim, c.N, c.Nx, c.Ny, c.Nz = conf.ndim, conf.N, conf.Nx, conf.Ny, conf.Nz
-        c.Lx, c.Ly, c.Lz = conf.Lx, conf.Ly, conf.Lz
-        c.epsilon, c.M_phi, c.M0_psi, c.eta, c.gamma_surf = (conf.epsilon, conf.M_phi, conf.M0_psi,
-                                                            conf.eta, conf.gamma_surf)
-        c.gamma_area, c.gamma_bend = conf.gamma_area, conf.gamma_bend
-        c.gamma_in, c.beta_in, c.gamma_out, c.beta_out, c.lamda = (conf.gamma_in, conf.beta_in,
-                                                                   conf.gamma_out, conf.beta_out, conf.lamda)
-        c.psi_in_v, c.psi_out_v = _ptr(self._pin), _ptr(self._pout)
-        c.S1, c.S2, c.S3, c.S4 = conf.S1, conf.S2, conf.S3, conf.S4
-        c.C1, c.C2, c.C3 = conf.C1, conf.C2, conf.C3
-        c.dt, c.T, c.tol = conf.dt, conf.T, conf.tol
-        c.precision, c.precond, c.itmax, c.atol, c.device = precision, precond, itmax, atol, device
         self.precision = precision
+        self._owned = _borrow is None
+        if _borrow is not None:
+            self.h = _borrow
+            return
+        c, self._keep = config_c(conf, precision, itmax, atol, device, precond)
         h = ct.c_void_p()
         rc = self.lib.sav_create(ct.byref(c), ct.byref(h))
         if rc != 0:
@@ -232,7 +253,8 @@ class Operators:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.sav_destroy(self.h)
+            if self._owned:
+                self.lib.sav_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -444,6 +466,55 @@ class Operators:
         self._check(self.lib.sav_net_grad(self.h, x.ctypes.data, t.ctypes.data, nimg, H, W, int(border),
                                           g.ctypes.data, ct.byref(loss)), "net_grad")
         return loss.value, g
+
+
+class Ensemble:
+    """Independent instances driven concurrently (sav_ensemble_*).  ``members[i]`` are
+    borrowed Operators for state set-up and read-back."""
+
+    def __init__(self, confs: List[Config], precision: int = SAV_PREC_FP64, itmax: int = 20, atol: float = 1e-14,
+                 device: int = 0):
+        self.lib = load_library()
+        self.confs = list(confs)
+        arr = (SavConfigC * len(self.confs))()
+        self._keep = []
+        for i, conf in enumerate(self.confs):
+            c, keep = config_c(conf, precision, itmax, atol, device)
+            arr[i] = c
+            self._keep.append(keep)
+        e = ct.c_void_p()
+        rc = self.lib.sav_ensemble_create(arr, len(self.confs), ct.byref(e))
+        if rc != 0:
+            raise SavError(rc, self.lib.sav_ensemble_last_error(None).decode())
+        self.e = e
+        self.members = [Operators(conf, precision, _borrow=ct.c_void_p(self.lib.sav_ensemble_member(e, i)))
+                        for i, conf in enumerate(self.confs)]
+
+    def run(self, t_nodes: np.ndarray, n0: int = 1, threads: int = 8):
+        t = np.ascontiguousarray(t_nodes, dtype=np.float64)
+        n, m = len(self.confs), max(len(t) - 1, 0)
+        E = np.zeros((n, m))
+        A = np.zeros((n, m))
+        kit = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        rc = self.lib.sav_ensemble_run(self.e, _ptr(t), len(t), int(n0), int(threads), _ptr(E), _ptr(A),
+                                       kit.ctypes.data, st.ctypes.data)
+        if rc < 0:
+            raise SavError(rc, "ensemble_run: " + self.lib.sav_ensemble_last_error(self.e).decode())
+        return rc, E, A, kit, st
+
+    def close(self):
+        if getattr(self, "e", None):
+            for m in self.members:
+                m.h = None
+            self.lib.sav_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def build_operators(conf: Config, **kw) -> Operators:
