@@ -20,7 +20,9 @@ struct KrylovCtx {
   double S4, atol, rtol;
   int itmax;
   cplx<R> *x, *r, *p, *v, *s, *t, *b, *Bx, *By;
-  double* partials;           // >= 3 * max blocks doubles
+  double* partials;           // >= nsys * ps doubles
+  long long vs = 0;           // elements between the vectors of consecutive systems of a batch
+  long long ps = 0;           // partials between consecutive systems
   KScal* ks;                  // device, one per solve slot
   KScal* hks;                 // pinned host mirror
   hipStream_t st;
@@ -46,6 +48,12 @@ struct KrylovCtx {
 // number of iterations; ctx.hks[slot] holds the final Krylov scalars.
 template <typename R>
 int bicgstab_2d(KrylovCtx<R>& k, int slot);
+// nsys independent systems solved together (every launch covers all of them): system z
+// uses the vectors at offset z * ctx.vs (b, x, r, p, v, s, t, Bx, By) and KScal slot + z;
+// each system's iterates are exactly those of its own solve.  Returns the largest
+// iteration count; ctx.hks[slot + z] holds each system's final scalars.
+template <typename R>
+int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys);
 
 // A*y then the diagonal preconditioner: out = P^{-1} (a/dt y - sum_d D1f_d F(M F^{-1}(D1f_d S4 y)))
 // (one matvec + preconditioner, used by tests through the solver).

@@ -13,6 +13,7 @@
 // All Krylov scalars stay on the device; every kernel of an iteration returns at
 // once when the solve has finished, so iterations can be queued ahead of the
 // host's convergence check.
+#include <algorithm>
 #include <cmath>
 
 #include <hip/hip_ext.h>
@@ -27,11 +28,14 @@ using KColCfg = ColTile<R, L>;
 
 // Row kernel: one team per row, 256 threads per workgroup; the LDS of each row is
 // padded by one slot every 16 (FP64) or 32 (FP32) positions against bank conflicts.
+// In FP64, where a team is exactly one wavefront, its LDS exchanges synchronise the
+// wavefront only (no workgroup barrier; measured faster in FP64 only).
 template <typename R, int L>
 struct KRowCfg {
   static constexpr int PADS = sizeof(R) == 8 ? 4 : 5;
-  using FI = TeamFFT<R, L, true, (sizeof(R) == 8 ? 128 : 64), PADS>;
-  using FF = TeamFFT<R, L, false, (sizeof(R) == 8 ? 128 : 64), PADS>;
+  static constexpr int WS = (sizeof(R) == 8 && L / make_fft_plan(L, 64).E == 64) ? 1 : 0;
+  using FI = TeamFFT<R, L, true, 64, PADS, default_twm<R>(), WS>;
+  using FF = TeamFFT<R, L, false, 64, PADS, default_twm<R>(), WS>;
   static constexpr int TT = FI::TT;
   static constexpr int ROWS = TT >= 256 ? 1 : 256 / TT;
   static constexpr int NT = ROWS * TT;
@@ -53,8 +57,14 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT), 4) void k_mv_cols_inv(const cp
                                                                     const double* __restrict__ kx0,
                                                                     const double* __restrict__ ky,
                                                                     const cplx<R>* __restrict__ tw,
-                                                                    const KScal* __restrict__ ks) {
+                                                                    const KScal* __restrict__ ks, long long vs) {
+  // blockIdx.z: system of a batched solve (vectors vs apart, one KScal each)
+  const int z = blockIdx.z;
+  ks += z;
   if (ks->done) return;
+  X += z * vs;
+  Bx += z * vs;
+  By += z * vs;
   using F = TeamFFT<R, L, true>;
   constexpr int C = KColCfg<R, L>::C;
   constexpr int E = F::E;
@@ -98,8 +108,12 @@ template <typename R, int L>
 __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __restrict__ Bx, cplx<R>* __restrict__ By,
                                                                 const R* __restrict__ M, long long nrows_half,
                                                                 double scale, const cplx<R>* __restrict__ tw,
-                                                                const KScal* __restrict__ ks) {
+                                                                const KScal* __restrict__ ks, long long vs) {
+  const int z = blockIdx.y;
+  ks += z;
   if (ks->done) return;
+  Bx += z * vs;
+  By += z * vs;
   using FI = typename KRowCfg<R, L>::FI;
   using FF = typename KRowCfg<R, L>::FF;
   constexpr int TT = FI::TT, E = FI::E, ROWS = KRowCfg<R, L>::ROWS, RS = KRowCfg<R, L>::RS;
@@ -111,20 +125,31 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
   cplx<R>* data = (row < nrows_half ? Bx : By) + rr * L;
   const R* m = M + rr * L;
   cplx<R>* my = lds + rloc * RS;
+  // FP64: M_psi is loaded with the row rather than after the first transform (measured
+  // faster in FP64, slower in FP32)
+  constexpr bool MPRE = sizeof(R) == 8;
   cplx<R> v[E];
+  R mv[MPRE ? E : 1];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = act ? data[FI::in_pos(t, e)] : cplx<R>{R(0), R(0)};
+  for (int e = 0; e < E; ++e) {
+    v[e] = act ? data[FI::in_pos(t, e)] : cplx<R>{R(0), R(0)};
+    if constexpr (MPRE) mv[e] = act ? m[FI::out_pos(t, e)] : R(0);
+  }
   FI::run(v, my, 1, t, tw);
   const R sc = (R)scale;
-  __syncthreads();
+  FI::sync();
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int pos = FI::out_pos(t, e);
-    const R mv = act ? m[pos] : R(0);
+    R mval;
+    if constexpr (MPRE)
+      mval = mv[e];
+    else
+      mval = act ? m[pos] : R(0);
     const cplx<R> z = cplx<R>{sc * v[e].x, sc * v[e].y};
-    my[FI::lidx(pos)] = cplx<R>{mv * z.x, mv * z.y};
+    my[FI::lidx(pos)] = cplx<R>{mval * z.x, mval * z.y};
   }
-  __syncthreads();
+  FI::sync();
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = my[FF::lidx(FF::in_pos(t, e))];
   FF::run(v, my, 1, t, tw);
@@ -145,8 +170,16 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_fwd(
     const cplx<R>* __restrict__ Bx, const cplx<R>* __restrict__ By, const cplx<R>* __restrict__ Y,
     const cplx<R>* __restrict__ Cv, cplx<R>* __restrict__ out, int Nx, double S4, const double* __restrict__ kx0,
     const double* __restrict__ ky, const cplx<R>* __restrict__ tw, const DScal* __restrict__ ds,
-    const KScal* __restrict__ ks, double* __restrict__ partials) {
+    const KScal* __restrict__ ks, double* __restrict__ partials, long long vs, long long ps) {
+  const int z = blockIdx.z;
+  ks += z;
   if (ks->done) return;
+  Bx += z * vs;
+  By += z * vs;
+  Y += z * vs;
+  if (Cv) Cv += z * vs;
+  out += z * vs;
+  partials += z * ps;
   using F = TeamFFT<R, L, false>;
   constexpr int C = KColCfg<R, L>::C;
   constexpr int E = F::E;
@@ -245,8 +278,15 @@ __global__ __launch_bounds__(256) void k_kry_init(const cplx<R>* __restrict__ b,
                                                   cplx<R>* __restrict__ p, cplx<R>* __restrict__ x, long long n,
                                                   int Nx, int Ny, double S4, const double* __restrict__ kx0,
                                                   const double* __restrict__ ky, const DScal* __restrict__ ds,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ partials, long long vs, long long ps) {
   __shared__ double scratch[3 * 4];
+  // blockIdx.y: system of a batched solve
+  const int z = blockIdx.y;
+  b += z * vs;
+  r += z * vs;
+  p += z * vs;
+  x += z * vs;
+  partials += z * ps;
   double acc[3] = {0.0, 0.0, 0.0};
   const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
@@ -285,8 +325,11 @@ __device__ __forceinline__ void sum_partials(const double* partials, int nb, dou
 
 // Finalise a 3-wide partial array with one block (deterministic order).
 __global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ partials, int nb, KScal* __restrict__ ks,
-                                                 int stage, double atol, double rtol, int itmax) {
+                                                 int stage, double atol, double rtol, int itmax, long long ps) {
   __shared__ double scratch[3 * 4];
+  // one block per system of a batched solve
+  ks += blockIdx.x;
+  partials += blockIdx.x * ps;
   if (stage != 0 && ks->done) return;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < nb; i += 256) {
@@ -360,8 +403,16 @@ __global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ part
 template <typename R>
 __global__ __launch_bounds__(256) void k_kry_upd1(const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ v,
                                                   const cplx<R>* __restrict__ p, cplx<R>* __restrict__ s,
-                                                  cplx<R>* __restrict__ x, long long n, const KScal* __restrict__ ks) {
+                                                  cplx<R>* __restrict__ x, long long n, const KScal* __restrict__ ks,
+                                                  long long vs) {
+  const int z = blockIdx.y;
+  ks += z;
   if (ks->done) return;
+  r += z * vs;
+  v += z * vs;
+  p += z * vs;
+  s += z * vs;
+  x += z * vs;
   const R ar = (R)ks->ar, ai = (R)ks->ai;
   const cplx<R> ma = cplx<R>{-ar, -ai};
   const cplx<R> al = cplx<R>{ar, ai};
@@ -375,9 +426,17 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_kry_upd2(const cplx<R>* __restrict__ s, const cplx<R>* __restrict__ t,
                                                   const cplx<R>* __restrict__ c, cplx<R>* __restrict__ x,
                                                   cplx<R>* __restrict__ r, long long n, const KScal* __restrict__ ks,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ partials, long long vs, long long ps) {
   __shared__ double scratch[3 * 4];
+  const int z = blockIdx.y;
+  ks += z;
   if (ks->done) return;
+  s += z * vs;
+  t += z * vs;
+  c += z * vs;
+  x += z * vs;
+  r += z * vs;
+  partials += z * ps;
   const R wr = (R)ks->wr, wi = (R)ks->wi;
   const cplx<R> om = cplx<R>{wr, wi};
   const cplx<R> mom = cplx<R>{-wr, -wi};
@@ -402,8 +461,14 @@ __global__ __launch_bounds__(256) void k_kry_upd2(const cplx<R>* __restrict__ s,
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_kry_upd3(const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ v,
-                                                  cplx<R>* __restrict__ p, long long n, const KScal* __restrict__ ks) {
+                                                  cplx<R>* __restrict__ p, long long n, const KScal* __restrict__ ks,
+                                                  long long vs) {
+  const int z = blockIdx.y;
+  ks += z;
   if (ks->done) return;
+  r += z * vs;
+  v += z * vs;
+  p += z * vs;
   const cplx<R> mom = cplx<R>{(R)-ks->wr, (R)-ks->wi};
   const cplx<R> be = cplx<R>{(R)ks->br, (R)ks->bi};
   const cplx<R> one = cplx<R>{R(1), R(0)};
@@ -445,37 +510,42 @@ static void tev_post(KrylovCtx<R>& k, int id, double bytes) {
   }
 }
 
-// Accumulates the timed launches that belonged to real iterations (<= iters).
+// Accumulates the timed launches that did work: a launch queued for iteration i
+// counts the bytes of the systems whose solve ran at least i iterations.
 template <typename R>
-static void tev_collect(KrylovCtx<R>& k, int iters) {
+static void tev_collect(KrylovCtx<R>& k, const int* iters, int nsys) {
   for (int i = 0; i < k.ev_used; ++i) {
-    if (k.ev_iter[i] > iters) continue;
+    int active = 0;
+    for (int z = 0; z < nsys; ++z) active += k.ev_iter[i] <= iters[z] ? 1 : 0;
+    if (active == 0) continue;
     float ms = 0.f;
     SAV_HIP(hipEventElapsedTime(&ms, k.ev[2 * i], k.ev[2 * i + 1]));
     k.tk_ms += ms;
-    k.tk_bytes += k.ev_bytes[i];
+    k.tk_bytes += k.ev_bytes[i] * active;
     k.tk_count += 1;
   }
   k.ev_used = 0;
 }
 
+// One matvec on nsys systems (vectors k.vs apart, KScal slots slot .. slot+nsys-1).
 // mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
 template <typename R>
 static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
-                         const cplx<R>* partner = nullptr) {
+                         const cplx<R>* partner = nullptr, int nsys = 1) {
   const KScal* ks = k.ks + slot;
+  const long long vs = k.vs, ps = k.ps;
   int nb3 = 0;
   const double sn = (double)sizeof(R) * (double)k.nfull;
   hipEvent_t e0, e1;
   tev_pre(k, 1, e0, e1);
   switch (k.Ny) {
-#define X(L)                                                                                                  \
-  case L: {                                                                                                   \
-    using CC = KColCfg<R, L>;                                                                                 \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);                                                               \
-    hipExtLaunchKernelGGL((k_mv_cols_inv<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx, \
-                          k.S4, k.kx0, k.ky, k.tw_y, ks);                                                     \
-    break;                                                                                                    \
+#define X(L)                                                                                                      \
+  case L: {                                                                                                       \
+    using CC = KColCfg<R, L>;                                                                                     \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);                                                             \
+    hipExtLaunchKernelGGL((k_mv_cols_inv<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx,     \
+                          k.S4, k.kx0, k.ky, k.tw_y, ks, vs);                                                     \
+    break;                                                                                                        \
   }
     SAV_KRY_SIZES(X)
 #undef X
@@ -487,13 +557,13 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   const long long nrh = (long long)k.N * k.Ny;
   const double scale = 1.0 / ((double)k.Nx * k.Ny);
   switch (k.Nx) {
-#define X(L)                                                                                             \
-  case L: {                                                                                              \
-    using CC = KRowCfg<R, L>;                                                                            \
-    dim3 grid((unsigned)((2 * nrh + CC::ROWS - 1) / CC::ROWS));                                          \
+#define X(L)                                                                                                \
+  case L: {                                                                                                 \
+    using CC = KRowCfg<R, L>;                                                                               \
+    dim3 grid((unsigned)((2 * nrh + CC::ROWS - 1) / CC::ROWS), nsys);                                       \
     hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, k.M, nrh,  \
-                          scale, k.tw_x, ks);                                                            \
-    break;                                                                                               \
+                          scale, k.tw_x, ks, vs);                                                           \
+    break;                                                                                                  \
   }
     SAV_KRY_SIZES(X)
 #undef X
@@ -504,22 +574,22 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   tev_pre(k, 3, e0, e1);
 #define KL(L, M)                                                                                              \
   hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, y, partner, \
-                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials)
+                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials, vs, ps)
   switch (k.Ny) {
-#define X(L)                                      \
-  case L: {                                       \
-    using CC = KColCfg<R, L>;                     \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N);   \
-    nb3 = (int)(grid.x * grid.y);                 \
-    switch (mode) {                               \
-      case 1: KL(L, 1); break;                    \
-      case 2: KL(L, 2); break;                    \
-      case 4: KL(L, 4); break;                    \
-      case 5: KL(L, 5); break;                    \
-      case 6: KL(L, 6); break;                    \
-      default: KL(L, 0); break;                   \
-    }                                             \
-    break;                                        \
+#define X(L)                                          \
+  case L: {                                           \
+    using CC = KColCfg<R, L>;                         \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys); \
+    nb3 = (int)(grid.x * grid.y);                     \
+    switch (mode) {                                   \
+      case 1: KL(L, 1); break;                        \
+      case 2: KL(L, 2); break;                        \
+      case 4: KL(L, 4); break;                        \
+      case 5: KL(L, 5); break;                        \
+      case 6: KL(L, 6); break;                        \
+      default: KL(L, 0); break;                       \
+    }                                                 \
+    break;                                            \
   }
     SAV_KRY_SIZES(X)
 #undef X
@@ -548,40 +618,50 @@ void matvec_plain_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot) 
 }
 
 template <typename R>
-int bicgstab_2d(KrylovCtx<R>& k, int slot) {
+static void fin_launch(KrylovCtx<R>& k, int nb, int slot, int stage, int nsys) {
+  hipLaunchKernelGGL(k_kry_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nb, k.ks + slot, stage, k.atol, k.rtol,
+                     k.itmax, k.ps);
+}
+
+template <typename R>
+int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
   const long long n = k.nfull;
   const int nbv = vec_blocks(n);
+  const long long vs = k.vs, ps = k.ps;
   KScal* ks = k.ks + slot;
-  hipLaunchKernelGGL((k_kry_init<R>), dim3(nbv), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4, k.kx0,
-                     k.ky, k.ds, k.partials);
+  hipLaunchKernelGGL((k_kry_init<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4,
+                     k.kx0, k.ky, k.ds, k.partials, vs, ps);
   SAV_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 0, k.atol, k.rtol, k.itmax);
+  fin_launch(k, nbv, slot, 0, nsys);
   SAV_LAUNCH_CHECK();
   int queued = 0;
-  const bool logging = k.snap != nullptr;
-  int batch = logging ? 1 : (k.guess[slot] > 0 ? k.guess[slot] : 8);
+  const bool logging = k.snap != nullptr && nsys == 1;
+  int guess = 0;
+  for (int z = 0; z < nsys; ++z) guess = std::max(guess, k.guess[slot + z]);
+  int batch = logging ? 1 : (guess > 0 ? guess : 8);
   k.snap_n = 0;
   while (true) {
     const int nb = batch < k.itmax - queued ? batch : k.itmax - queued;
     for (int it = 0; it < nb; ++it) {
       k.cur_iter = queued + it + 1;
       // v = P^{-1} A p ; alpha = rho / <c, v>
-      int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b);
-      hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 1, k.atol, k.rtol, k.itmax);
+      int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b, nsys);
+      fin_launch(k, nb3, slot, 1, nsys);
       // s = r - alpha v ; x += alpha p
-      hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks);
+      hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks, vs);
       // t = P^{-1} A s ; omega = <t, s> / <t, t>
-      nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s);
-      hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 2, k.atol, k.rtol, k.itmax);
+      nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s, nsys);
+      fin_launch(k, nb3, slot, 2, nsys);
       // x += omega s ; r = s - omega t ; <c, r>, ||r||
-      hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks, k.partials);
-      hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 3, k.atol, k.rtol, k.itmax);
+      hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks,
+                         k.partials, vs, ps);
+      fin_launch(k, nbv, slot, 3, nsys);
       // p = r + beta (p - omega v)
-      hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks);
+      hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks, vs);
       SAV_LAUNCH_CHECK();
     }
     queued += nb;
-    SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
+    SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, nsys * sizeof(KScal), hipMemcpyDeviceToHost, k.st));
     SAV_HIP(hipStreamSynchronize(k.st));
     if (logging && k.snap_n < k.snap_cap && k.hks[slot].iter > k.snap_n) {
       // iterate x_k after iteration k (only iterations that ran)
@@ -589,13 +669,25 @@ int bicgstab_2d(KrylovCtx<R>& k, int slot) {
                              k.st));
       k.snap_n++;
     }
-    if (k.hks[slot].done || queued >= k.itmax) break;
+    bool all_done = true;
+    for (int z = 0; z < nsys; ++z) all_done = all_done && k.hks[slot + z].done;
+    if (all_done || queued >= k.itmax) break;
     if (!logging) batch = 2;
   }
-  const int iters = k.hks[slot].iter;
-  k.guess[slot] = iters > 0 ? iters : 1;
-  if (k.timed) tev_collect(k, iters);
-  return iters;
+  int iters[4] = {0, 0, 0, 0};
+  int maxit = 0;
+  for (int z = 0; z < nsys; ++z) {
+    iters[z] = k.hks[slot + z].iter;
+    k.guess[slot + z] = iters[z] > 0 ? iters[z] : 1;
+    maxit = std::max(maxit, iters[z]);
+  }
+  if (k.timed) tev_collect(k, iters, nsys);
+  return maxit;
+}
+
+template <typename R>
+int bicgstab_2d(KrylovCtx<R>& k, int slot) {
+  return bicgstab_batch(k, slot, 1);
 }
 
 // Flexible (right-preconditioned) BiCGSTAB with a possibly nonlinear preconditioner
@@ -671,7 +763,7 @@ int fbicgstab_2d(KrylovCtx<R>& k, int slot, const std::function<void(const cplx<
   KScal* ks = k.ks + slot;
   hipLaunchKernelGGL((k_fkry_init<R>), dim3(nbv), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4,
                      k.kx0, k.ky, k.ds, k.partials);
-  hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 0, k.atol, k.rtol, k.itmax);
+  fin_launch(k, nbv, slot, 0, 1);
   SAV_LAUNCH_CHECK();
   SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
   SAV_HIP(hipStreamSynchronize(k.st));
@@ -679,15 +771,15 @@ int fbicgstab_2d(KrylovCtx<R>& k, int slot, const std::function<void(const cplx<
     k.cur_iter = it + 1;
     prec(k.p, k.Bp);                                        // p_hat
     int nb3 = matvec_launch(k, k.Bp, k.v, 5, slot, k.b);    // v = A p_hat, <c, v>
-    hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 1, k.atol, k.rtol, k.itmax);
-    hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.Bp, k.s, k.x, n, ks);
+    fin_launch(k, nb3, slot, 1, 1);
+    hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.Bp, k.s, k.x, n, ks, 0LL);
     prec(k.s, k.Bs);                                        // s_hat
     nb3 = matvec_launch(k, k.Bs, k.t, 6, slot, k.s);        // t = A s_hat, <t, s>, <t, t>
-    hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nb3, ks, 2, k.atol, k.rtol, k.itmax);
+    fin_launch(k, nb3, slot, 2, 1);
     hipLaunchKernelGGL((k_fkry_upd2<R>), dim3(nbv), dim3(256), 0, k.st, k.s, k.Bs, k.t, k.b, k.x, k.r, n, k.Nx,
                        k.Ny, k.S4, k.kx0, k.ky, k.ds, ks, k.partials);
-    hipLaunchKernelGGL(k_kry_fin, dim3(1), dim3(256), 0, k.st, k.partials, nbv, ks, 3, k.atol, k.rtol, k.itmax);
-    hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks);
+    fin_launch(k, nbv, slot, 3, 1);
+    hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks, 0LL);
     SAV_LAUNCH_CHECK();
     SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, sizeof(KScal), hipMemcpyDeviceToHost, k.st));
     SAV_HIP(hipStreamSynchronize(k.st));
@@ -702,6 +794,8 @@ template void matvec_plain_2d<double>(KrylovCtx<double>&, const cplx<double>*, c
 template void matvec_plain_2d<float>(KrylovCtx<float>&, const cplx<float>*, cplx<float>*, int);
 template int bicgstab_2d<double>(KrylovCtx<double>&, int);
 template int bicgstab_2d<float>(KrylovCtx<float>&, int);
+template int bicgstab_batch<double>(KrylovCtx<double>&, int, int);
+template int bicgstab_batch<float>(KrylovCtx<float>&, int, int);
 template void matvec_prec_2d<double>(KrylovCtx<double>&, const cplx<double>*, cplx<double>*, int);
 template void matvec_prec_2d<float>(KrylovCtx<float>&, const cplx<float>*, cplx<float>*, int);
 

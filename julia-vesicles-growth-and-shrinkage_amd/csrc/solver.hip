@@ -227,15 +227,18 @@ class Solver final : public ISolver {
       kc.atol = c.atol;
       kc.rtol = c.tol;
       kc.itmax = c.itmax;
-      kc.x = dalloc<C>(nf);
-      kc.r = dalloc<C>(nf);
-      kc.p = dalloc<C>(nf);
-      kc.v = dalloc<C>(nf);
-      kc.s = dalloc<C>(nf);
-      kc.t = dalloc<C>(nf);
-      kc.b = dalloc<C>(nf);
-      kc.Bx = dalloc<C>(nf);
-      kc.By = dalloc<C>(nf);
+      // three systems (the batched step-1 solves), nf apart
+      kc.vs = nf;
+      kc.ps = 3 * 4096;  // >= 3 x (blocks of any Krylov launch per system)
+      kc.x = dalloc<C>(3 * nf);
+      kc.r = dalloc<C>(3 * nf);
+      kc.p = dalloc<C>(3 * nf);
+      kc.v = dalloc<C>(3 * nf);
+      kc.s = dalloc<C>(3 * nf);
+      kc.t = dalloc<C>(3 * nf);
+      kc.b = dalloc<C>(3 * nf);
+      kc.Bx = dalloc<C>(3 * nf);
+      kc.By = dalloc<C>(3 * nf);
       kc.partials = partials;
       kc.ks = dalloc<KScal>(4);
       kc.hks = hks;
@@ -517,6 +520,34 @@ class Solver final : public ISolver {
     });
     if (collecting) harvest(kc.snap_n);
     return it;
+  }
+
+  // The three psi solves of step 1 (rhs: 3 consecutive half spectra of N fields) as one
+  // batched BiCGSTAB: systems z = 0, 1, 2 use KScal slots 0, 1, 2 and the vectors at
+  // offset z * kc.vs.  Each system's iterates equal those of its own solve.
+  void psi_solve3(const C* rhs, C* o0, C* o1, C* o2) {
+    const long long NP = (long long)N * P, nf = kc.nfull;
+    R* rphys = reinterpret_cast<R*>(kc.Bx);  // 3 N real fields of scratch before the solve
+    irfft({job(rhs, rphys, 3 * N)});
+    C* b = kc.b;
+    const long long vs = kc.vs;
+    foreach_n(s, 3 * NP, [=] __device__(long long i) {
+      const long long z = i / NP, j = i % NP;
+      b[z * vs + j] = CD{rphys[i], T(0)};
+    });
+    fft_full_2d<R>(g, b, 3 * N, false, s);  // vs == nf: the three systems are contiguous
+    bicgstab_batch<R>(kc, 0, 3);
+    const C* x = kc.x;
+    const int Nxl = Nx, Nxhl = Nxh;
+    const long long nh = (long long)N * H;
+    foreach_n(s, 3 * nh, [=] __device__(long long i) {
+      const long long z = i / nh, j = i % nh;
+      const long long row = j / Nxhl;
+      const int ix = (int)(j % Nxhl);
+      C* o = z == 0 ? o0 : (z == 1 ? o1 : o2);
+      o[j] = x[z * vs + row * Nxl + ix];
+    });
+    (void)nf;
   }
 
   // ==========================================================================
@@ -1062,9 +1093,15 @@ class Solver final : public ISolver {
     }
     // psi solves
     if (nd == 2) {
-      kit[0] = psi_solve(w_rhs, cf(PSI11), 0);
-      kit[1] = psi_solve(w_rhs + NH, cf(PSI12), 1);
-      kit[2] = psi_solve(w_rhs + 2 * NH, cf(PSI21), 2);
+      if (!use_learned && !collecting) {
+        // the three solves share the operator: one batched BiCGSTAB (Solvers.jl:73 x 3)
+        psi_solve3(w_rhs, cf(PSI11), cf(PSI12), cf(PSI21));
+        for (int z = 0; z < 3; ++z) kit[z] = hks[z].iter;
+      } else {
+        kit[0] = psi_solve(w_rhs, cf(PSI11), 0);
+        kit[1] = psi_solve(w_rhs + NH, cf(PSI12), 1);
+        kit[2] = psi_solve(w_rhs + 2 * NH, cf(PSI21), 2);
+      }
     } else {
       // spectral divide by a/dt - M0*S4*Lap (Solvers_3d.jl:30,40,57,88)
       const C* r = w_rhs;
