@@ -632,21 +632,124 @@ def time_nodes(dt: float, T: float) -> np.ndarray:
     return np.concatenate([t1, t2[1:]])
 
 
+# ---------------------------------------------------------------------------
+# Snapshots (simulation.jl:112-143) and checkpoints
+# ---------------------------------------------------------------------------
+SNAPSHOT_KEYS = ("phi", "psi", "mu", "nu", "p", "u", "R1", "R2", "R3", "Q")
+
+
+def save_steps(Dt: np.ndarray, save_interval: float) -> List[int]:
+    """1-based loop indices n after which run_simulation writes a snapshot: the rule of
+    simulation.jl:113-114, curr = floor((Dt[n] + 1e-12) / save_interval), save when it
+    exceeds the last saved index (initially -1).  Dt[n] is the time at the START of step
+    n (Julia 1-based), so the snapshot holds the state after step n under that time."""
+    out, last = [], -1
+    for n in range(1, len(Dt)):
+        cur = math.floor((float(Dt[n - 1]) + 1e-12) / save_interval)
+        if cur > last:
+            out.append(n)
+            last = cur
+    return out
+
+
+def snapshot_name(conf: Config, t: float) -> str:
+    """state_2d_t<t>.mat / state_3d_t<t>.mat with t printed as %.2e (simulation.jl:115,134-137)."""
+    return f"state_{conf.ndim}d_t{t:.2e}.mat"
+
+
+def write_snapshot(path: os.PathLike, st: FieldState):
+    """The keys matwrite stores (simulation.jl:118-129), arrays in the Julia shapes.  MAT
+    version 5 (scipy.io); MAT.jl's matread and MATLAB read it like the v7.3 files of the
+    reference."""
+    import scipy.io
+    scipy.io.savemat(str(path), {k: getattr(st, k) for k in SNAPSHOT_KEYS}, do_compression=True,
+                     oned_as="column")
+
+
+def read_snapshot(path: os.PathLike) -> dict:
+    import scipy.io
+    d = scipy.io.loadmat(str(path))
+    return {k: (float(d[k].ravel()[0]) if k in ("R1", "R2", "R3", "Q") else d[k]) for k in SNAPSHOT_KEYS}
+
+
+def write_checkpoint(path: os.PathLike, present: FieldState, old: FieldState, n_next: int, t_next: float):
+    """Everything a resume needs (no reference counterpart: the reference's snapshots drop
+    `old`, A0 and the spectra): both states with spectra, Q/R, A0, dt, and the next loop
+    index and time.  numpy .npz, nothing executable inside."""
+    data = {"n_next": np.array(n_next), "t_next": np.array(t_next)}
+    for tag, st in (("present", present), ("old", old)):
+        for k in FieldState.ARRAYS:
+            data[f"{tag}_{k}"] = np.asarray(getattr(st, k))
+        data[f"{tag}_scal"] = np.array([st.Q, st.R1, st.R2, st.R3, st.dt])
+        data[f"{tag}_A0"] = np.asarray(st.A0)
+    np.savez_compressed(str(path), **data)
+
+
+def read_checkpoint(path: os.PathLike):
+    d = np.load(str(path), allow_pickle=False)
+    out = []
+    for tag in ("present", "old"):
+        kw = {k: np.asfortranarray(d[f"{tag}_{k}"]) for k in FieldState.ARRAYS}
+        Q, R1, R2, R3, dt = (float(v) for v in d[f"{tag}_scal"])
+        out.append(FieldState(**kw, Q=Q, R1=R1, R2=R2, R3=R3, A0=np.array(d[f"{tag}_A0"]), dt=dt))
+    return out[0], out[1], int(d["n_next"]), float(d["t_next"])
+
+
 def run_simulation(dt_val: float, T_val: float, state_type: int, save_path: Optional[str] = None,
                    save_interval: float = 0.001, conf: Optional[Config] = None,
                    state: Optional[FieldState] = None, t_nodes: Optional[np.ndarray] = None,
-                   precision: int = SAV_PREC_FP64):
+                   precision: int = SAV_PREC_FP64, checkpoint: bool = False):
     """Returns (present, energy_history, area_ratio_history, Dt) like simulation.jl:154.
 
     With ``conf``/``state``/``t_nodes`` left out it runs the reference's own set-up
-    (set_para_base, generate_initial_condition, two-phase Dt schedule)."""
+    (set_para_base, generate_initial_condition, two-phase Dt schedule).  With
+    ``save_path`` it writes the reference's snapshots (``save_path/MAT/case3/``, the
+    schedule of simulation.jl:113-114; no plots) and, with ``checkpoint``, a resumable
+    checkpoint next to each snapshot."""
     conf = conf or set_para_base(dt_val, T_val)
     ops = build_operators(conf, precision=precision)
     Dt = time_nodes(conf.dt, T_val) if t_nodes is None else np.asarray(t_nodes, dtype=np.float64)
     present = state if state is not None else generate_initial_condition(conf, ops, state_type)
     ops.set_state(SAV_PRESENT, present)
     ops.init_sav()
-    rc, E, A, kit = ops.run(Dt, 1)
+    E, A = [], []
+    if save_path is None:
+        rc, e, a, kit = ops.run(Dt, 1)
+        E, A = list(e), list(a)
+    else:
+        mat_folder = Path(save_path) / "MAT" / "case3"
+        mat_folder.mkdir(parents=True, exist_ok=True)
+        saves = set(save_steps(Dt, save_interval))
+        n0 = 1
+        for n in sorted(saves | {len(Dt) - 1}):
+            # steps n0 .. n (loop indices), time nodes Dt[n0-1] .. Dt[n]
+            rc, e, a, kit = ops.run(Dt[n0 - 1: n + 1], n0)
+            E += list(e)
+            A += list(a)
+            n0 = n + 1
+            if n in saves:
+                st = ops.get_state(SAV_PRESENT)
+                write_snapshot(mat_folder / snapshot_name(conf, float(Dt[n - 1])), st)
+                if checkpoint:
+                    write_checkpoint(mat_folder / (snapshot_name(conf, float(Dt[n - 1]))[:-4] + ".ckpt.npz"),
+                                     st, ops.get_state(SAV_OLD), n + 1, float(Dt[n]))
     present = ops.get_state(SAV_PRESENT)
     ops.close()
-    return present, list(E), list(A), Dt
+    return present, E, A, Dt
+
+
+def resume_simulation(checkpoint_path: os.PathLike, conf: Config, t_nodes: np.ndarray,
+                      precision: int = SAV_PREC_FP64):
+    """Continues a run from write_checkpoint's file over the remaining time nodes
+    (t_nodes[0] must be the checkpoint's time); returns (present, E, A)."""
+    present, old, n_next, t_next = read_checkpoint(checkpoint_path)
+    t = np.asarray(t_nodes, dtype=np.float64)
+    if abs(t[0] - t_next) > 1e-12 * max(1.0, abs(t_next)):
+        raise ValueError(f"t_nodes[0] = {t[0]} is not the checkpoint time {t_next}")
+    ops = build_operators(conf, precision=precision)
+    ops.set_state(SAV_PRESENT, present)
+    ops.set_state(SAV_OLD, old)
+    rc, E, A, kit = ops.run(t, n_next)
+    out = ops.get_state(SAV_PRESENT)
+    ops.close()
+    return out, list(E), list(A)
