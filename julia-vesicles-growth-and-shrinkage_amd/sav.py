@@ -536,9 +536,48 @@ def to_physical(u_hat: np.ndarray, ops: Operators) -> np.ndarray:
 # Initial conditions (src/Init.jl:74-217), with the unseeded rand() of
 # state_type 4 replaced by a seeded SplitMix64 stream.
 # ---------------------------------------------------------------------------
+def initial_fields_3d(conf: Config, state_type: int):
+    """(phi, psi) in Julia shape (Nx, Ny, Nz, N) of Init_3d.jl:67-153.  N = 1, state_type 1: the ellipsoid
+    tanh((0.2 Lx - sqrt((x-cx)^2/2 + (y-cy)^2/2 + (z-cz)^2)) / (sqrt 2 eps)).
+    N = 2, state_type 7: two star shapes centred at x = 0.73 and 1.27.  The reference's
+    branch indexes 2D grids (X, Y) and a 3D slice phi[:, :, n] of a 4D array, which Julia
+    rejects; here the same formula R0 + amp cos(k theta) - r is evaluated with the 3D
+    distance r and the azimuth theta = atan(x - cx, y - cy)."""
+    Nx, Ny, Nz, N = conf.Nx, conf.Ny, conf.Nz, conf.N
+    Lx, Ly, Lz, eps = conf.Lx, conf.Ly, conf.Lz, conf.epsilon
+    x = np.linspace(0, Lx - Lx / Nx, Nx)[:, None, None]
+    y = np.linspace(0, Ly - Ly / Ny, Ny)[None, :, None]
+    z = np.linspace(0, Lz - Lz / Nz, Nz)[None, None, :]
+    s2e = math.sqrt(2) * eps
+    phi = np.zeros((Nx, Ny, Nz, N), order="F")
+    if N == 1:
+        if state_type != 1:
+            raise ValueError(f"state_type {state_type} has no 3D shape for N = 1 in the reference")
+        R = 0.2 * Lx
+        dist = np.sqrt((x - Lx / 2) ** 2 / 2 + (y - Ly / 2) ** 2 / 2 + (z - Lz / 2) ** 2)
+        phi[..., 0] = np.tanh((R - dist) / s2e)
+    elif N == 2:
+        if state_type != 7:
+            raise ValueError(f"state_type {state_type} invalid for N = 2 (the reference defines 7)")
+        if conf.goal == "s":
+            raise ValueError("state_type 7 needs goal g (Init_3d.jl)")
+        for n, cxn in enumerate((0.73, 1.27)):
+            r = np.sqrt((x - cxn) ** 2 + (y - Ly / 2) ** 2 + (z - Lz / 2) ** 2)
+            th = np.arctan2(x - cxn, y - Ly / 2)
+            phi[..., n] = np.tanh((0.2 + 0.02 * np.cos(10 * th) - r) / s2e)
+    else:
+        raise ValueError(f"N = {N} has no 3D shape in the reference")
+    psi = np.zeros_like(phi, order="F")
+    for n in range(N):
+        psi[..., n] = (-0.1 * phi[..., n] + 0.7) if conf.goal == "s" else (-0.35 * phi[..., n] + 0.45)
+    return phi, psi
+
+
 def generate_initial_condition(conf: Config, ops: Optional[Operators], state_type: int, seed: int = 1234):
-    if conf.ndim != 2:
-        raise ValueError("generate_initial_condition: 2D shapes only (the 3D branch of Init_3d.jl is broken)")
+    """generate_initial_condition of Init.jl:74-217 (2D) and Init_3d.jl:67-153 (3D)."""
+    if conf.ndim == 3:
+        phi, psi = initial_fields_3d(conf, state_type)
+        return state_from_physical(conf, ops, phi, psi)
     Nx, Ny, N = conf.Nx, conf.Ny, conf.N
     Lx, Ly, eps = conf.Lx, conf.Ly, conf.epsilon
     x = np.linspace(0, Lx - conf.dx, Nx)
@@ -622,7 +661,7 @@ def julia_shape(conf: Config, a: np.ndarray) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
-# run_simulation (src/simulation.jl:2-155) without plots and .mat output
+# run_simulation (src/simulation.jl:2-155) without plots
 # ---------------------------------------------------------------------------
 def time_nodes(dt: float, T: float) -> np.ndarray:
     """Two-phase schedule of simulation.jl:15-19: 0:1e-7:1e-5, then 1e-5:dt:T (first dropped)."""

@@ -62,3 +62,21 @@ def test_checkpoint_roundtrip(jv, tmp_path):
         for k in jv.FieldState.ARRAYS:
             assert np.array_equal(getattr(x, k), getattr(y, k))
         assert (x.Q, x.R1, x.R2, x.R3, x.dt) == (y.Q, y.R1, y.R2, y.R3, y.dt)
+
+
+def test_initial_fields_3d(jv):
+    base = jv.workload_c4(n=32, steps=1).conf
+    conf = base.with_(N=1, goal="s", psi_in_v=[1.0], psi_out_v=[1.0])
+    phi, psi = jv.initial_fields_3d(conf, 1)
+    assert phi.shape == (32, 32, 32, 1)
+    assert phi[16, 16, 16, 0] > 0.99 and phi[0, 0, 0, 0] < -0.99
+    assert np.allclose(psi, -0.1 * phi + 0.7)
+    # the two-star shape is sized for the unit cube (Init_3d.jl centres at x = 0.73, 1.27)
+    conf2 = jv.workload_c4(n=64, steps=1).conf.with_(N=2, goal="g", Lx=2.0, Ly=1.0, Lz=1.0, epsilon=0.01)
+    phi2, psi2 = jv.initial_fields_3d(conf2, 7)
+    i73 = int(round(0.73 / (conf2.Lx / 64)))
+    assert phi2[i73, 32, 32, 0] > 0.99 and phi2[i73, 32, 32, 1] < -0.99
+    assert np.allclose(psi2, -0.35 * phi2 + 0.45)
+    import pytest
+    with pytest.raises(ValueError):
+        jv.initial_fields_3d(conf2.with_(goal="s"), 7)
