@@ -51,9 +51,27 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_client()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+CLIENT_SRC = PKG / "abi_client.cpp"
+CLIENT = PKG / "sav_abi_client"
+
+
+def build_client() -> Path:
+    """The plain C++ ABI client (host compiler only, linked against libsav_hip.so)."""
+    cxx = shutil.which("g++") or "g++"
+    tmp = CLIENT.with_suffix(".tmp")
+    cmd = [cxx, "-O2", "-std=c++17", str(CLIENT_SRC), "-o", str(tmp), f"-L{PKG}", "-lsav_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"abi client build failed:\n{r.stderr}")
+    os.replace(tmp, CLIENT)
+    return CLIENT
 
 
 if __name__ == "__main__":
