@@ -275,12 +275,12 @@ inline std::vector<cplx<R>> default_twiddles(int L) {
 // Columns per workgroup of a strided (column) transform: 8 adjacent columns (128-byte
 // row segments in FP64, 64-byte in FP32), at least 256 threads per workgroup, at most
 // 1024 threads and the 160 KB of LDS of a gfx950 CU.
-template <typename R, int L>
+template <typename R, int L, int WANT = 8>
 struct ColTile {
   using F = TeamFFT<R, L, false>;
   static constexpr int TT = F::TT;
   static constexpr int ES = (int)sizeof(cplx<R>);
-  static constexpr int want = 8 > (256 / TT) ? 8 : (256 / TT);
+  static constexpr int want = WANT > (256 / TT) ? WANT : (256 / TT);
   static constexpr int cap0 = 163840 / (L * ES);
   static constexpr int capp2 = cap0 >= 256 ? 256 : cap0 >= 128 ? 128 : cap0 >= 64 ? 64 : cap0 >= 32 ? 32 : cap0 >= 16 ? 16 : cap0 >= 8 ? 8 : cap0 >= 4 ? 4 : cap0 >= 2 ? 2 : 1;
   static constexpr int C0 = want < capp2 ? want : capp2;
