@@ -79,6 +79,7 @@ def run_case(jv, wl, precision, steps, warmup, timed_kernel=0):
     if timed_kernel:
         n, ms, b = ops.kernel_timing_get()
         res["kernel"] = dict(name=ops.KERNELS[timed_kernel], launches=n, avg_ms=ms / max(n, 1), bytes=b)
+    res["autotune"] = ops.autotune_info() if conf.ndim == 2 else None
     st = ops.get_state(jv.SAV_PRESENT)
     ops.close()
     return res, st
@@ -262,6 +263,8 @@ def main():
         line["psi_iters_per_step"] = {"fp32": out["2d"]["fp32"]["krylov_iters"] / a.steps,
                                       "fp64": out["2d"]["fp64"]["krylov_iters"] / a.steps}
         line["fp64_steps_per_s_1024"] = round(out["2d"]["fp64"]["steps_per_s"], 3)
+        line["psi_matvec_column_tile_1024"] = {"fp32": out["2d"]["fp32"]["autotune"],
+                                               "fp64": out["2d"]["fp64"]["autotune"]}
         if "fp32_learned" in out["2d"]:
             L = out["2d"]["fp32_learned"]
             line["learned_precond_1024"] = {

@@ -199,6 +199,17 @@ int sav_last_elapsed_ms(sav_handle_t* h, double* ms) {
 }
 
 
+int sav_set_column_tile(sav_handle_t* h, int columns) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->set_column_tile(columns));
+}
+
+int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16) {
+  if (!h) return SAV_E_ARG;
+  int cw = 0;
+  SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16); if (columns) *columns = cw; return SAV_OK);
+}
+
 int sav_set_precond(sav_handle_t* h, int precond) {
   if (!h) return SAV_E_ARG;
   SAV_GUARD(h, return h->s->set_precond(precond));

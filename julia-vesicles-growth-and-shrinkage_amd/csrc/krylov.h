@@ -39,6 +39,9 @@ struct KrylovCtx {
   // iterate snapshots for logging training pairs (nullptr: off)
   cplx<R>* snap = nullptr;
   int snap_cap = 0, snap_n = 0;
+  // columns per workgroup of the matvec's column kernels: 0 = not tuned yet, 8 or 16
+  int colw = 0;
+  double tune_ms[2] = {0.0, 0.0};  // measured matvec time with 8 and with 16 columns
   // preconditioned directions of the flexible solver
   cplx<R>* Bp = nullptr;
   cplx<R>* Bs = nullptr;

@@ -23,8 +23,10 @@
 
 namespace sav {
 
-template <typename R, int L>
-using KColCfg = ColTile<R, L>;
+// Column kernels of the matvec: CW adjacent columns per workgroup (8 or 16; chosen per
+// handle by timing both at the first solve, see autotune_columns).
+template <typename R, int L, int CW = 8>
+using KColCfg = ColTile<R, L, CW>;
 
 // Row kernel: one team per row, 256 threads per workgroup; the LDS of each row is
 // padded by one slot every 16 (FP64) or 32 (FP32) positions against bank conflicts.
@@ -50,8 +52,8 @@ __device__ __forceinline__ double pinv_diag(double a_dt, double Mmax, double S4,
 // ---------------------------------------------------------------------------
 // K1: column inverse FFTs of D1f_x*(S4 x) and D1f_y*(S4 x)
 // ---------------------------------------------------------------------------
-template <typename R, int L>
-__global__ __launch_bounds__((KColCfg<R, L>::NT), 4) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
+template <typename R, int L, int CW>
+__global__ __launch_bounds__((KColCfg<R, L, CW>::NT), 4) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
                                                                     cplx<R>* __restrict__ Bx,
                                                                     cplx<R>* __restrict__ By, int Nx, double S4,
                                                                     const double* __restrict__ kx0,
@@ -66,7 +68,7 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT), 4) void k_mv_cols_inv(const cp
   Bx += z * vs;
   By += z * vs;
   using F = TeamFFT<R, L, true>;
-  constexpr int C = KColCfg<R, L>::C;
+  constexpr int C = KColCfg<R, L, CW>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -165,8 +167,8 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
 //   MODE 2: out = P^{-1} A y and partials <out, y>, <out, out>
 //   MODE 0: out only
 // ---------------------------------------------------------------------------
-template <typename R, int L, int MODE>
-__global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_fwd(
+template <typename R, int L, int MODE, int CW>
+__global__ __launch_bounds__((KColCfg<R, L, CW>::NT)) void k_mv_cols_fwd(
     const cplx<R>* __restrict__ Bx, const cplx<R>* __restrict__ By, const cplx<R>* __restrict__ Y,
     const cplx<R>* __restrict__ Cv, cplx<R>* __restrict__ out, int Nx, double S4, const double* __restrict__ kx0,
     const double* __restrict__ ky, const cplx<R>* __restrict__ tw, const DScal* __restrict__ ds,
@@ -181,7 +183,7 @@ __global__ __launch_bounds__((KColCfg<R, L>::NT)) void k_mv_cols_fwd(
   out += z * vs;
   partials += z * ps;
   using F = TeamFFT<R, L, false>;
-  constexpr int C = KColCfg<R, L>::C;
+  constexpr int C = KColCfg<R, L, CW>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
   __shared__ double scratch[3 * 16];
@@ -527,31 +529,73 @@ static void tev_collect(KrylovCtx<R>& k, const int* iters, int nsys) {
   k.ev_used = 0;
 }
 
-// One matvec on nsys systems (vectors k.vs apart, KScal slots slot .. slot+nsys-1).
-// mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
-template <typename R>
-static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
-                         const cplx<R>* partner = nullptr, int nsys = 1) {
-  const KScal* ks = k.ks + slot;
-  const long long vs = k.vs, ps = k.ps;
-  int nb3 = 0;
-  const double sn = (double)sizeof(R) * (double)k.nfull;
-  hipEvent_t e0, e1;
-  tev_pre(k, 1, e0, e1);
+template <typename R, int CW>
+static void launch_cols_inv(KrylovCtx<R>& k, const cplx<R>* y, const KScal* ks, int nsys, hipEvent_t e0,
+                            hipEvent_t e1) {
+  const long long vs = k.vs;
   switch (k.Ny) {
-#define X(L)                                                                                                      \
-  case L: {                                                                                                       \
-    using CC = KColCfg<R, L>;                                                                                     \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);                                                             \
-    hipExtLaunchKernelGGL((k_mv_cols_inv<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx,     \
-                          k.S4, k.kx0, k.ky, k.tw_y, ks, vs);                                                     \
-    break;                                                                                                        \
+#define X(L)                                                                                                     \
+  case L: {                                                                                                      \
+    using CC = KColCfg<R, L, CW>;                                                                                \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);                                                            \
+    hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx, \
+                          k.S4, k.kx0, k.ky, k.tw_y, ks, vs);                                                    \
+    break;                                                                                                       \
   }
     SAV_KRY_SIZES(X)
 #undef X
     default: throw HipError("unsupported Ny for the psi solve: " + std::to_string(k.Ny), -1);
   }
   SAV_LAUNCH_CHECK();
+}
+
+template <typename R, int CW>
+static int launch_cols_fwd(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, const cplx<R>* partner,
+                           const KScal* ks, int nsys, hipEvent_t e0, hipEvent_t e1) {
+  const long long vs = k.vs, ps = k.ps;
+  int nb3 = 0;
+#define KL(L, M)                                                                                                   \
+  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, y, partner, \
+                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials, vs, ps)
+  switch (k.Ny) {
+#define X(L)                                          \
+  case L: {                                           \
+    using CC = KColCfg<R, L, CW>;                     \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys); \
+    nb3 = (int)(grid.x * grid.y);                     \
+    switch (mode) {                                   \
+      case 1: KL(L, 1); break;                        \
+      case 2: KL(L, 2); break;                        \
+      case 4: KL(L, 4); break;                        \
+      case 5: KL(L, 5); break;                        \
+      case 6: KL(L, 6); break;                        \
+      default: KL(L, 0); break;                       \
+    }                                                 \
+    break;                                            \
+  }
+    SAV_KRY_SIZES(X)
+#undef X
+    default: break;
+  }
+#undef KL
+  SAV_LAUNCH_CHECK();
+  return nb3;
+}
+
+// One matvec on nsys systems (vectors k.vs apart, KScal slots slot .. slot+nsys-1).
+// mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
+template <typename R>
+static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
+                         const cplx<R>* partner = nullptr, int nsys = 1) {
+  const KScal* ks = k.ks + slot;
+  const long long vs = k.vs;
+  const double sn = (double)sizeof(R) * (double)k.nfull;
+  hipEvent_t e0, e1;
+  tev_pre(k, 1, e0, e1);
+  if (k.colw == 16)
+    launch_cols_inv<R, 16>(k, y, ks, nsys, e0, e1);
+  else
+    launch_cols_inv<R, 8>(k, y, ks, nsys, e0, e1);
   tev_post(k, 1, 6.0 * sn);
   tev_pre(k, 2, e0, e1);
   const long long nrh = (long long)k.N * k.Ny;
@@ -572,33 +616,59 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   SAV_LAUNCH_CHECK();
   tev_post(k, 2, 9.0 * sn);
   tev_pre(k, 3, e0, e1);
-#define KL(L, M)                                                                                              \
-  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, y, partner, \
-                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials, vs, ps)
-  switch (k.Ny) {
-#define X(L)                                          \
-  case L: {                                           \
-    using CC = KColCfg<R, L>;                         \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys); \
-    nb3 = (int)(grid.x * grid.y);                     \
-    switch (mode) {                                   \
-      case 1: KL(L, 1); break;                        \
-      case 2: KL(L, 2); break;                        \
-      case 4: KL(L, 4); break;                        \
-      case 5: KL(L, 5); break;                        \
-      case 6: KL(L, 6); break;                        \
-      default: KL(L, 0); break;                       \
-    }                                                 \
-    break;                                            \
-  }
-    SAV_KRY_SIZES(X)
-#undef X
-    default: break;
-  }
-#undef KL
-  SAV_LAUNCH_CHECK();
+  const int nb3 = k.colw == 16 ? launch_cols_fwd<R, 16>(k, y, out, mode, partner, ks, nsys, e0, e1)
+                               : launch_cols_fwd<R, 8>(k, y, out, mode, partner, ks, nsys, e0, e1);
   tev_post(k, 3, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn);
   return nb3;
+}
+
+// Autotuner of the matvec's column kernels: at the first solve of a handle, times
+// three matvecs (on the right-hand side in k.b, output into k.t, KScal slot 3) with 8
+// and with 16 columns per workgroup and keeps the faster.  Both widths compute the
+// same numbers (the same transform per column); only the launch shape differs.
+// Columns per workgroup that width CW resolves to for column length Ny (the LDS and
+// thread limits can cap 16 down to 8).
+template <typename R, int CW>
+static int effective_columns(int Ny) {
+  switch (Ny) {
+#define X(L) \
+  case L: return KColCfg<R, L, CW>::C;
+    SAV_KRY_SIZES(X)
+#undef X
+    default: return 0;
+  }
+}
+
+template <typename R>
+static void autotune_columns(KrylovCtx<R>& k) {
+  if (k.colw != 0) return;
+  if (effective_columns<R, 8>(k.Ny) == effective_columns<R, 16>(k.Ny)) {
+    k.colw = 8;  // one kernel shape either way: nothing to choose
+    return;
+  }
+  hipEvent_t a, b;
+  SAV_HIP(hipEventCreate(&a));
+  SAV_HIP(hipEventCreate(&b));
+  const int widths[2] = {8, 16};
+  const int saved_timed = k.timed;
+  k.timed = 0;
+  for (int w = 0; w < 2; ++w) {
+    k.colw = widths[w];
+    KScal z{};
+    SAV_HIP(hipMemcpyAsync(k.ks + 3, &z, sizeof(KScal), hipMemcpyHostToDevice, k.st));
+    matvec_launch(k, k.b, k.t, 0, 3);  // warm-up
+    SAV_HIP(hipEventRecord(a, k.st));
+    for (int r = 0; r < 3; ++r) matvec_launch(k, k.b, k.t, 0, 3);
+    SAV_HIP(hipEventRecord(b, k.st));
+    SAV_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SAV_HIP(hipEventElapsedTime(&ms, a, b));
+    k.tune_ms[w] = ms / 3.0;
+  }
+  k.timed = saved_timed;
+  k.colw = k.tune_ms[1] < k.tune_ms[0] ? 16 : 8;
+  SAV_HIP(hipEventDestroy(a));
+  SAV_HIP(hipEventDestroy(b));
 }
 
 template <typename R>
@@ -625,6 +695,7 @@ static void fin_launch(KrylovCtx<R>& k, int nb, int slot, int stage, int nsys) {
 
 template <typename R>
 int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
+  autotune_columns(k);
   const long long n = k.nfull;
   const int nbv = vec_blocks(n);
   const long long vs = k.vs, ps = k.ps;
@@ -758,6 +829,7 @@ __global__ __launch_bounds__(256) void k_fkry_upd2(const cplx<R>* __restrict__ s
 
 template <typename R>
 int fbicgstab_2d(KrylovCtx<R>& k, int slot, const std::function<void(const cplx<R>*, cplx<R>*)>& prec) {
+  autotune_columns(k);
   const long long n = k.nfull;
   const int nbv = vec_blocks(n);
   KScal* ks = k.ks + slot;

@@ -74,6 +74,9 @@ class ISolver {
   virtual int precond_collect(int on, int patches_per_snapshot, uint64_t seed) = 0;
   virtual long long precond_samples() = 0;
   virtual long long precond_fallbacks() = 0;
+  // psi matvec column-tile autotuner (2D)
+  virtual int set_column_tile(int cw) = 0;
+  virtual void column_tile(int* cw, double* ms8, double* ms16) = 0;
   virtual int precond_train(int iters, int batch, double lr, uint64_t seed, double* loss_first,
                             double* loss_last) = 0;
   virtual int net_forward(const float* x, int nimg, int Hh, int Ww, float* y) = 0;

@@ -850,6 +850,19 @@ class Solver final : public ISolver {
   long long precond_samples() override { return store_n; }
   long long precond_fallbacks() override { return learned_fallbacks; }
 
+  int set_column_tile(int cw) override {
+    if (nd != 2) return SAV_E_UNSUPPORTED;
+    if (cw != 0 && cw != 8 && cw != 16) return SAV_E_ARG;
+    kc.colw = cw;
+    return SAV_OK;
+  }
+
+  void column_tile(int* cw, double* ms8, double* ms16) override {
+    if (cw) *cw = kc.colw;
+    if (ms8) *ms8 = kc.tune_ms[0];
+    if (ms16) *ms16 = kc.tune_ms[1];
+  }
+
   int precond_train(int iters, int batch, double lr, uint64_t seed, double* l0, double* l1) override {
     ensure_net();
     if (store_n < 1 || iters < 1 || batch < 1) return SAV_E_ARG;

@@ -72,7 +72,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
-           "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
+           "sav_set_column_tile", "sav_autotune_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
            "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
            "sav_ensemble_member", "sav_ensemble_run")
@@ -121,6 +121,8 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_kernel_timing_get.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64), _dp, _dp]
     _fp = ct.c_void_p
     lib.sav_set_precond.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_set_column_tile.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_autotune_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
     lib.sav_precond_init.argtypes = [ct.c_void_p, ct.c_uint64]
     lib.sav_precond_load.argtypes = [ct.c_void_p, _fp, ct.c_int]
     lib.sav_precond_save.argtypes = [ct.c_void_p, _fp, ct.c_int]
@@ -406,6 +408,16 @@ class Operators:
         E, ar = ct.c_double(0), ct.c_double(0)
         self._check(self.lib.sav_modified_energy(self.h, ct.byref(E), ct.byref(ar)), "modified_energy")
         return E.value, ar.value
+
+    # -- psi matvec autotuner (no reference counterpart) -------------------------
+    def set_column_tile(self, columns: int):
+        """8 or 16 columns per workgroup in the psi matvec's column kernels; 0 retunes."""
+        self._check(self.lib.sav_set_column_tile(self.h, int(columns)), "set_column_tile")
+
+    def autotune_info(self):
+        cw, a, b = ct.c_int32(0), ct.c_double(0), ct.c_double(0)
+        self._check(self.lib.sav_autotune_info(self.h, ct.byref(cw), ct.byref(a), ct.byref(b)), "autotune_info")
+        return dict(columns=cw.value, ms_8=a.value, ms_16=b.value)
 
     # -- learned psi preconditioner (no reference counterpart) -----------------
     def set_precond(self, precond: int):
