@@ -149,8 +149,8 @@ int sav_modified_energy(sav_handle_t* h, double* energy, double* area_ratio);
 /* Autotuner of the 2D psi matvec (no reference counterpart): at the first psi solve of a
  * handle the column-FFT kernels are timed with 8 and with 16 columns per workgroup (three
  * matvecs each on the solve's right-hand side) and the faster is kept; both compute the
- * same numbers.  sav_set_column_tile(h, 8 | 16) fixes the width, 0 retunes at the next
- * solve.  sav_autotune_info returns the width in use (0 before the first solve) and the
+ * same numbers (the dot products are summed per column).  sav_set_column_tile(h, 8 | 16)
+ * fixes the width, 0 retunes at the next solve.  sav_autotune_info returns the width in use (0 before the first solve) and the
  * measured milliseconds per matvec of each width. */
 int sav_set_column_tile(sav_handle_t* h, int columns);
 int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16);

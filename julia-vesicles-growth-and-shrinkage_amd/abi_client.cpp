@@ -118,10 +118,13 @@ int main() {
     std::printf("set-up failed: %s\n", sav_last_error(h));
     return 1;
   }
+  // the time nodes of the ensemble run below, so both see the same dt = t[k] - t[k-1]
+  std::vector<double> t(steps + 1);
+  for (int k = 0; k <= steps; ++k) t[k] = k * 1e-4;
   std::vector<double> E(steps);
   for (int k = 1; k <= steps; ++k) {
     sav_step_stats_t st{};
-    const int rc = sav_step(h, 1e-4, k, &st);
+    const int rc = sav_step(h, t[k] - t[k - 1], k, &st);
     std::printf("step %d rc %d energy %.12e area_ratio %.3e psi_iters %d %d %d\n", k, rc, st.energy, st.area_ratio,
                 st.krylov_iters[0], st.krylov_iters[1], st.krylov_iters[2]);
     if (rc != SAV_OK || !st.krylov_solved[0] || !st.krylov_solved[1] || !st.krylov_solved[2]) return 2;
@@ -140,8 +143,7 @@ int main() {
   State s0(n), s1(n);
   if (set_initial(sav_ensemble_member(e, 0), s0) != SAV_OK || set_initial(sav_ensemble_member(e, 1), s1) != SAV_OK)
     return 5;
-  std::vector<double> t(steps + 1), EE(2 * steps);
-  for (int k = 0; k <= steps; ++k) t[k] = k * 1e-4;
+  std::vector<double> EE(2 * steps);
   std::vector<int32_t> kit(2), status(2);
   const int rc = sav_ensemble_run(e, t.data(), steps + 1, 1, 2, EE.data(), nullptr, kit.data(), status.data());
   sav_ensemble_destroy(e);

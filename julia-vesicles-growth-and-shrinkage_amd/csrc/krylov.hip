@@ -186,7 +186,6 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT)) void k_mv_cols_fwd(
   constexpr int C = KColCfg<R, L, CW>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
-  __shared__ double scratch[3 * 16];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int col = blockIdx.x * C + c;
   const bool act = col < Nx;
@@ -262,12 +261,32 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT)) void k_mv_cols_fwd(
     }
   }
   if constexpr ((MODE & 3) != 0) {
-    block_sum<3>(acc, scratch);
-    if (threadIdx.x == 0) {
-      const long long b = (long long)blockIdx.y * gridDim.x + blockIdx.x;
-      partials[b * 3 + 0] = acc[0];
-      partials[b * 3 + 1] = acc[1];
-      partials[b * 3 + 2] = acc[2];
+    // one partial per column, summed over the column's team in a fixed order: the dot
+    // products do not depend on how many columns a workgroup holds
+    constexpr int TT = F::TT;
+    static_assert(3 * C * TT * (int)sizeof(double) <= C * L * (int)sizeof(cplx<R>), "LDS reuse");
+    __syncthreads();  // the transforms are done with the LDS
+    double* red = reinterpret_cast<double*>(lds);
+    red[3 * threadIdx.x + 0] = acc[0];
+    red[3 * threadIdx.x + 1] = acc[1];
+    red[3 * threadIdx.x + 2] = acc[2];
+    __syncthreads();
+    if (threadIdx.x < C) {
+      const int cc = threadIdx.x;
+      const int colc = blockIdx.x * C + cc;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int tt = 0; tt < TT; ++tt) {
+        const int th = tt * C + cc;
+        s0 += red[3 * th + 0];
+        s1 += red[3 * th + 1];
+        s2 += red[3 * th + 2];
+      }
+      if (colc < Nx) {
+        const long long b = (long long)blockIdx.y * Nx + colc;
+        partials[b * 3 + 0] = s0;
+        partials[b * 3 + 1] = s1;
+        partials[b * 3 + 2] = s2;
+      }
     }
   }
 }
@@ -562,7 +581,7 @@ static int launch_cols_fwd(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int 
   case L: {                                           \
     using CC = KColCfg<R, L, CW>;                     \
     dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys); \
-    nb3 = (int)(grid.x * grid.y);                     \
+    nb3 = k.Nx * k.N;  /* one partial per column */    \
     switch (mode) {                                   \
       case 1: KL(L, 1); break;                        \
       case 2: KL(L, 2); break;                        \
@@ -622,10 +641,6 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   return nb3;
 }
 
-// Autotuner of the matvec's column kernels: at the first solve of a handle, times
-// three matvecs (on the right-hand side in k.b, output into k.t, KScal slot 3) with 8
-// and with 16 columns per workgroup and keeps the faster.  Both widths compute the
-// same numbers (the same transform per column); only the launch shape differs.
 // Columns per workgroup that width CW resolves to for column length Ny (the LDS and
 // thread limits can cap 16 down to 8).
 template <typename R, int CW>
@@ -639,6 +654,11 @@ static int effective_columns(int Ny) {
   }
 }
 
+// Autotuner of the matvec's column kernels: at the first solve of a handle, times
+// three matvecs (on the right-hand side in k.b, output into k.t, KScal slot 3) with 8
+// and with 16 columns per workgroup and keeps the faster.  Both widths compute the same
+// numbers: the same transform per column, and the Krylov dot products are summed per
+// column in a fixed order.
 template <typename R>
 static void autotune_columns(KrylovCtx<R>& k) {
   if (k.colw != 0) return;

@@ -229,7 +229,9 @@ class Solver final : public ISolver {
       kc.itmax = c.itmax;
       // three systems (the batched step-1 solves), nf apart
       kc.vs = nf;
-      kc.ps = 3 * 4096;  // >= 3 x (blocks of any Krylov launch per system)
+      // partials per system: 3 x (one per column of the column-forward kernel, or one per
+      // block of a vector kernel, at most 512)
+      kc.ps = 3 * std::max<long long>((long long)Nx * N, 512);
       kc.x = dalloc<C>(3 * nf);
       kc.r = dalloc<C>(3 * nf);
       kc.p = dalloc<C>(3 * nf);
