@@ -59,10 +59,12 @@ def _setup(jv, wl, precision):
     return ops
 
 
-def run_case(jv, wl, precision, steps, warmup, timed_kernel=0):
-    """Warmup steps untimed, then `steps` timed steps (inputs resident in HBM)."""
+def run_case(jv, wl, precision, steps, warmup, timed_kernel=0, graph=False):
+    """Warmup steps untimed, then `steps` timed steps (inputs resident in HBM); with
+    `graph`, each step is replayed as one captured HIP graph."""
     conf = wl.conf
     ops = _setup(jv, wl, precision)
+    ops.set_graph(graph)
     t = np.arange(warmup + steps + 1) * conf.dt
     if warmup:
         ops.run(t[: warmup + 1], 1)
@@ -80,6 +82,8 @@ def run_case(jv, wl, precision, steps, warmup, timed_kernel=0):
         n, ms, b = ops.kernel_timing_get()
         res["kernel"] = dict(name=ops.KERNELS[timed_kernel], launches=n, avg_ms=ms / max(n, 1), bytes=b)
     res["autotune"] = ops.autotune_info() if conf.ndim == 2 else None
+    if graph:
+        res["graph"] = ops.graph_info()
     st = ops.get_state(jv.SAV_PRESENT)
     ops.close()
     return res, st
@@ -126,7 +130,7 @@ def run_case_learned(jv, wl, precision, steps, warmup, collect_steps=4, train_it
                 loss_first=l0, loss_last=l1, energy_monotone=bool(np.all(np.diff(E) <= 0)))
 
 
-def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16)):
+def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False):
     """The 64-instance ensemble (16 each of 128^2, 256^2, 384^2, 512^2, 1-3 vesicles):
     aggregate steps/s for each host-thread count (1 = the sequential per-instance loop),
     after one warm-up step of every member."""
@@ -142,6 +146,7 @@ def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16)):
         st.Q = st.R1 = st.R2 = st.R3 = 1.0
         m.set_state(jv.SAV_PRESENT, st)
         m.init_sav()
+    ens.set_graph(graph)
     dt = wls[0].conf.dt
     n0 = 1
     ens.run(np.arange(2) * dt, n0, threads=16)

@@ -150,10 +150,23 @@ int sav_modified_energy(sav_handle_t* h, double* energy, double* area_ratio);
  * handle the column-FFT kernels are timed with 8 and with 16 columns per workgroup (three
  * matvecs each on the solve's right-hand side) and the faster is kept; both compute the
  * same numbers (the dot products are summed per column).  sav_set_column_tile(h, 8 | 16)
- * fixes the width, 0 retunes at the next solve.  sav_autotune_info returns the width in use (0 before the first solve) and the
- * measured milliseconds per matvec of each width. */
+ * fixes the width, 0 retunes at the next solve.  sav_autotune_info returns the width in
+ * use (0 before the first solve) and the measured milliseconds per matvec of each width. */
 int sav_set_column_tile(sav_handle_t* h, int columns);
 int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16);
+
+/* Graph replay of steps (no reference counterpart; the loop body of src/simulation.jl:73-110
+ * as one HIP graph).  With on = 1, sav_step and sav_run capture a whole step (steps 1-7,
+ * the back-transform of the new state, the monitor) once per buffer rotation and psi
+ * iteration budget, and replay it: one graph launch and one host synchronisation per
+ * step.  The graph queues one psi iteration more than the previous step needed; a
+ * system still iterating after that budget makes the step run again without the graph
+ * (present and old are untouched by the replay).  Results are bit-identical to on = 0
+ * (the default).  Steps run without the graph while a kernel timing hook is on, with the
+ * learned or stencil preconditioner, and before the first 2D psi solve.
+ * sav_graph_info: steps replayed, steps repeated without the graph, graphs held. */
+int sav_set_graph(sav_handle_t* h, int on);
+int sav_graph_info(sav_handle_t* h, long long* replays, long long* repeats, long long* graphs);
 
 /* Milliseconds of device time of the last sav_run / sav_step (HIP events on the handle's stream). */
 int sav_last_elapsed_ms(sav_handle_t* h, double* ms);

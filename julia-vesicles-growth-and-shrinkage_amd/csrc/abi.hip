@@ -210,6 +210,16 @@ int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* m
   SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16); if (columns) *columns = cw; return SAV_OK);
 }
 
+int sav_set_graph(sav_handle_t* h, int on) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->set_graph(on));
+}
+
+int sav_graph_info(sav_handle_t* h, long long* replays, long long* repeats, long long* graphs) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, h->s->graph_info(replays, repeats, graphs); return SAV_OK);
+}
+
 int sav_set_precond(sav_handle_t* h, int precond) {
   if (!h) return SAV_E_ARG;
   SAV_GUARD(h, return h->s->set_precond(precond));

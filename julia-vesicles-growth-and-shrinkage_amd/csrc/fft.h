@@ -1,4 +1,4 @@
-// Batched mixed-radix (2/3/4/8) Stockham FFTs for gfx950.
+// Batched mixed-radix (2/3/4/8, optionally 16/32) Stockham FFTs for gfx950.
 //
 // One transform of length L is carried by a "team" of TT = L/E threads, each
 // holding E elements in registers.  A radix-R pass makes every thread do E/R
@@ -28,25 +28,37 @@ struct FftPlanData {
 // ttmax: largest team (lanes per transform).  FP64 uses 128-lane teams (E = 8 for
 // L = 1024: low VGPR count, 4 waves/SIMD); FP32 uses 64-lane teams so that a
 // 512-thread column workgroup still spans 8 adjacent columns (64-byte segments).
-constexpr FftPlanData make_fft_plan(int L, int ttmax = 64) {
+// maxr: largest power-of-two radix of a pass (8, 16 or 32): fewer passes, hence fewer
+// LDS exchanges, for more registers per lane.  rev: the passes in reverse order (the
+// first pass of a reversed plan has the radix of the last pass of the plain one).
+constexpr FftPlanData make_fft_plan(int L, int ttmax = 64, int maxr = 8, bool rev = false) {
   FftPlanData p{};
   int n = L, np = 0, has3 = 0;
   while (n % 3 == 0) { p.rad[np++] = 3; n /= 3; has3 = 1; }
   int a = 0;
   while (n % 2 == 0) { n /= 2; ++a; }
-  int maxr = 1;
-  int n8 = a / 3, rem = a % 3;
-  for (int i = 0; i < n8; ++i) { p.rad[np++] = 8; maxr = 8; }
-  if (rem == 1) { p.rad[np++] = 2; if (maxr < 2) maxr = 2; }
-  if (rem == 2) { p.rad[np++] = 4; if (maxr < 4) maxr = 4; }
+  const int lg = maxr >= 32 ? 5 : maxr >= 16 ? 4 : 3;
+  int used = 1;
+  const int nbig = a / lg, rem = a % lg;
+  for (int i = 0; i < nbig; ++i) { p.rad[np++] = 1 << lg; used = 1 << lg; }
+  if (rem > 0) { p.rad[np++] = 1 << rem; if (used < (1 << rem)) used = 1 << rem; }
   p.np = np;
-  int lcm = has3 ? 3 * maxr : maxr;
+  int lcm = has3 ? 3 * used : used;
   if (lcm < 1) lcm = 1;
   int E = lcm;
   while (L / E > ttmax && (L % (2 * E)) == 0 && 2 * E <= 32) E *= 2;
   p.E = E;
+  if (rev)
+    for (int i = 0; i < np / 2; ++i) {
+      const int t = p.rad[i];
+      p.rad[i] = p.rad[np - 1 - i];
+      p.rad[np - 1 - i] = t;
+    }
   return p;
 }
+
+// Largest radix of the psi matvec's row plan: 16 for powers of two, else 8.
+constexpr int row_plan_maxr(int L) { return (L & (L - 1)) == 0 ? 16 : 8; }
 
 constexpr bool fft_len_supported(int L) {
   int n = L;
@@ -116,11 +128,72 @@ __device__ __forceinline__ void bfly8(cplx<R>* a) {
 }
 
 template <typename R, bool INV, int RAD>
+__device__ __forceinline__ void bfly(cplx<R>* a);
+
+// cos(2 pi m / 32), m = 0..8 (correctly rounded); the other angles by symmetry.
+constexpr double kCos32[9] = {1.0,
+                              0.9807852804032304491261822361342390369739,
+                              0.9238795325112867561281831893967882868224,
+                              0.8314696123025452370787883776179057567386,
+                              0.7071067811865475244008443621048490392848,
+                              0.5555702330196022247428308139485328743749,
+                              0.3826834323650897717284599840303988667613,
+                              0.1950903220161282678482848684770222409277,
+                              0.0};
+
+// a * W_RAD^m, W = e^{-2 pi i / RAD} (e^{+} for INV); m is a constant after unrolling.
+template <typename R, bool INV, int RAD>
+__device__ __forceinline__ cplx<R> rot_const(cplx<R> a, int m) {
+  m %= RAD;
+  if (m == 0) return a;
+  if (2 * m == RAD) return cplx<R>{-a.x, -a.y};
+  if (4 * m == RAD) return INV ? mul_i(a) : mul_mi(a);
+  if (4 * m == 3 * RAD) return INV ? mul_mi(a) : mul_i(a);
+  const int q = m * (32 / RAD);  // angle 2 pi q / 32
+  // cos and sin of 2 pi q / 32 from the first octant
+  const int o = q & 7, quad = q >> 3;
+  const double c0 = kCos32[o], s0 = kCos32[8 - o];
+  const double cq = quad == 0 ? c0 : quad == 1 ? -s0 : quad == 2 ? -c0 : s0;
+  const double sq = quad == 0 ? s0 : quad == 1 ? c0 : quad == 2 ? -s0 : -c0;
+  const R c = (R)cq, sn = (R)(INV ? sq : -sq);
+  return cplx<R>{a.x * c - a.y * sn, a.x * sn + a.y * c};
+}
+
+// Radix R1*R2 butterfly from R1- and R2-point ones (Cooley-Tukey in registers):
+// X[k1 + R1 k2] = sum_n2 W_R2^{n2 k2} W_RAD^{n2 k1} sum_n1 a[R2 n1 + n2] W_R1^{n1 k1}.
+template <typename R, bool INV, int R1, int R2>
+__device__ __forceinline__ void bfly_comp(cplx<R>* a) {
+  constexpr int RAD = R1 * R2;
+  cplx<R> y[RAD];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    cplx<R> t[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) t[n1] = a[R2 * n1 + n2];
+    bfly<R, INV, R1>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) y[n2 * R1 + k1] = rot_const<R, INV, RAD>(t[k1], n2 * k1);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cplx<R> u[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) u[n2] = y[n2 * R1 + k1];
+    bfly<R, INV, R2>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) a[k1 + R1 * k2] = u[k2];
+  }
+}
+
+template <typename R, bool INV, int RAD>
 __device__ __forceinline__ void bfly(cplx<R>* a) {
   if constexpr (RAD == 2) bfly2<R, INV>(a);
   else if constexpr (RAD == 3) bfly3<R, INV>(a);
   else if constexpr (RAD == 4) bfly4<R, INV>(a[0], a[1], a[2], a[3]);
   else if constexpr (RAD == 8) bfly8<R, INV>(a);
+  else if constexpr (RAD == 16) bfly_comp<R, INV, 4, 4>(a);
+  else if constexpr (RAD == 32) bfly_comp<R, INV, 8, 4>(a);
+  else static_assert(RAD == 2, "unsupported radix");
 }
 
 // ---------------------------------------------------------------------------
@@ -143,33 +216,38 @@ constexpr int twiddle_table_len(int L) {
 
 // TTMAX: largest team; PADS: LDS padding, one slot every 2^PADS positions (0 = none),
 // which spreads the stride-R Stockham writes of the early passes over the banks.
-// TWM: 0 = plain table exp(-2 pi i n / L), 1 = pass-ordered table (twiddle_table_len).
-// WSYNC: the team is one wavefront (TT == 64, one team per wave): LDS exchanges between
-// passes need only a wavefront-level barrier instead of a workgroup barrier.
+// TWM: 0 = plain table exp(-2 pi i n / L), 1 = pass-ordered table (twiddle_table_len),
+// 2 = k-major pass table of the plan (kmajor_twiddles).
+// WSYNC: the workgroup is one wavefront holding whole teams (TT divides 64): LDS
+// exchanges between passes need only a wavefront-level barrier.
+// MAXR, REV: the plan's largest radix and pass order (make_fft_plan); the pass-ordered
+// twiddle table (TWM = 1) exists for the default plan only.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Default twiddle layout per precision: FP64 reads the compact plain table (a smaller
-// cache footprint wins), FP32 the pass-ordered one (fewer scattered loads per butterfly).
+// Default twiddle layout: the k-major pass table, in which the lanes of a team read
+// consecutive entries for each twiddle of a butterfly (one or two cache lines per load
+// instruction; the plain and pass-ordered tables scatter a wavefront's loads over many).
 template <typename R>
 constexpr int default_twm() {
-  return sizeof(R) == 8 ? 0 : 1;
+  return 2;
 }
 
 template <typename R, int L, bool INV, int TTMAX = (sizeof(R) == 8 ? 128 : 64), int PADS = 0,
-          int TWM = default_twm<R>(), int WSYNC = 0>
+          int TWM = default_twm<R>(), int WSYNC = 0, int MAXR = 8, bool REV = false>
 struct TeamFFT {
-  static constexpr FftPlanData P = make_fft_plan(L, TTMAX);
+  static constexpr FftPlanData P = make_fft_plan(L, TTMAX, MAXR, REV);
+  static_assert(TWM != 1 || (MAXR == 8 && !REV), "pass-ordered twiddles exist for the default plan only");
   static constexpr int E = P.E;
   static constexpr int TT = L / E;
   static constexpr int R0 = P.rad[0];
   static constexpr int RL = P.rad[P.np - 1];
   static constexpr int LDS_LEN = PADS ? L + (L >> PADS) + 1 : L;   // LDS slots per transform
   __device__ static __forceinline__ int lidx(int p) { return PADS ? p + (p >> PADS) : p; }
-  static_assert(!WSYNC || L / P.E == 64, "wavefront-synchronised teams must be exactly one wavefront");
+  static_assert(!WSYNC || 64 % (L / P.E) == 0, "wavefront-synchronised teams must lie within one wavefront");
   __device__ static __forceinline__ void sync() {
     if constexpr (WSYNC) wave_lds_sync();
     else __syncthreads();
@@ -199,10 +277,10 @@ struct TeamFFT {
         const int j = t + q * TT;
         if constexpr (LS > 1) {
           const int jm = j % LS;
-          const cplx<R>* twr = tw + TWO + jm * 8;
+          const cplx<R>* twr = tw + TWO + (TWM == 2 ? jm : jm * 8);
 #pragma unroll
           for (int k = 1; k < RAD; ++k) {
-            cplx<R> w = TWM ? twr[k - 1] : tw[jm * k * TWS];
+            cplx<R> w = TWM == 1 ? twr[k - 1] : TWM == 2 ? twr[(k - 1) * LS] : tw[jm * k * TWS];
             if (INV) w.y = -w.y;
             v[q * RAD + k] = v[q * RAD + k] * w;
           }
@@ -219,7 +297,7 @@ struct TeamFFT {
           for (int m = 0; m < RAD; ++m) lds[lidx(base + m * LS) * ls] = v[q * RAD + m];
         }
         sync();
-        passes<PI + 1, LS * RAD, (PI > 0 ? TWO + LS * 8 : TWO)>(v, lds, ls, t, tw);
+        passes<PI + 1, LS * RAD, (PI > 0 ? TWO + LS * (TWM == 2 ? RAD - 1 : 8) : TWO)>(v, lds, ls, t, tw);
       }
     }
   }
@@ -255,9 +333,33 @@ inline void build_pass_twiddles(int L, cplx<R>* out) {
   if (off == 0) out[0] = cplx<R>{R(1), R(0)};
 }
 
+// Host: k-major pass table (TWM = 2) of the plan make_fft_plan(L, ttmax, maxr, rev): for
+// pass p >= 1 (stride LS_p, radix R_p) W_{LS R}^{jm k} sits at off_p + (k - 1) LS_p + jm,
+// so the lanes of a team read consecutive entries for each k.
+template <typename R>
+inline std::vector<cplx<R>> kmajor_twiddles(int L, int ttmax, int maxr, bool rev) {
+  const FftPlanData P = make_fft_plan(L, ttmax, maxr, rev);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  std::vector<cplx<R>> out;
+  int ls = 1;
+  for (int p = 0; p < P.np; ++p) {
+    const int rad = P.rad[p];
+    if (p > 0)
+      for (int k = 1; k < rad; ++k)
+        for (int jm = 0; jm < ls; ++jm) {
+          const long double a = -two_pi * (long double)(jm * k) / (long double)(ls * rad);
+          out.push_back(cplx<R>{(R)cosl(a), (R)sinl(a)});
+        }
+    ls *= rad;
+  }
+  if (out.empty()) out.push_back(cplx<R>{R(1), R(0)});
+  return out;
+}
+
 // Host: the table the default TeamFFT<R, L, ...> expects for length L.
 template <typename R>
 inline std::vector<cplx<R>> default_twiddles(int L) {
+  if (default_twm<R>() == 2) return kmajor_twiddles<R>(L, 64, 8, false);
   if (default_twm<R>() == 0) {
     std::vector<cplx<R>> w(L);
     const long double two_pi = 6.283185307179586476925286766559005768L;

@@ -15,6 +15,8 @@ struct KrylovCtx {
   const double* ky;
   const cplx<R>* tw_x;        // length-Nx table
   const cplx<R>* tw_y;        // length-Ny table
+  const cplx<R>* tw_ri;       // row plan of the matvec (row_plan_maxr), inverse: k-major table
+  const cplx<R>* tw_rf;       // the same plan reversed (forward)
   const R* M;                 // M_psi(phi*), physical (N, Ny, Nx)
   const DScal* ds;            // a, dt, Mmax
   double S4, atol, rtol;
@@ -57,6 +59,11 @@ int bicgstab_2d(KrylovCtx<R>& k, int slot);
 // iteration count; ctx.hks[slot + z] holds each system's final scalars.
 template <typename R>
 int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys);
+// The same solve queued with exactly `iters` iterations (capped at itmax) and no host
+// synchronisation, for graph capture: a system that converges earlier skips the
+// remaining kernels.  ctx.colw must be chosen already.  The caller reads ctx.ks.
+template <typename R>
+void bicgstab_queue(KrylovCtx<R>& k, int slot, int nsys, int iters);
 
 // A*y then the diagonal preconditioner: out = P^{-1} (a/dt y - sum_d D1f_d F(M F^{-1}(D1f_d S4 y)))
 // (one matvec + preconditioner, used by tests through the solver).

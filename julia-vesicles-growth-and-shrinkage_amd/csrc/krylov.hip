@@ -34,13 +34,19 @@ using KColCfg = ColTile<R, L, CW>;
 // wavefront only (no workgroup barrier; measured faster in FP64 only).
 template <typename R, int L>
 struct KRowCfg {
-  static constexpr int PADS = sizeof(R) == 8 ? 4 : 5;
-  static constexpr int WS = (sizeof(R) == 8 && L / make_fft_plan(L, 64).E == 64) ? 1 : 0;
-  using FI = TeamFFT<R, L, true, 64, PADS, default_twm<R>(), WS>;
-  using FF = TeamFFT<R, L, false, 64, PADS, default_twm<R>(), WS>;
+  // Radix-16 passes for power-of-two rows (fewer LDS exchanges), k-major twiddle tables.
+  // The forward transform runs the inverse's plan reversed, so its first pass reads
+  // exactly the elements the inverse's last pass left in each lane: the M_psi product is
+  // taken in registers, with no LDS exchange between the two transforms.  One wavefront
+  // per workgroup (64 / TT rows), synchronised at wavefront level.
+  static constexpr int MAXR = row_plan_maxr(L);
+  static constexpr int PADS = 4;
+  using FI = TeamFFT<R, L, true, 64, PADS, 2, 1, MAXR, false>;
+  using FF = TeamFFT<R, L, false, 64, PADS, 2, 1, MAXR, true>;
+  static_assert(FF::R0 == FI::RL && FF::TT == FI::TT && FF::E == FI::E, "reversed plan");
   static constexpr int TT = FI::TT;
-  static constexpr int ROWS = TT >= 256 ? 1 : 256 / TT;
-  static constexpr int NT = ROWS * TT;
+  static constexpr int ROWS = 64 / TT;
+  static constexpr int NT = 64;
   static constexpr int RS = FI::LDS_LEN | 1;   // odd row stride
 };
 
@@ -109,7 +115,8 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), 4) void k_mv_cols_inv(cons
 template <typename R, int L>
 __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __restrict__ Bx, cplx<R>* __restrict__ By,
                                                                 const R* __restrict__ M, long long nrows_half,
-                                                                double scale, const cplx<R>* __restrict__ tw,
+                                                                double scale, const cplx<R>* __restrict__ twi,
+                                                                const cplx<R>* __restrict__ twf,
                                                                 const KScal* __restrict__ ks, long long vs) {
   const int z = blockIdx.y;
   ks += z;
@@ -137,24 +144,20 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
     v[e] = act ? data[FI::in_pos(t, e)] : cplx<R>{R(0), R(0)};
     if constexpr (MPRE) mv[e] = act ? m[FI::out_pos(t, e)] : R(0);
   }
-  FI::run(v, my, 1, t, tw);
+  FI::run(v, my, 1, t, twi);
   const R sc = (R)scale;
-  FI::sync();
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int pos = FI::out_pos(t, e);
     R mval;
     if constexpr (MPRE)
       mval = mv[e];
     else
-      mval = act ? m[pos] : R(0);
+      mval = act ? m[FI::out_pos(t, e)] : R(0);
     const cplx<R> z = cplx<R>{sc * v[e].x, sc * v[e].y};
-    my[FI::lidx(pos)] = cplx<R>{mval * z.x, mval * z.y};
+    v[e] = cplx<R>{mval * z.x, mval * z.y};
   }
-  FI::sync();
-#pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = my[FF::lidx(FF::in_pos(t, e))];
-  FF::run(v, my, 1, t, tw);
+  // (the forward transform's first LDS write is preceded by its own barrier)
+  FF::run(v, my, 1, t, twf);
   if (act) {
 #pragma unroll
     for (int e = 0; e < E; ++e) data[FF::out_pos(t, e)] = v[e];
@@ -625,7 +628,7 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
     using CC = KRowCfg<R, L>;                                                                               \
     dim3 grid((unsigned)((2 * nrh + CC::ROWS - 1) / CC::ROWS), nsys);                                       \
     hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, k.M, nrh,  \
-                          scale, k.tw_x, ks, vs);                                                           \
+                          scale, k.tw_ri, k.tw_rf, ks, vs);                                                           \
     break;                                                                                                  \
   }
     SAV_KRY_SIZES(X)
@@ -713,18 +716,57 @@ static void fin_launch(KrylovCtx<R>& k, int nb, int slot, int stage, int nsys) {
                      k.itmax, k.ps);
 }
 
+// x = 0, r = p = P^{-1} b, rho, the stopping threshold: KScal slots slot .. slot + nsys - 1
+// start "running".
 template <typename R>
-int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
-  autotune_columns(k);
+static void kry_start(KrylovCtx<R>& k, int slot, int nsys) {
+  const long long n = k.nfull;
+  const int nbv = vec_blocks(n);
+  hipLaunchKernelGGL((k_kry_init<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4,
+                     k.kx0, k.ky, k.ds, k.partials, k.vs, k.ps);
+  SAV_LAUNCH_CHECK();
+  fin_launch(k, nbv, slot, 0, nsys);
+  SAV_LAUNCH_CHECK();
+}
+
+// One BiCGSTAB iteration of every system still running (a system whose `done` flag is set
+// skips every kernel), queued on k.st with no host synchronisation.
+template <typename R>
+static void kry_iteration(KrylovCtx<R>& k, int slot, int nsys) {
   const long long n = k.nfull;
   const int nbv = vec_blocks(n);
   const long long vs = k.vs, ps = k.ps;
   KScal* ks = k.ks + slot;
-  hipLaunchKernelGGL((k_kry_init<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.r, k.p, k.x, n, k.Nx, k.Ny, k.S4,
-                     k.kx0, k.ky, k.ds, k.partials, vs, ps);
+  // v = P^{-1} A p ; alpha = rho / <c, v>
+  int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b, nsys);
+  fin_launch(k, nb3, slot, 1, nsys);
+  // s = r - alpha v ; x += alpha p
+  hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks, vs);
+  // t = P^{-1} A s ; omega = <t, s> / <t, t>
+  nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s, nsys);
+  fin_launch(k, nb3, slot, 2, nsys);
+  // x += omega s ; r = s - omega t ; <c, r>, ||r||
+  hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks,
+                     k.partials, vs, ps);
+  fin_launch(k, nbv, slot, 3, nsys);
+  // p = r + beta (p - omega v)
+  hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks, vs);
   SAV_LAUNCH_CHECK();
-  fin_launch(k, nbv, slot, 0, nsys);
-  SAV_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bicgstab_queue(KrylovCtx<R>& k, int slot, int nsys, int iters) {
+  if (k.colw == 0) throw HipError("bicgstab_queue: the column tile of the matvec is not chosen yet", -1);
+  kry_start(k, slot, nsys);
+  for (int it = 0; it < iters && it < k.itmax; ++it) kry_iteration(k, slot, nsys);
+}
+
+template <typename R>
+int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
+  autotune_columns(k);
+  const long long n = k.nfull;
+  KScal* ks = k.ks + slot;
+  kry_start(k, slot, nsys);
   int queued = 0;
   const bool logging = k.snap != nullptr && nsys == 1;
   int guess = 0;
@@ -735,21 +777,7 @@ int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
     const int nb = batch < k.itmax - queued ? batch : k.itmax - queued;
     for (int it = 0; it < nb; ++it) {
       k.cur_iter = queued + it + 1;
-      // v = P^{-1} A p ; alpha = rho / <c, v>
-      int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b, nsys);
-      fin_launch(k, nb3, slot, 1, nsys);
-      // s = r - alpha v ; x += alpha p
-      hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks, vs);
-      // t = P^{-1} A s ; omega = <t, s> / <t, t>
-      nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s, nsys);
-      fin_launch(k, nb3, slot, 2, nsys);
-      // x += omega s ; r = s - omega t ; <c, r>, ||r||
-      hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks,
-                         k.partials, vs, ps);
-      fin_launch(k, nbv, slot, 3, nsys);
-      // p = r + beta (p - omega v)
-      hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks, vs);
-      SAV_LAUNCH_CHECK();
+      kry_iteration(k, slot, nsys);
     }
     queued += nb;
     SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, nsys * sizeof(KScal), hipMemcpyDeviceToHost, k.st));
@@ -888,6 +916,8 @@ template int bicgstab_2d<double>(KrylovCtx<double>&, int);
 template int bicgstab_2d<float>(KrylovCtx<float>&, int);
 template int bicgstab_batch<double>(KrylovCtx<double>&, int, int);
 template int bicgstab_batch<float>(KrylovCtx<float>&, int, int);
+template void bicgstab_queue<double>(KrylovCtx<double>&, int, int, int);
+template void bicgstab_queue<float>(KrylovCtx<float>&, int, int, int);
 template void matvec_prec_2d<double>(KrylovCtx<double>&, const cplx<double>*, cplx<double>*, int);
 template void matvec_prec_2d<float>(KrylovCtx<float>&, const cplx<float>*, cplx<float>*, int);
 

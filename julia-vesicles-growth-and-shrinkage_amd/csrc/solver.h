@@ -77,6 +77,8 @@ class ISolver {
   // psi matvec column-tile autotuner (2D)
   virtual int set_column_tile(int cw) = 0;
   virtual void column_tile(int* cw, double* ms8, double* ms16) = 0;
+  virtual int set_graph(int on) = 0;
+  virtual void graph_info(long long* replays, long long* repeats, long long* graphs) = 0;
   virtual int precond_train(int iters, int batch, double lr, uint64_t seed, double* loss_first,
                             double* loss_last) = 0;
   virtual int net_forward(const float* x, int nimg, int Hh, int Ww, float* y) = 0;

@@ -113,6 +113,7 @@ class Solver final : public ISolver {
   // Krylov
   KrylovCtx<R> kc{};
   KScal* hks = nullptr;
+  DStateScal* hst = nullptr;  // pinned: scalars of the state a step produced
   C* w_bphys_c = nullptr;
   float last_ms = 0.f;
 
@@ -210,6 +211,7 @@ class Solver final : public ISolver {
     tmp = dalloc<C>((16LL * N + 16) * H);
     // Krylov workspace (2D)
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks), 4 * sizeof(KScal), hipHostMallocDefault));
+    SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hst), sizeof(DStateScal), hipHostMallocDefault));
     std::memset(hks, 0, 4 * sizeof(KScal));
     if (nd == 2) {
       const long long nf = (long long)N * Nx * Ny;
@@ -220,6 +222,8 @@ class Solver final : public ISolver {
       kc.kx0 = d_kx0;
       kc.ky = d_ky;
       kc.tw_x = g.tw_x;
+      kc.tw_ri = upload(kmajor_twiddles<R>(Nx, 64, row_plan_maxr(Nx), false));
+      kc.tw_rf = upload(kmajor_twiddles<R>(Nx, 64, row_plan_maxr(Nx), true));
       kc.tw_y = g.tw_y;
       kc.M = w_M;
       kc.ds = ds;
@@ -256,6 +260,8 @@ class Solver final : public ISolver {
     for (void* p : allocs) hipFree(p);
     if (hs) hipHostFree(hs);
     if (hks) hipHostFree(hks);
+    if (hst) hipHostFree(hst);
+    for (auto& gr : graphs) hipGraphExecDestroy(gr.exec);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
     if (s) hipStreamDestroy(s);
@@ -538,7 +544,10 @@ class Solver final : public ISolver {
       b[z * vs + j] = CD{rphys[i], T(0)};
     });
     fft_full_2d<R>(g, b, 3 * N, false, s);  // vs == nf: the three systems are contiguous
-    bicgstab_batch<R>(kc, 0, 3);
+    if (capture_iters > 0)
+      bicgstab_queue<R>(kc, 0, 3, capture_iters);
+    else
+      bicgstab_batch<R>(kc, 0, 3);
     const C* x = kc.x;
     const int Nxl = Nx, Nxhl = Nxh;
     const long long nh = (long long)N * H;
@@ -855,6 +864,7 @@ class Solver final : public ISolver {
   int set_column_tile(int cw) override {
     if (nd != 2) return SAV_E_UNSUPPORTED;
     if (cw != 0 && cw != 8 && cw != 16) return SAV_E_ARG;
+    if (cw != kc.colw) drop_graphs();  // captured matvecs use the previous width
     kc.colw = cw;
     return SAV_OK;
   }
@@ -1111,7 +1121,8 @@ class Solver final : public ISolver {
       if (!use_learned && !collecting) {
         // the three solves share the operator: one batched BiCGSTAB (Solvers.jl:73 x 3)
         psi_solve3(w_rhs, cf(PSI11), cf(PSI12), cf(PSI21));
-        for (int z = 0; z < 3; ++z) kit[z] = hks[z].iter;
+        if (capture_iters == 0)
+          for (int z = 0; z < 3; ++z) kit[z] = hks[z].iter;
       } else {
         kit[0] = psi_solve(w_rhs, cf(PSI11), 0);
         kit[1] = psi_solve(w_rhs + NH, cf(PSI12), 1);
@@ -1151,6 +1162,7 @@ class Solver final : public ISolver {
   }
 
   int kit[3] = {0, 0, 0};
+  int capture_iters = 0;  // > 0 while a step is captured: psi iterations queued without host checks
 
   void max_reduce(const R* x, long long n, double* out) {
     reduce_max_impl(x, n, out);
@@ -1754,31 +1766,22 @@ class Solver final : public ISolver {
     return 0;
   }
 
-  int step_impl(double dt, int n, sav_step_stats_t* st, bool monitor_on) {
-    const double a = n == 1 ? 1.0 : 1.5, b = n == 1 ? -1.0 : -2.0, cc = n == 1 ? 0.0 : 0.5;
-    present_dt = dt;
-    set_bdf(a, b, cc, dt);
-    step1();
-    int rc = 0;
-    step2();
-    step3();
-    step4();
-    step5();
-    step67();
-    // fatal W + C < 0 or non-finite: the step is not committed
-    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
-    SAV_HIP(hipStreamSynchronize(s));
-    if (hs->flags & 7) return SAV_E_WC_NEGATIVE;
-    if (hs->flags & 16) return SAV_E_NONFINITE;
-    // rotate: old <- present, present <- new
+  void rotate() {  // old <- present, present <- new (update_state!, Types.jl:243-265)
     const int oldp = ip, oldo = io;
     ip = inew;
     io = oldp;
     inew = oldo;
+  }
+  // After rotate(): the back-transform of the new state (simulation.jl:99-104) and the
+  // energy/area monitor (simulation.jl:146-148).
+  void step_tail(bool monitor_on) {
     irfft({job(shalf[ip], sphys[ip], 4 * N + nd + 1)});
     if (monitor_on) monitor();
-    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
-    SAV_HIP(hipStreamSynchronize(s));
+  }
+
+  // Step statistics and status from hs, hks, hst (copied from the device by the caller).
+  int finish_step(sav_step_stats_t* st) {
+    int rc = 0;
     bool notconv = false;
     if (st) {
       std::memset(st, 0, sizeof(*st));
@@ -1790,11 +1793,9 @@ class Solver final : public ISolver {
       st->energy = hs->energy;
       st->area_ratio = hs->area_ratio;
       st->Q = hs->Q;
-      DStateScal sc;
-      SAV_HIP(hipMemcpy(&sc, dstate + ip, sizeof(sc), hipMemcpyDeviceToHost));
-      st->R1 = sc.R1;
-      st->R2 = sc.R2;
-      st->R3 = sc.R3;
+      st->R1 = hst->R1;
+      st->R2 = hst->R2;
+      st->R3 = hst->R3;
     }
     if (nd == 2)
       for (int k = 0; k < 3; ++k)
@@ -1803,6 +1804,158 @@ class Solver final : public ISolver {
     else if (hs->flags & 8) rc = SAV_W_THETA2_NONNEG;
     if (st) st->status = rc;
     return rc;
+  }
+
+  // ==========================================================================
+  // Graph replay of whole steps (no reference counterpart).  A step (steps 1-7, the
+  // back-transform of the new state and the monitor) is captured once per buffer
+  // rotation, psi iteration budget K and monitor flag into a HIP graph and replayed: one
+  // graph launch and one host synchronisation per step instead of ~160 kernel launches
+  // and a host check per batch of Krylov iterations.  The psi solve's iteration count is
+  // data dependent: the graph queues K = (largest count of the previous step) + 1
+  // batched iterations and every Krylov kernel of a converged system returns at once.  If
+  // a system still iterates after K < itmax iterations, the step is repeated eagerly: the
+  // graph wrote only the next-state buffer and work arrays, present and old are intact.
+  // Results are bit-identical to the eager step.  set_bdf runs outside the graph (its
+  // coefficients and dt are kernel arguments); every kernel in the graph reads them from
+  // the device scalars.
+  // ==========================================================================
+  struct StepGraph {
+    int ip, io, K, mon;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
+  bool graph_on = false;
+  long long graph_replays = 0, graph_repeats = 0;
+
+  bool graph_usable() const {
+    return graph_on && kc.timed == 0 && (nd == 3 || (!use_learned && !collecting && kc.colw != 0));
+  }
+  void drop_graphs() {
+    if (!graphs.empty()) SAV_HIP(hipStreamSynchronize(s));
+    for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
+    graphs.clear();
+  }
+
+  hipGraphExec_t capture_step(int K, bool monitor_on) {
+    const int sip = ip, sio = io, snew = inew;
+    hipGraph_t gph = nullptr;
+    SAV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      capture_iters = K > 0 ? K : 1;
+      step1();
+      step2();
+      step3();
+      step4();
+      step5();
+      step67();
+      rotate();
+      step_tail(monitor_on);
+    } catch (...) {
+      capture_iters = 0;
+      ip = sip;
+      io = sio;
+      inew = snew;
+      (void)hipStreamEndCapture(s, &gph);
+      if (gph) (void)hipGraphDestroy(gph);
+      throw;
+    }
+    capture_iters = 0;
+    ip = sip;
+    io = sio;
+    inew = snew;
+    SAV_HIP(hipStreamEndCapture(s, &gph));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gph);
+    SAV_HIP(e);
+    return ex;
+  }
+
+  // Replays the step's graph (set_bdf already queued).  Returns false when the step
+  // must be repeated eagerly.
+  bool step_graph(bool monitor_on, sav_step_stats_t* st, int& rc) {
+    int K = 0;
+    if (nd == 2) {
+      for (int z = 0; z < 3; ++z) K = std::max(K, kc.guess[z]);
+      K = std::min(K + 1, kc.itmax);
+    }
+    hipGraphExec_t ex = nullptr;
+    for (auto& gr : graphs)
+      if (gr.ip == ip && gr.io == io && gr.K == K && gr.mon == (int)monitor_on) ex = gr.exec;
+    if (!ex) {
+      ex = capture_step(K, monitor_on);
+      graphs.push_back(StepGraph{ip, io, K, (int)monitor_on, ex});
+    }
+    SAV_HIP(hipGraphLaunch(ex, s));
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    if (nd == 2) SAV_HIP(hipMemcpyAsync(hks, kc.ks, 3 * sizeof(KScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipMemcpyAsync(hst, dstate + inew, sizeof(DStateScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    ++graph_replays;
+    // fatal W + C < 0 or non-finite: the step is not committed
+    if (hs->flags & 7) {
+      rc = SAV_E_WC_NEGATIVE;
+      return true;
+    }
+    if (hs->flags & 16) {
+      rc = SAV_E_NONFINITE;
+      return true;
+    }
+    if (nd == 2 && K < kc.itmax)
+      for (int z = 0; z < 3; ++z)
+        if (!hks[z].done) {
+          ++graph_repeats;
+          return false;
+        }
+    if (nd == 2)
+      for (int z = 0; z < 3; ++z) {
+        kit[z] = hks[z].iter;
+        kc.guess[z] = kit[z] > 0 ? kit[z] : 1;
+      }
+    rotate();
+    rc = finish_step(st);
+    return true;
+  }
+
+  int set_graph(int on) override {
+    if (on != 0 && on != 1) return SAV_E_ARG;
+    graph_on = on == 1;
+    if (!graph_on) drop_graphs();
+    return 0;
+  }
+  void graph_info(long long* replays, long long* repeats, long long* ngraphs) override {
+    if (replays) *replays = graph_replays;
+    if (repeats) *repeats = graph_repeats;
+    if (ngraphs) *ngraphs = (long long)graphs.size();
+  }
+
+  int step_impl(double dt, int n, sav_step_stats_t* st, bool monitor_on) {
+    const double a = n == 1 ? 1.0 : 1.5, b = n == 1 ? -1.0 : -2.0, cc = n == 1 ? 0.0 : 0.5;
+    present_dt = dt;
+    set_bdf(a, b, cc, dt);
+    if (graph_usable()) {
+      int rc = 0;
+      if (step_graph(monitor_on, st, rc)) return rc;
+      set_bdf(a, b, cc, dt);  // psi iteration budget exceeded: repeat the step eagerly
+    }
+    step1();
+    step2();
+    step3();
+    step4();
+    step5();
+    step67();
+    // fatal W + C < 0 or non-finite: the step is not committed
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    if (hs->flags & 7) return SAV_E_WC_NEGATIVE;
+    if (hs->flags & 16) return SAV_E_NONFINITE;
+    rotate();
+    step_tail(monitor_on);
+    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipMemcpyAsync(hst, dstate + ip, sizeof(DStateScal), hipMemcpyDeviceToHost, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    return finish_step(st);
   }
 
   int step(double dt, int n, sav_step_stats_t* st) override {

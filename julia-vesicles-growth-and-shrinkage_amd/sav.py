@@ -72,7 +72,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
-           "sav_set_column_tile", "sav_autotune_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
+           "sav_set_column_tile", "sav_autotune_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
            "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
            "sav_ensemble_member", "sav_ensemble_run")
@@ -123,6 +123,9 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_set_precond.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_set_column_tile.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_autotune_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
+    lib.sav_set_graph.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_graph_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
+                                   ct.POINTER(ct.c_longlong)]
     lib.sav_precond_init.argtypes = [ct.c_void_p, ct.c_uint64]
     lib.sav_precond_load.argtypes = [ct.c_void_p, _fp, ct.c_int]
     lib.sav_precond_save.argtypes = [ct.c_void_p, _fp, ct.c_int]
@@ -419,6 +422,16 @@ class Operators:
         self._check(self.lib.sav_autotune_info(self.h, ct.byref(cw), ct.byref(a), ct.byref(b)), "autotune_info")
         return dict(columns=cw.value, ms_8=a.value, ms_16=b.value)
 
+    # -- graph replay of steps (no reference counterpart) -------------------------
+    def set_graph(self, on: bool):
+        """Replay each step as one captured HIP graph (bit-identical results)."""
+        self._check(self.lib.sav_set_graph(self.h, int(bool(on))), "set_graph")
+
+    def graph_info(self):
+        a, b, c = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
+        self._check(self.lib.sav_graph_info(self.h, ct.byref(a), ct.byref(b), ct.byref(c)), "graph_info")
+        return dict(replays=a.value, repeats=b.value, graphs=c.value)
+
     # -- learned psi preconditioner (no reference counterpart) -----------------
     def set_precond(self, precond: int):
         """SAV_PRECOND_DIAG (Solvers.jl:57) or SAV_PRECOND_LEARNED for the following psi solves."""
@@ -501,6 +514,11 @@ class Ensemble:
         self.e = e
         self.members = [Operators(conf, precision, _borrow=ct.c_void_p(self.lib.sav_ensemble_member(e, i)))
                         for i, conf in enumerate(self.confs)]
+
+    def set_graph(self, on: bool):
+        """Graph replay of steps on every member (Operators.set_graph)."""
+        for m in self.members:
+            m.set_graph(on)
 
     def run(self, t_nodes: np.ndarray, n0: int = 1, threads: int = 8):
         t = np.ascontiguousarray(t_nodes, dtype=np.float64)
