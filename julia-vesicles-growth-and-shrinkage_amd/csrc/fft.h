@@ -60,6 +60,18 @@ constexpr FftPlanData make_fft_plan(int L, int ttmax = 64, int maxr = 8, bool re
 // Largest radix of the psi matvec's row plan: 16 for powers of two, else 8.
 constexpr int row_plan_maxr(int L) { return (L & (L - 1)) == 0 ? 16 : 8; }
 
+// The psi matvec's column plans for power-of-two columns of at least 512: the inverse
+// kernel (one input field) runs radix-32 passes in teams of 32 lanes (E = 32), the
+// forward kernel (two input fields) radix-16 passes in teams of 64 (E = 16); other
+// lengths use the default plan.  FWD selects the forward kernel's plan.
+constexpr int col_plan_maxr(int L, bool fwd) {
+  return ((L & (L - 1)) == 0 && L >= 512) ? (fwd ? 16 : 32) : 8;
+}
+template <typename R>
+constexpr int col_plan_ttmax(int L, bool fwd) {
+  return col_plan_maxr(L, fwd) == 32 ? 32 : col_plan_maxr(L, fwd) == 16 ? 64 : (sizeof(R) == 8 ? 128 : 64);
+}
+
 constexpr bool fft_len_supported(int L) {
   int n = L;
   while (n % 3 == 0) n /= 3;
@@ -377,9 +389,9 @@ inline std::vector<cplx<R>> default_twiddles(int L) {
 // Columns per workgroup of a strided (column) transform: 8 adjacent columns (128-byte
 // row segments in FP64, 64-byte in FP32), at least 256 threads per workgroup, at most
 // 1024 threads and the 160 KB of LDS of a gfx950 CU.
-template <typename R, int L, int WANT = 8>
+template <typename R, int L, int WANT = 8, int MAXR = 8, int TTMAX = (sizeof(R) == 8 ? 128 : 64)>
 struct ColTile {
-  using F = TeamFFT<R, L, false>;
+  using F = TeamFFT<R, L, false, TTMAX, 0, default_twm<R>(), 0, MAXR>;
   static constexpr int TT = F::TT;
   static constexpr int ES = (int)sizeof(cplx<R>);
   static constexpr int want = WANT > (256 / TT) ? WANT : (256 / TT);

@@ -17,6 +17,8 @@ struct KrylovCtx {
   const cplx<R>* tw_y;        // length-Ny table
   const cplx<R>* tw_ri;       // row plan of the matvec (row_plan_maxr), inverse: k-major table
   const cplx<R>* tw_rf;       // the same plan reversed (forward)
+  const cplx<R>* tw_ci;       // column plans of the matvec (col_plan_maxr), inverse kernel
+  const cplx<R>* tw_cf;       // forward kernel (k-major tables)
   const R* M;                 // M_psi(phi*), physical (N, Ny, Nx)
   const DScal* ds;            // a, dt, Mmax
   double S4, atol, rtol;

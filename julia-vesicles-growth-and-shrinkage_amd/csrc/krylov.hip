@@ -24,12 +24,21 @@
 namespace sav {
 
 // Column kernels of the matvec: CW adjacent columns per workgroup (8 or 16; chosen per
-// handle by timing both at the first solve, see autotune_columns).
-template <typename R, int L, int CW = 8>
-using KColCfg = ColTile<R, L, CW>;
+// handle by timing both at the first solve, see autotune_columns), the column plan of
+// col_plan_maxr / col_plan_ttmax (FWD: the forward kernel's plan) with k-major twiddles
+// (KrylovCtx::tw_ci, tw_cf).
+template <typename R, int L, int CW = 8, bool FWD = false>
+using KColCfg = ColTile<R, L, CW, col_plan_maxr(L, FWD), col_plan_ttmax<R>(L, FWD)>;
+template <typename R, int L, bool INV>
+using KColFFT = TeamFFT<R, L, INV, col_plan_ttmax<R>(L, !INV), 0, 2, 0, col_plan_maxr(L, !INV)>;
+// radix-32 teams hold 32 elements per lane: let them use the registers they need
+template <typename R, int L>
+constexpr int kcol_min_waves() {
+  return col_plan_maxr(L, false) == 32 ? 1 : 4;
+}
 
-// Row kernel: one team per row, 256 threads per workgroup; the LDS of each row is
-// padded by one slot every 16 (FP64) or 32 (FP32) positions against bank conflicts.
+// Row kernel: see KRowCfg; the LDS of each row is padded by one slot every 16 positions
+// against bank conflicts.
 // In FP64, where a team is exactly one wavefront, its LDS exchanges synchronise the
 // wavefront only (no workgroup barrier; measured faster in FP64 only).
 template <typename R, int L>
@@ -59,7 +68,7 @@ __device__ __forceinline__ double pinv_diag(double a_dt, double Mmax, double S4,
 // K1: column inverse FFTs of D1f_x*(S4 x) and D1f_y*(S4 x)
 // ---------------------------------------------------------------------------
 template <typename R, int L, int CW>
-__global__ __launch_bounds__((KColCfg<R, L, CW>::NT), 4) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
+__global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
                                                                     cplx<R>* __restrict__ Bx,
                                                                     cplx<R>* __restrict__ By, int Nx, double S4,
                                                                     const double* __restrict__ kx0,
@@ -73,7 +82,7 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), 4) void k_mv_cols_inv(cons
   X += z * vs;
   Bx += z * vs;
   By += z * vs;
-  using F = TeamFFT<R, L, true>;
+  using F = KColFFT<R, L, true>;
   constexpr int C = KColCfg<R, L, CW>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
@@ -171,7 +180,7 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
 //   MODE 0: out only
 // ---------------------------------------------------------------------------
 template <typename R, int L, int MODE, int CW>
-__global__ __launch_bounds__((KColCfg<R, L, CW>::NT)) void k_mv_cols_fwd(
+__global__ __launch_bounds__((KColCfg<R, L, CW, true>::NT)) void k_mv_cols_fwd(
     const cplx<R>* __restrict__ Bx, const cplx<R>* __restrict__ By, const cplx<R>* __restrict__ Y,
     const cplx<R>* __restrict__ Cv, cplx<R>* __restrict__ out, int Nx, double S4, const double* __restrict__ kx0,
     const double* __restrict__ ky, const cplx<R>* __restrict__ tw, const DScal* __restrict__ ds,
@@ -185,8 +194,8 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT)) void k_mv_cols_fwd(
   if (Cv) Cv += z * vs;
   out += z * vs;
   partials += z * ps;
-  using F = TeamFFT<R, L, false>;
-  constexpr int C = KColCfg<R, L, CW>::C;
+  using F = KColFFT<R, L, false>;
+  constexpr int C = KColCfg<R, L, CW, true>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -561,7 +570,7 @@ static void launch_cols_inv(KrylovCtx<R>& k, const cplx<R>* y, const KScal* ks, 
     using CC = KColCfg<R, L, CW>;                                                                                \
     dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);                                                            \
     hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx, \
-                          k.S4, k.kx0, k.ky, k.tw_y, ks, vs);                                                    \
+                          k.S4, k.kx0, k.ky, k.tw_ci, ks, vs);                                                    \
     break;                                                                                                       \
   }
     SAV_KRY_SIZES(X)
@@ -578,11 +587,11 @@ static int launch_cols_fwd(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int 
   int nb3 = 0;
 #define KL(L, M)                                                                                                   \
   hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, y, partner, \
-                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_y, k.ds, ks, k.partials, vs, ps)
+                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_cf, k.ds, ks, k.partials, vs, ps)
   switch (k.Ny) {
 #define X(L)                                          \
   case L: {                                           \
-    using CC = KColCfg<R, L, CW>;                     \
+    using CC = KColCfg<R, L, CW, true>;               \
     dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys); \
     nb3 = k.Nx * k.N;  /* one partial per column */    \
     switch (mode) {                                   \
@@ -644,13 +653,13 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   return nb3;
 }
 
-// Columns per workgroup that width CW resolves to for column length Ny (the LDS and
-// thread limits can cap 16 down to 8).
+// Columns per workgroup that width CW resolves to for column length Ny, inverse kernel
+// x 1000 + forward kernel (the LDS and thread limits can cap 16 down to 8).
 template <typename R, int CW>
 static int effective_columns(int Ny) {
   switch (Ny) {
 #define X(L) \
-  case L: return KColCfg<R, L, CW>::C;
+  case L: return 1000 * KColCfg<R, L, CW>::C + KColCfg<R, L, CW, true>::C;
     SAV_KRY_SIZES(X)
 #undef X
     default: return 0;
