@@ -455,11 +455,31 @@ __global__ __launch_bounds__(256) void k_kry_upd1(const cplx<R>* __restrict__ r,
   }
 }
 
+// s = r - alpha v (the iterate's update x += alpha p waits for k_kry_upd2, which reads
+// x once per iteration; `done` changes only after upd2, so both run or neither).
+template <typename R>
+__global__ __launch_bounds__(256) void k_kry_upd1s(const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ v,
+                                                   cplx<R>* __restrict__ s, long long n,
+                                                   const KScal* __restrict__ ks, long long vs) {
+  const int z = blockIdx.y;
+  ks += z;
+  if (ks->done) return;
+  r += z * vs;
+  v += z * vs;
+  s += z * vs;
+  const cplx<R> ma = cplx<R>{(R)-ks->ar, (R)-ks->ai};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    s[i] = r[i] + ma * v[i];
+}
+
+// x = (x + alpha p) + omega s (the two updates of Krylov.jl's iteration, rounded as two
+// steps); r = s - omega t; partial <c, r>, ||r||^2.
 template <typename R>
 __global__ __launch_bounds__(256) void k_kry_upd2(const cplx<R>* __restrict__ s, const cplx<R>* __restrict__ t,
-                                                  const cplx<R>* __restrict__ c, cplx<R>* __restrict__ x,
-                                                  cplx<R>* __restrict__ r, long long n, const KScal* __restrict__ ks,
-                                                  double* __restrict__ partials, long long vs, long long ps) {
+                                                  const cplx<R>* __restrict__ c, const cplx<R>* __restrict__ p,
+                                                  cplx<R>* __restrict__ x, cplx<R>* __restrict__ r, long long n,
+                                                  const KScal* __restrict__ ks, double* __restrict__ partials,
+                                                  long long vs, long long ps) {
   __shared__ double scratch[3 * 4];
   const int z = blockIdx.y;
   ks += z;
@@ -467,16 +487,19 @@ __global__ __launch_bounds__(256) void k_kry_upd2(const cplx<R>* __restrict__ s,
   s += z * vs;
   t += z * vs;
   c += z * vs;
+  p += z * vs;
   x += z * vs;
   r += z * vs;
   partials += z * ps;
   const R wr = (R)ks->wr, wi = (R)ks->wi;
   const cplx<R> om = cplx<R>{wr, wi};
   const cplx<R> mom = cplx<R>{-wr, -wi};
+  const cplx<R> al = cplx<R>{(R)ks->ar, (R)ks->ai};
   double acc[3] = {0.0, 0.0, 0.0};
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
     const cplx<R> si = s[i];
-    x[i] = x[i] + om * si;
+    const cplx<R> x1 = x[i] + al * p[i];
+    x[i] = x1 + om * si;
     const cplx<R> ri = si + mom * t[i];
     r[i] = ri;
     const cplx<R> ci = c[i];
@@ -749,13 +772,13 @@ static void kry_iteration(KrylovCtx<R>& k, int slot, int nsys) {
   // v = P^{-1} A p ; alpha = rho / <c, v>
   int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b, nsys);
   fin_launch(k, nb3, slot, 1, nsys);
-  // s = r - alpha v ; x += alpha p
-  hipLaunchKernelGGL((k_kry_upd1<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, k.s, k.x, n, ks, vs);
+  // s = r - alpha v
+  hipLaunchKernelGGL((k_kry_upd1s<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.s, n, ks, vs);
   // t = P^{-1} A s ; omega = <t, s> / <t, t>
   nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s, nsys);
   fin_launch(k, nb3, slot, 2, nsys);
-  // x += omega s ; r = s - omega t ; <c, r>, ||r||
-  hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.s, k.t, k.b, k.x, k.r, n, ks,
+  // x += alpha p, x += omega s ; r = s - omega t ; <c, r>, ||r||
+  hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.s, k.t, k.b, k.p, k.x, k.r, n, ks,
                      k.partials, vs, ps);
   fin_launch(k, nbv, slot, 3, nsys);
   // p = r + beta (p - omega v)
