@@ -226,6 +226,8 @@ def main():
                for k in ("phi", "psi", "mu", "nu")}
         err["u"] = float(np.max(np.abs(s32.u - s64.u)) / max(np.max(np.abs(s64.u)), 1e-300))
         out["2d"] = dict(fp32=r32, fp64=r64, fp32_vs_fp64_max_rel_err=err)
+        # the same FP32 run with each step replayed as one captured HIP graph
+        out["2d"]["fp32_graph"], _ = run_case(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup, graph=True)
         if not a.no_learned:
             out["2d"]["fp32_learned"] = run_case_learned(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup)
     if a.only in ("all", "3d"):
@@ -268,6 +270,8 @@ def main():
         line["psi_iters_per_step"] = {"fp32": out["2d"]["fp32"]["krylov_iters"] / a.steps,
                                       "fp64": out["2d"]["fp64"]["krylov_iters"] / a.steps}
         line["fp64_steps_per_s_1024"] = round(out["2d"]["fp64"]["steps_per_s"], 3)
+        G = out["2d"]["fp32_graph"]
+        line["fp32_graph_replay_1024"] = {"steps_per_s": round(G["steps_per_s"], 3), **G["graph"]}
         line["psi_matvec_column_tile_1024"] = {"fp32": out["2d"]["fp32"]["autotune"],
                                                "fp64": out["2d"]["fp64"]["autotune"]}
         if "fp32_learned" in out["2d"]:
@@ -287,12 +291,15 @@ def main():
         line["fp32_vs_fp64_max_rel_err_128cubed"] = out["3d"]["fp32_vs_fp64_max_rel_err"]
     if a.only == "all":
         ens = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32)
+        ensg = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32, threads_list=(1, 16), graph=True)
         line["ensemble_64_fp32"] = {"instances": ens["instances"], "steps": ens["steps"],
                                     "sequential_steps_per_s": round(ens["by_threads"][1]["steps_per_s"], 2),
                                     "best_steps_per_s": round(ens["by_threads"][ens["best_threads"]]["steps_per_s"], 2),
                                     "best_threads": ens["best_threads"],
                                     "by_threads": {k: round(v["steps_per_s"], 2) for k, v in ens["by_threads"].items()},
-                                    "speedup_vs_sequential": round(ens["speedup_vs_sequential"], 3)}
+                                    "speedup_vs_sequential": round(ens["speedup_vs_sequential"], 3),
+                                    "graph_replay_by_threads": {k: round(v["steps_per_s"], 2)
+                                                                for k, v in ensg["by_threads"].items()}}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(jv, wl3 if "2d" in out else jv.workload_c4(n=128, steps=1))
     print(json.dumps(line), flush=True)
