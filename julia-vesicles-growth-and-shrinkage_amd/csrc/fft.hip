@@ -220,6 +220,170 @@ __global__ __launch_bounds__((RowCfg<R, L>::NT)) void k_rows_c2c(cplx<R>* data, 
 }
 
 // ---------------------------------------------------------------------------
+// 3D planes: one workgroup holds a whole (x, y) plane of a half spectrum,
+// (Nx/2+1) x Ny complex values (x fastest, contiguous in HBM), in LDS, and runs the
+// y transforms and the x real<->half transforms of that plane in one pass over HBM.
+// Square planes (Nx == Ny == N) whose half spectrum fits the 160 KB of a CU's LDS.
+// Teams of both phases lie within one wavefront and exchange through their own
+// column or row of the plane with wavefront-level barriers.  When a phase has more
+// teams than columns or rows, the wavefronts past the last one skip the phase and the
+// spare teams of the last wavefront repeat its last column or row (identical values
+// written to identical addresses), so no barrier is ever divergent.
+// ---------------------------------------------------------------------------
+template <typename R, int N>
+struct PlaneCfg {
+  static constexpr int LX = N / 2, NXH = N / 2 + 1;
+  template <bool INV>
+  using FY = TeamFFT<R, N, INV, 64, 0, default_twm<R>(), 1>;
+  template <bool INV>
+  using FX = TeamFFT<R, LX, INV, 64, 0, default_twm<R>(), 1>;
+  static constexpr int TTY = FY<false>::TT, TTX = FX<false>::TT;
+  static constexpr int need = NXH * TTY > N * TTX ? NXH * TTY : N * TTX;
+  static constexpr int NT0 = ((need + 63) / 64) * 64;
+  static constexpr int NT = NT0 > 1024 ? 1024 : NT0;
+  static constexpr int LDSN = N * NXH;
+  static constexpr bool fits = (long long)LDSN * sizeof(cplx<R>) <= 163840;
+};
+
+// Team slot of the calling thread in round r of a phase with `count` items handled by
+// teams of TT lanes: returns false when the caller's whole wavefront has nothing to do
+// in this round; otherwise sets `item` (clamped to count - 1 for spare teams).
+template <int TT>
+__device__ __forceinline__ bool plane_slot(int r, int teams_per_round, int count, int& item) {
+  const int team = threadIdx.x / TT;
+  const int wave_first = (threadIdx.x / 64) * (64 / TT);
+  if (r * teams_per_round + wave_first >= count) return false;
+  const int i = r * teams_per_round + team;
+  item = i < count ? i : count - 1;
+  return true;
+}
+
+// y transforms of the NXH columns of a plane held in LDS (element (x, y) at x + y NXH).
+template <typename R, int N, bool INV>
+__device__ __forceinline__ void plane_cols(cplx<R>* pl, const cplx<R>* __restrict__ tw) {
+  using PC = PlaneCfg<R, N>;
+  using F = typename PC::template FY<INV>;
+  constexpr int TT = F::TT, E = F::E, NXH = PC::NXH, TPR = PC::NT / TT;
+  const int t = threadIdx.x % TT;
+  for (int r = 0; r * TPR < NXH; ++r) {
+    int col;
+    if (!plane_slot<TT>(r, TPR, NXH, col)) continue;
+    cplx<R> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = pl[F::in_pos(t, e) * NXH + col];
+    F::run(v, pl + col, NXH, t, tw);
+    F::sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) pl[F::out_pos(t, e) * NXH + col] = v[e];
+  }
+}
+
+// Forward: rfft over (x, y) of whole planes, out = half spectrum planes (the z pass
+// follows).  grid = (planes per field, fields, jobs).
+template <typename R, int N>
+__global__ __launch_bounds__((PlaneCfg<R, N>::NT)) void k_plane_r2c(SpecJobs jobs, const cplx<R>* __restrict__ tw_x,
+                                                                 const cplx<R>* __restrict__ tw_x2,
+                                                                 const cplx<R>* __restrict__ tw_y) {
+  using PC = PlaneCfg<R, N>;
+  using F = typename PC::template FX<false>;
+  constexpr int L = PC::LX, NXH = PC::NXH, TT = F::TT, E = F::E, TPR = PC::NT / TT;
+  constexpr int KN = (L + TT) / TT;  // split outputs k = t + j TT <= L per lane
+  __shared__ cplx<R> pl[PC::LDSN];
+  const SpecJob& jb = jobs.j[blockIdx.z];
+  if ((int)blockIdx.y >= jb.count) return;
+  const long long plane = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+  const R* src = reinterpret_cast<const R*>(jb.src) + plane * (long long)(N * N);
+  cplx<R>* dst = reinterpret_cast<cplx<R>*>(jb.dst) + plane * (long long)PC::LDSN;
+  const int t = threadIdx.x % TT;
+  // rows: length-L complex FFT of z[j] = x[2j] + i x[2j+1], then the split
+  // X[k] = (Z_k + conj Z_{L-k}) / 2 - i W^k (Z_k - conj Z_{L-k}) / 2, W = e^{-2 pi i / 2L}
+  for (int r = 0; r * TPR < N; ++r) {
+    int row;
+    if (!plane_slot<TT>(r, TPR, N, row)) continue;
+    const cplx<R>* s = reinterpret_cast<const cplx<R>*>(src + (long long)row * N);
+    cplx<R>* my = pl + row * NXH;
+    cplx<R> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = s[F::in_pos(t, e)];
+    F::run(v, my, 1, t, tw_x);
+    F::sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) my[F::out_pos(t, e)] = v[e];
+    F::sync();
+    cplx<R> xs[KN];
+#pragma unroll
+    for (int j = 0; j < KN; ++j) {
+      const int k = t + j * TT;
+      if (k <= L) {
+        const cplx<R> zk = my[k == L ? 0 : k];
+        const cplx<R> zm = conj(my[k == 0 ? 0 : L - k]);
+        const cplx<R> ev = R(0.5) * (zk + zm);
+        const cplx<R> od = mul_mi(R(0.5) * (zk - zm));
+        xs[j] = ev + tw_x2[k] * od;
+      }
+    }
+    F::sync();
+#pragma unroll
+    for (int j = 0; j < KN; ++j) {
+      const int k = t + j * TT;
+      if (k <= L) my[k] = xs[j];
+    }
+  }
+  __syncthreads();
+  plane_cols<R, N, false>(pl, tw_y);
+  __syncthreads();
+  for (int i = threadIdx.x; i < PC::LDSN; i += PC::NT) dst[i] = pl[i];
+}
+
+// Inverse: irfft over (x, y) of whole half-spectrum planes (after the z pass):
+// y transforms, then the c2r rows that ignore the imaginary parts of the x DC and
+// Nyquist bins (FFTW's c2r semantics); output scaled by `scale`.
+template <typename R, int N>
+__global__ __launch_bounds__((PlaneCfg<R, N>::NT)) void k_plane_c2r(SpecJobs jobs, const cplx<R>* __restrict__ tmp,
+                                                                 double scale, const cplx<R>* __restrict__ tw_x,
+                                                                 const cplx<R>* __restrict__ tw_x2,
+                                                                 const cplx<R>* __restrict__ tw_y) {
+  using PC = PlaneCfg<R, N>;
+  using F = typename PC::template FX<true>;
+  constexpr int L = PC::LX, NXH = PC::NXH, TT = F::TT, E = F::E, TPR = PC::NT / TT;
+  __shared__ cplx<R> pl[PC::LDSN];
+  const SpecJob& jb = jobs.j[blockIdx.z];
+  if ((int)blockIdx.y >= jb.count) return;
+  const long long plane = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+  const cplx<R>* src = tmp + jb.toff + plane * (long long)PC::LDSN;
+  R* dst = reinterpret_cast<R*>(jb.dst) + plane * (long long)(N * N);
+  for (int i = threadIdx.x; i < PC::LDSN; i += PC::NT) pl[i] = src[i];
+  __syncthreads();
+  plane_cols<R, N, true>(pl, tw_y);
+  __syncthreads();
+  const R sc = (R)scale;
+  const int t = threadIdx.x % TT;
+  for (int r = 0; r * TPR < N; ++r) {
+    int row;
+    if (!plane_slot<TT>(r, TPR, N, row)) continue;
+    cplx<R>* my = pl + row * NXH;
+    cplx<R> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = F::in_pos(t, e);
+      cplx<R> xk = my[k];
+      cplx<R> xm = my[L - k];
+      if (k == 0) {
+        xk.y = R(0);
+        xm.y = R(0);
+      }
+      const cplx<R> ev = xk + conj(xm);
+      const cplx<R> od = (xk - conj(xm)) * conj(tw_x2[k]);
+      v[e] = ev + mul_i(od);
+    }
+    F::run(v, my, 1, t, tw_x);
+    cplx<R>* o = reinterpret_cast<cplx<R>*>(dst + (long long)row * N);
+#pragma unroll
+    for (int e = 0; e < E; ++e) o[F::out_pos(t, e)] = cplx<R>{sc * v[e].x, sc * v[e].y};
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host dispatch
 // ---------------------------------------------------------------------------
 #define SAV_FFT_SIZES(X) \
@@ -343,9 +507,55 @@ static ColGeom geom_z(const FftGeom<R>& g) {
   return c;
 }
 
+// Plane sizes with fused (x, y) kernels: N x N planes of 3D fields.
+#define SAV_PLANE_SIZES(X) X(16) X(32) X(64) X(96) X(128)
+
+template <typename R>
+static bool plane_path(const FftGeom<R>& g) {
+  if (g.ndim != 3 || g.Nx != g.Ny) return false;
+  switch (g.Nx) {
+#define X(NN) \
+  case NN: return PlaneCfg<R, NN>::fits;
+    SAV_PLANE_SIZES(X)
+#undef X
+    default: return false;
+  }
+}
+
+template <typename R>
+static void launch_plane(const FftGeom<R>& g, const SpecJobs& jobs, bool inverse, const cplx<R>* tmp, double scale,
+                         hipStream_t s) {
+  const dim3 grid(g.Nz, max_count(jobs), jobs.n);
+  switch (g.Nx) {
+#define X(NN)                                                                                              \
+  case NN: {                                                                                               \
+    if constexpr (PlaneCfg<R, NN>::fits) {                                                                 \
+      if (inverse)                                                                                         \
+        hipLaunchKernelGGL((k_plane_c2r<R, NN>), grid, dim3(PlaneCfg<R, NN>::NT), 0, s, jobs, tmp, scale,    \
+                           g.tw_xh, g.tw_x2, g.tw_y);                                                      \
+      else                                                                                                 \
+        hipLaunchKernelGGL((k_plane_r2c<R, NN>), grid, dim3(PlaneCfg<R, NN>::NT), 0, s, jobs, g.tw_xh,       \
+                           g.tw_x2, g.tw_y);                                                               \
+    }                                                                                                      \
+    break;                                                                                                 \
+  }
+    SAV_PLANE_SIZES(X)
+#undef X
+    default: throw HipError("no plane kernel for N = " + std::to_string(g.Nx), -1);
+  }
+  SAV_LAUNCH_CHECK();
+}
+
 template <typename R>
 void rfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs, hipStream_t s) {
   if (jobs.n == 0) return;
+  if (plane_path(g)) {
+    launch_plane<R>(g, jobs, false, nullptr, 1.0, s);
+    ColGeom cz = geom_z(g);
+    cz.use_src = 0;
+    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s);
+    return;
+  }
   const int rows = g.Ny * (g.ndim == 3 ? g.Nz : 1);
   launch_r2c<R>(g.Nx / 2, jobs, rows, g.tw_xh, g.tw_x2, s);
   ColGeom cy = geom_y(g);
@@ -379,6 +589,10 @@ void irfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs_in, cplx<R>* tmp, hipS
     ColGeom cz = geom_z(g);
     cz.use_src = 1;
     launch_cols<R, true>(g.Nz, tj, cz, g.tw_z, g.k, s);
+    if (plane_path(g)) {
+      launch_plane<R>(g, jobs, true, tmp, 1.0 / ((double)g.Nx * g.Ny * g.Nz), s);
+      return;
+    }
     ColGeom cy = geom_y(g);
     cy.use_src = 0;
     launch_cols<R, true>(g.Ny, tj, cy, g.tw_y, g.k, s);
