@@ -103,4 +103,36 @@ __device__ __forceinline__ void block_sum(double (&acc)[K], double* scratch) {
 
 template <typename R> __device__ __forceinline__ double to_d(R v) { return (double)v; }
 
+// ---------------------------------------------------------------------------
+// XCD-aware workgroup numbering.  The workgroups of a launch reach the 8 XCDs of the
+// chip round-robin by linear id, and each XCD has its own L2.  Tiles that share cache
+// lines (e.g. two 64-byte column segments of one 128-byte line) should run on one XCD
+// at about the same time, so that the second read of a line hits that XCD's L2 instead
+// of fetching the line from HBM again.
+// ---------------------------------------------------------------------------
+constexpr unsigned kNumXcd = 8;
+
+// Bijection on [0, total): the ids one XCD receives (lin = x, x + 8, x + 16, ...) map to a
+// contiguous range of logical ids, in dispatch order.
+__device__ __forceinline__ unsigned xcd_spread(unsigned lin, unsigned total) {
+  const unsigned q = total / kNumXcd, r = total % kNumXcd, x = lin % kNumXcd, k = lin / kNumXcd;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+struct BlockIdx3 {
+  int x, y, z;
+};
+
+// Logical block index of the calling workgroup.  With a multiple of 8 blocks along x the
+// renumbering stays within each (y, z) row of tiles (every row keeps all 8 XCDs, so rows
+// that exit early, such as converged systems of a batched solve, idle no XCD); otherwise
+// it runs over the whole grid.
+__device__ __forceinline__ BlockIdx3 xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  if (gx % kNumXcd == 0) return BlockIdx3{(int)xcd_spread(blockIdx.x, gx), (int)blockIdx.y, (int)blockIdx.z};
+  const unsigned total = gx * gy * gridDim.z;
+  const unsigned l = xcd_spread(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), total);
+  return BlockIdx3{(int)(l % gx), (int)((l / gx) % gy), (int)(l / (gx * gy))};
+}
+
 }  // namespace sav

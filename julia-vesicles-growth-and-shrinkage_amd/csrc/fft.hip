@@ -40,13 +40,14 @@ __global__ __launch_bounds__((ColCfg<R, L>::NT)) void k_cols(SpecJobs jobs, ColG
   constexpr int C = ColCfg<R, L>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
-  const SpecJob& jb = jobs.j[blockIdx.z];
-  const int fidx = blockIdx.y / cg.nsub;
-  const int sub = blockIdx.y % cg.nsub;
+  const BlockIdx3 bk = xcd_block();
+  const SpecJob& jb = jobs.j[bk.z];
+  const int fidx = bk.y / cg.nsub;
+  const int sub = bk.y % cg.nsub;
   if (fidx >= jb.count) return;
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
+  const int col = bk.x * C + c;
   const bool act = col < cg.inner;
   const long long base = (long long)fidx * cg.field_stride + (long long)sub * cg.sub_stride + col;
   cplx<R>* dst = reinterpret_cast<cplx<R>*>(jb.dst);

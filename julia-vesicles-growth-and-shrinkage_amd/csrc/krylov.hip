@@ -75,8 +75,9 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) 
                                                                     const double* __restrict__ ky,
                                                                     const cplx<R>* __restrict__ tw,
                                                                     const KScal* __restrict__ ks, long long vs) {
-  // blockIdx.z: system of a batched solve (vectors vs apart, one KScal each)
-  const int z = blockIdx.z;
+  // block z: system of a batched solve (vectors vs apart, one KScal each)
+  const BlockIdx3 bk = xcd_block();
+  const int z = bk.z;
   ks += z;
   if (ks->done) return;
   X += z * vs;
@@ -87,9 +88,9 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) 
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
+  const int col = bk.x * C + c;
   const bool act = col < Nx;
-  const long long base = (long long)blockIdx.y * Nx * L + col;
+  const long long base = (long long)bk.y * Nx * L + col;
   const R kxv = act ? (R)kx0[col] : R(0);
   const R s4 = (R)S4;
   // b1 = S4 x is kept; the two derivative transforms run one after the other
@@ -185,7 +186,8 @@ __global__ __launch_bounds__((KColCfg<R, L, CW, true>::NT)) void k_mv_cols_fwd(
     const cplx<R>* __restrict__ Cv, cplx<R>* __restrict__ out, int Nx, double S4, const double* __restrict__ kx0,
     const double* __restrict__ ky, const cplx<R>* __restrict__ tw, const DScal* __restrict__ ds,
     const KScal* __restrict__ ks, double* __restrict__ partials, long long vs, long long ps) {
-  const int z = blockIdx.z;
+  const BlockIdx3 bk = xcd_block();
+  const int z = bk.z;
   ks += z;
   if (ks->done) return;
   Bx += z * vs;
@@ -199,9 +201,9 @@ __global__ __launch_bounds__((KColCfg<R, L, CW, true>::NT)) void k_mv_cols_fwd(
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
+  const int col = bk.x * C + c;
   const bool act = col < Nx;
-  const long long base = (long long)blockIdx.y * Nx * L + col;
+  const long long base = (long long)bk.y * Nx * L + col;
   const double a_dt = ds->a / ds->dt;
   const double Mmax = ds->Mmax;
   const double kxd = act ? kx0[col] : 0.0;
@@ -285,7 +287,7 @@ __global__ __launch_bounds__((KColCfg<R, L, CW, true>::NT)) void k_mv_cols_fwd(
     __syncthreads();
     if (threadIdx.x < C) {
       const int cc = threadIdx.x;
-      const int colc = blockIdx.x * C + cc;
+      const int colc = bk.x * C + cc;
       double s0 = 0.0, s1 = 0.0, s2 = 0.0;
       for (int tt = 0; tt < TT; ++tt) {
         const int th = tt * C + cc;
@@ -294,7 +296,7 @@ __global__ __launch_bounds__((KColCfg<R, L, CW, true>::NT)) void k_mv_cols_fwd(
         s2 += red[3 * th + 2];
       }
       if (colc < Nx) {
-        const long long b = (long long)blockIdx.y * Nx + colc;
+        const long long b = (long long)bk.y * Nx + colc;
         partials[b * 3 + 0] = s0;
         partials[b * 3 + 1] = s1;
         partials[b * 3 + 2] = s2;
@@ -537,10 +539,13 @@ __global__ __launch_bounds__(256) void k_kry_upd3(const cplx<R>* __restrict__ r,
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+// Workgroups of a Krylov vector kernel over n elements per system: at most one per 1024
+// elements, hence at most N Nx of them (Ny <= 1024), which is what the per-system
+// partial slots (3 max(N Nx, 512) doubles) hold.
 static int vec_blocks(long long n) {
   long long nb = (n + 1023) / 1024;
   if (nb < 1) nb = 1;
-  if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
+  if (nb > KRY_VEC_BLOCKS) nb = KRY_VEC_BLOCKS;
   return (int)nb;
 }
 

@@ -4,7 +4,13 @@
 
 namespace sav {
 
-constexpr int RED_MAX_BLOCKS = 512;
+// Workgroups of the Krylov vector kernels per system.
+constexpr int KRY_VEC_BLOCKS = 2048;
+// Workgroups of a reduce_n / max stage: 8 per CU of 256, so that the streaming loads
+// of a reduction have enough wavefronts in flight.
+constexpr int RED_STAGE_BLOCKS = 2048;
+// Doubles in the solver's partials buffer.
+constexpr long long RED_PARTIALS = 3 * 65536;
 
 template <class F>
 __global__ __launch_bounds__(256) void k_foreach(long long n, F f) {
@@ -56,7 +62,10 @@ template <int K, class F>
 inline void reduce_n(hipStream_t s, long long n, int ngroups, F f, double* partials, double* out) {
   long long nb = (n + 1023) / 1024;
   if (nb < 1) nb = 1;
-  if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
+  if (nb > RED_STAGE_BLOCKS) nb = RED_STAGE_BLOCKS;
+  const long long cap = RED_PARTIALS / ((long long)K * (ngroups > 0 ? ngroups : 1));
+  if (nb > cap) nb = cap;
+  if (nb < 1) throw HipError("reduce_n: too many groups for the partials buffer", -1);
   hipLaunchKernelGGL((k_reduce<K, F>), dim3((unsigned)nb, ngroups), dim3(256), 0, s, n, f, partials);
   SAV_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_red_fin, dim3(K, ngroups), dim3(256), 0, s, partials, (int)nb, K, out);

@@ -174,7 +174,7 @@ class Solver final : public ISolver {
     d_psi_out = upload(hc.psi_out);
     ds = dalloc<DScal>(1);
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(DScal), hipHostMallocDefault));
-    partials = dalloc<double>(3 * 65536);
+    partials = dalloc<double>(RED_PARTIALS);
     red = dalloc<double>(1024);
     // work buffers
     const long long NP = (long long)N * P, NH = (long long)N * H;
@@ -236,7 +236,7 @@ class Solver final : public ISolver {
       // three systems (the batched step-1 solves), nf apart
       kc.vs = nf;
       // partials per system: 3 x (one per column of the column-forward kernel, or one per
-      // block of a vector kernel, at most 512)
+      // block of a vector kernel, at most N Nx)
       kc.ps = 3 * std::max<long long>((long long)Nx * N, 512);
       kc.x = dalloc<C>(3 * nf);
       kc.r = dalloc<C>(3 * nf);
@@ -2137,7 +2137,7 @@ template <typename R>
 void Solver<R>::reduce_max_impl(const R* x, long long n, double* out) {
   long long nb = (n + 1023) / 1024;
   if (nb < 1) nb = 1;
-  if (nb > RED_MAX_BLOCKS) nb = RED_MAX_BLOCKS;
+  if (nb > RED_STAGE_BLOCKS) nb = RED_STAGE_BLOCKS;
   hipLaunchKernelGGL((k_max_stage<R>), dim3((unsigned)nb), dim3(256), 0, s, x, n, partials);
   hipLaunchKernelGGL(k_max_fin, dim3(1), dim3(256), 0, s, partials, (int)nb, out);
   SAV_LAUNCH_CHECK();
