@@ -31,10 +31,13 @@ struct KrylovCtx {
   KScal* hks;                 // pinned host mirror
   hipStream_t st;
   int guess[3];               // expected iteration count per slot (batching of convergence checks)
-  // optional event timing of one matvec kernel (1: column inverse, 2: rows, 3: column forward)
+  // optional event timing of one matvec kernel (1: column inverse, 2: rows, 3: column forward):
+  // every ev_stride-th launch of it carries start/stop events (ev_seq counts its launches)
   int timed = 0;
   hipEvent_t* ev = nullptr;   // 2 * ev_cap events
   int ev_cap = 0, ev_used = 0;
+  int ev_stride = 3;
+  long long ev_seq = 0;
   int cur_iter = 0;
   int ev_iter[512];
   double ev_bytes[512];

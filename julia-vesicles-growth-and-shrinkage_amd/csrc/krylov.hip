@@ -551,20 +551,22 @@ static int vec_blocks(long long n) {
 
 #define SAV_KRY_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
-// Start/stop events of the next launch of kernel `id` when it is the timed one (null
-// otherwise). They are passed to hipExtLaunchKernel, which stamps them from the
-// dispatch itself, so their interval is the kernel's own duration.
+// Start/stop events of the next launch of kernel `id` when it is the timed one and the
+// launch is sampled (null otherwise). They are passed to hipExtLaunchKernel, which stamps
+// them from the dispatch itself, so their interval is the kernel's own duration. The
+// events delay the following dispatch by a few microseconds, so only every ev_stride-th
+// launch carries them (an odd stride samples both matvecs of an iteration alike).
 template <typename R>
 static void tev_pre(KrylovCtx<R>& k, int id, hipEvent_t& e0, hipEvent_t& e1) {
   e0 = e1 = nullptr;
-  if (k.timed == id && k.ev_used < k.ev_cap) {
+  if (k.timed == id && k.ev_used < k.ev_cap && (k.ev_seq++ % k.ev_stride) == 0) {
     e0 = k.ev[2 * k.ev_used];
     e1 = k.ev[2 * k.ev_used + 1];
   }
 }
 template <typename R>
-static void tev_post(KrylovCtx<R>& k, int id, double bytes) {
-  if (k.timed == id && k.ev_used < k.ev_cap) {
+static void tev_post(KrylovCtx<R>& k, hipEvent_t e0, double bytes) {
+  if (e0) {
     k.ev_iter[k.ev_used] = k.cur_iter;
     k.ev_bytes[k.ev_used] = bytes;
     k.ev_used++;
@@ -655,7 +657,7 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
     launch_cols_inv<R, 16>(k, y, ks, nsys, e0, e1);
   else
     launch_cols_inv<R, 8>(k, y, ks, nsys, e0, e1);
-  tev_post(k, 1, 6.0 * sn);
+  tev_post(k, e0, 6.0 * sn);
   tev_pre(k, 2, e0, e1);
   const long long nrh = (long long)k.N * k.Ny;
   const double scale = 1.0 / ((double)k.Nx * k.Ny);
@@ -673,11 +675,11 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
     default: throw HipError("unsupported Nx for the psi solve: " + std::to_string(k.Nx), -1);
   }
   SAV_LAUNCH_CHECK();
-  tev_post(k, 2, 9.0 * sn);
+  tev_post(k, e0, 9.0 * sn);
   tev_pre(k, 3, e0, e1);
   const int nb3 = k.colw == 16 ? launch_cols_fwd<R, 16>(k, y, out, mode, partner, ks, nsys, e0, e1)
                                : launch_cols_fwd<R, 8>(k, y, out, mode, partner, ks, nsys, e0, e1);
-  tev_post(k, 3, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn);
+  tev_post(k, e0, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn);
   return nb3;
 }
 

@@ -1774,6 +1774,12 @@ class Solver final : public ISolver {
     io = oldp;
     inew = oldo;
   }
+  void unrotate() {  // undoes rotate() of a step that is not committed
+    const int oldp = io, oldo = inew;
+    inew = ip;
+    ip = oldp;
+    io = oldo;
+  }
   // After rotate(): the back-transform of the new state (simulation.jl:99-104) and the
   // energy/area monitor (simulation.jl:146-148).
   void step_tail(bool monitor_on) {
@@ -1947,16 +1953,18 @@ class Solver final : public ISolver {
     step4();
     step5();
     step67();
-    // fatal W + C < 0 or non-finite: the step is not committed
-    SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
-    SAV_HIP(hipStreamSynchronize(s));
-    if (hs->flags & 7) return SAV_E_WC_NEGATIVE;
-    if (hs->flags & 16) return SAV_E_NONFINITE;
+    // The back-transform and the monitor write only the new state's buffers, so they are
+    // queued before the status is known (one host synchronisation per step); a fatal
+    // W + C < 0 or non-finite step is then not committed: present and old are intact.
     rotate();
     step_tail(monitor_on);
     SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
     SAV_HIP(hipMemcpyAsync(hst, dstate + ip, sizeof(DStateScal), hipMemcpyDeviceToHost, s));
     SAV_HIP(hipStreamSynchronize(s));
+    if (hs->flags & 23) {
+      unrotate();
+      return (hs->flags & 7) ? SAV_E_WC_NEGATIVE : SAV_E_NONFINITE;
+    }
     return finish_step(st);
   }
 
