@@ -59,11 +59,16 @@ def _setup(jv, wl, precision):
     return ops
 
 
+COLUMN_TILE = 0  # psi-matvec columns per workgroup: 0 = autotuned per handle, else fixed (8 or 16)
+
+
 def run_case(jv, wl, precision, steps, warmup, timed_kernel=0, graph=False):
     """Warmup steps untimed, then `steps` timed steps (inputs resident in HBM); with
     `graph`, each step is replayed as one captured HIP graph."""
     conf = wl.conf
     ops = _setup(jv, wl, precision)
+    if COLUMN_TILE and conf.ndim == 2:
+        ops.set_column_tile(COLUMN_TILE)
     ops.set_graph(graph)
     t = np.arange(warmup + steps + 1) * conf.dt
     if warmup:
@@ -186,6 +191,89 @@ def cpu_baseline(jv, wl, nsteps=3):
                        f"{dt:.1f} s")
 
 
+def algorithmic_bytes(conf, s, k):
+    """Algorithmic HBM bytes of one SAV step: every logical transform reads its input and
+    writes its output once, fused pointwise work counts its unique operands, a state
+    rotation is free.  u = one real N-field, v = one scalar field (s bytes per value);
+    k = psi BiCGSTAB iterations of the step (88 u each: two matvecs and the vector
+    updates over full complex spectra).  2D: (282 + 88 k) u + 94 v; 3D: 280 u + 144 v."""
+    pts = conf.Nx * conf.Ny * (conf.Nz if conf.ndim == 3 else 1)
+    u, v = s * pts * conf.N, s * pts
+    if conf.ndim == 3:
+        return 280 * u + 144 * v
+    return (282 + 88 * k) * u + 94 * v
+
+
+def _run_full(jv, wl, precision):
+    """The workload's whole step schedule (uniform dt, BDF1 then BDF2) after the SAV
+    initialisation; the steps are timed together."""
+    conf = wl.conf
+    ops = _setup(jv, wl, precision)
+    t = np.arange(wl.steps + 1) * conf.dt
+    _sync()
+    t0 = time.perf_counter()
+    rc, E, A, kit = ops.run(t, 1)
+    _sync()
+    el = time.perf_counter() - t0
+    st = ops.get_state(jv.SAV_PRESENT)
+    ops.close()
+    return dict(rc=int(rc), steps_per_s=wl.steps / el, seconds=el, krylov_iters=int(kit), E=np.asarray(E)), st
+
+
+def _rel(a, b):
+    den = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a - b))) / den if den > 0 else float(np.max(np.abs(a - b)))
+
+
+def run_configs(jv, cpu=True):
+    """One row per benchmark configuration at its full step count: SAV steps/s in FP64
+    and in FP32 (fields FP32, reductions FP64), psi iterations per step, the max-norm
+    relative error of FP32 against FP64 for phi, psi, u and the modified energy (over
+    every step), algorithmic bytes per step and the rate they imply.  The single-vesicle
+    128^2 case also runs the FP64 CPU oracle over all its steps and reports the GPU's
+    FP64 error against it."""
+    rows = {}
+    for wl in (jv.workload_c1(), jv.workload_c2(), jv.workload_c3(), jv.workload_c4()):
+        conf = wl.conf
+        r64, s64 = _run_full(jv, wl, jv.SAV_PREC_FP64)
+        r32, s32 = _run_full(jv, wl, jv.SAV_PREC_FP32)
+        row = dict(grid=f"{conf.Nx}^{conf.ndim}" if conf.ndim == 3 else f"{conf.Nx}x{conf.Ny}", N=conf.N,
+                   steps=wl.steps, epsilon=conf.epsilon, dt=conf.dt, status=[r64["rc"], r32["rc"]])
+        for name, r, s in (("fp64", r64, 8), ("fp32", r32, 4)):
+            k = r["krylov_iters"] / wl.steps
+            B = algorithmic_bytes(conf, s, k)
+            row[name] = dict(steps_per_s=round(r["steps_per_s"], 2), psi_iters_per_step=round(k, 3),
+                             algorithmic_gb_per_step=round(B / 1e9, 4),
+                             algorithmic_tb_per_s=round(B * r["steps_per_s"] / 1e12, 3))
+        row["fp32_vs_fp64_max_rel_err"] = {
+            "phi": _rel(s32.phi, s64.phi), "psi": _rel(s32.psi, s64.psi), "u": _rel(s32.u, s64.u),
+            "energy": _rel(r32["E"], r64["E"])}
+        row["energy_monotone"] = {"fp64": bool(np.all(np.diff(r64["E"]) <= 0)),
+                                  "fp32": bool(np.all(np.diff(r32["E"]) <= 0))}
+        if cpu and wl.name == "single_vesicle_128":
+            from oracle import sav_oracle as so
+            phi, psi = wl.initial_fields()
+            orc = so.Oracle(conf, workers=min(os.cpu_count() or 1, 16))
+            pres, old = orc.init_sav(so.state_from_physical(phi, psi, orc.ops, conf.ndim))
+            t0 = time.perf_counter()
+            for n in range(1, wl.steps + 1):
+                orc.step(pres, old, conf.dt, n)
+            el = time.perf_counter() - t0
+            row["cpu_oracle"] = dict(steps_per_s=round(wl.steps / el, 3), cores=min(os.cpu_count() or 1, 16),
+                                     gpu_fp64_vs_oracle_max_rel_err={
+                                         "phi": _rel(s64.phi.T, pres.phi), "psi": _rel(s64.psi.T, pres.psi)})
+        rows[wl.name] = row
+        print(f"config {wl.name}: fp64 {row['fp64']['steps_per_s']} fp32 {row['fp32']['steps_per_s']} steps/s",
+              file=sys.stderr, flush=True)
+    ens = run_ensemble(jv, 500, jv.SAV_PREC_FP32, threads_list=(1, 16))
+    rows["ensemble_64"] = dict(steps=500, precision="fp32",
+                               sequential_steps_per_s=round(ens["by_threads"][1]["steps_per_s"], 2),
+                               threads16_steps_per_s=round(ens["by_threads"][16]["steps_per_s"], 2),
+                               speedup_vs_sequential=round(ens["speedup_vs_sequential"], 3),
+                               status=[v["rc"] for v in ens["by_threads"].values()])
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +283,12 @@ def main():
     ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble"], default="all")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--column-tile", type=int, default=0, choices=[0, 8, 16],
+                    help="fix the psi matvec's columns per workgroup (0: autotuned; fixing it makes "
+                         "separate profiler runs launch the same kernels)")
+    ap.add_argument("--configs", action="store_true",
+                    help="one row per benchmark configuration at its full step count (a separate report, "
+                         "not the headline line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB (x4), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -203,6 +297,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not built)"}))
         sys.exit(2)
+    global COLUMN_TILE
+    COLUMN_TILE = a.column_tile
     import __graft_entry__ as ge
     jv = ge.load_package()
     import torch  # noqa: F401  (device-wide synchronisation around the timed region)
@@ -214,6 +310,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if a.configs:
+        print(json.dumps(run_configs(jv, cpu=not a.no_cpu)), flush=True)
+        return
     if a.only == "ensemble":
         print(json.dumps(run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32)), flush=True)
         return
