@@ -36,7 +36,7 @@ struct KrylovCtx {
   int timed = 0;
   hipEvent_t* ev = nullptr;   // 2 * ev_cap events
   int ev_cap = 0, ev_used = 0;
-  int ev_stride = 3;
+  int ev_stride = 9;
   long long ev_seq = 0;
   int cur_iter = 0;
   int ev_iter[512];
