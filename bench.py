@@ -295,7 +295,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(sharding over several GPUs is designed, not built)"}))
         sys.exit(2)
     global COLUMN_TILE
     COLUMN_TILE = a.column_tile

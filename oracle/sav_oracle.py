@@ -26,8 +26,28 @@ reads only the real part of the DC and Nyquist bins -- the same semantics as
 FFTW's ``irfft`` on non-Hermitian half spectra.  ``tests/test_oracle_kat.py``
 pins this against a brute-force sum.
 
-Parity status per function (which check pins it) is listed in DESIGN.md of the
-repository; functions with no independent check say "parity unpinned" there.
+What pins each function (tests/test_oracle_kat.py, tests/test_golden.py):
+
+* ``Ops`` wavenumbers and D1/Lap/Bih: the reference's own known-answer tests
+  (test/test_Types_Ops.jl:71-136, Nyquist signs included).
+* ``rfft``/``irfft``: the reference's round-trip and derivative tests
+  (test_Types_Ops.jl:142-314), a brute-force DFT, and the brute-force c2r definition
+  on non-Hermitian input.
+* ``bdf_coeff``: test_Types_Ops.jl:369-395.
+* ``f_surf``/``calculate_area``, ``f_bend``/``get_omega``: closed forms (tanh stripe,
+  constant phi).
+* ``get_W2``/``get_H2``, ``get_W3``/``get_H3``/``get_MG``: Gateaux derivatives
+  (H2 R2 = dW2/dphi exactly when gamma_surf = 0; H3, G exact derivatives of W3).
+* ``get_H1``/``get_W1``: constant-phi closed form only; H1 is not an exact derivative
+  (Utils.jl:198-206), so its general formula is parity unpinned beyond restating it.
+* ``bicgstab``: dense ``numpy.linalg.solve`` of the assembled operator on 8x8 grids,
+  the constant-mobility case (a spectral divide), a zero right-hand side, itmax
+  reporting; the exact iterate of Krylov.jl at ``itmax`` is parity unpinned (it needs
+  the Julia package).
+* ``solve_step1`` .. ``solve_step7``, ``step``: invariants (spectral div u = 0,
+  modified energy non-increasing, mean psi conserved, BDF1 first step); end-to-end
+  trajectories of the Julia code are parity unpinned (no Julia here, no golden data in
+  the reference).
 """
 from __future__ import annotations
 
