@@ -61,15 +61,18 @@ constexpr FftPlanData make_fft_plan(int L, int ttmax = 64, int maxr = 8, bool re
 constexpr int row_plan_maxr(int L) { return (L & (L - 1)) == 0 ? 16 : 8; }
 
 // The psi matvec's column plans for power-of-two columns of at least 512: the inverse
-// kernel (one input field) runs radix-32 passes in teams of 32 lanes (E = 32), the
-// forward kernel (two input fields) radix-16 passes in teams of 64 (E = 16); other
-// lengths use the default plan.  FWD selects the forward kernel's plan.
+// kernel (one input field) runs radix-32 passes in teams of 32 lanes (E = 32) in FP32 and
+// radix-16 passes in teams of 64 in FP64 (at 128 KB of LDS per workgroup, one workgroup
+// per CU: 64-lane teams keep two wavefronts per SIMD instead of one); the forward kernel
+// (two input fields) radix-16 passes in teams of 64 (E = 16); other lengths use the
+// default plan.  FWD selects the forward kernel's plan.
+template <typename R>
 constexpr int col_plan_maxr(int L, bool fwd) {
-  return ((L & (L - 1)) == 0 && L >= 512) ? (fwd ? 16 : 32) : 8;
+  return ((L & (L - 1)) == 0 && L >= 512) ? ((fwd || sizeof(R) == 8) ? 16 : 32) : 8;
 }
 template <typename R>
 constexpr int col_plan_ttmax(int L, bool fwd) {
-  return col_plan_maxr(L, fwd) == 32 ? 32 : col_plan_maxr(L, fwd) == 16 ? 64 : (sizeof(R) == 8 ? 128 : 64);
+  return col_plan_maxr<R>(L, fwd) == 32 ? 32 : col_plan_maxr<R>(L, fwd) == 16 ? 64 : (sizeof(R) == 8 ? 128 : 64);
 }
 
 constexpr bool fft_len_supported(int L) {

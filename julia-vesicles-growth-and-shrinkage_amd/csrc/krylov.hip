@@ -28,13 +28,15 @@ namespace sav {
 // col_plan_maxr / col_plan_ttmax (FWD: the forward kernel's plan) with k-major twiddles
 // (KrylovCtx::tw_ci, tw_cf).
 template <typename R, int L, int CW = 8, bool FWD = false>
-using KColCfg = ColTile<R, L, CW, col_plan_maxr(L, FWD), col_plan_ttmax<R>(L, FWD)>;
+using KColCfg = ColTile<R, L, CW, col_plan_maxr<R>(L, FWD), col_plan_ttmax<R>(L, FWD)>;
 template <typename R, int L, bool INV>
-using KColFFT = TeamFFT<R, L, INV, col_plan_ttmax<R>(L, !INV), 0, 2, 0, col_plan_maxr(L, !INV)>;
-// radix-32 teams hold 32 elements per lane: let them use the registers they need
+using KColFFT = TeamFFT<R, L, INV, col_plan_ttmax<R>(L, !INV), 0, 2, 0, col_plan_maxr<R>(L, !INV)>;
+// Registers of the column-inverse kernel: radix-32 teams hold 32 elements per lane (one
+// wavefront per SIMD); FP64 64-lane teams hold S4 x and a transform of 16 complex doubles
+// per lane, at two wavefronts per SIMD (the LDS holds one workgroup per CU anyway).
 template <typename R, int L>
 constexpr int kcol_min_waves() {
-  return col_plan_maxr(L, false) == 32 ? 1 : 4;
+  return col_plan_maxr<R>(L, false) == 32 ? 1 : (sizeof(R) == 8 ? 2 : 4);
 }
 
 // Row kernel: see KRowCfg; the LDS of each row is padded by one slot every 16 positions

@@ -224,8 +224,8 @@ class Solver final : public ISolver {
       kc.tw_x = g.tw_x;
       kc.tw_ri = upload(kmajor_twiddles<R>(Nx, 64, row_plan_maxr(Nx), false));
       kc.tw_rf = upload(kmajor_twiddles<R>(Nx, 64, row_plan_maxr(Nx), true));
-      kc.tw_ci = upload(kmajor_twiddles<R>(Ny, col_plan_ttmax<R>(Ny, false), col_plan_maxr(Ny, false), false));
-      kc.tw_cf = upload(kmajor_twiddles<R>(Ny, col_plan_ttmax<R>(Ny, true), col_plan_maxr(Ny, true), false));
+      kc.tw_ci = upload(kmajor_twiddles<R>(Ny, col_plan_ttmax<R>(Ny, false), col_plan_maxr<R>(Ny, false), false));
+      kc.tw_cf = upload(kmajor_twiddles<R>(Ny, col_plan_ttmax<R>(Ny, true), col_plan_maxr<R>(Ny, true), false));
       kc.tw_y = g.tw_y;
       kc.M = w_M;
       kc.ds = ds;
