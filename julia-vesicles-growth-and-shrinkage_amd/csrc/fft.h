@@ -75,6 +75,12 @@ constexpr int col_plan_ttmax(int L, bool fwd) {
   return col_plan_maxr<R>(L, fwd) == 32 ? 32 : col_plan_maxr<R>(L, fwd) == 16 ? 64 : (sizeof(R) == 8 ? 128 : 64);
 }
 
+// Largest radix of the general column transforms (k_cols: y and z passes, the full
+// complex 2D transforms): 16 in FP32 for power-of-two lengths of at least 512 (16 16 4 at
+// 1024: two LDS exchanges instead of three), else the default 8.
+constexpr int cols_maxr(int L, bool fp32) { return (fp32 && (L & (L - 1)) == 0 && L >= 512) ? 16 : 8; }
+
+
 constexpr bool fft_len_supported(int L) {
   int n = L;
   while (n % 3 == 0) n /= 3;
@@ -387,6 +393,12 @@ inline std::vector<cplx<R>> default_twiddles(int L) {
   std::vector<cplx<R>> w(twiddle_table_len(L));
   build_pass_twiddles<R>(L, w.data());
   return w;
+}
+
+// Host: the twiddle table k_cols<R, L> reads.
+template <typename R>
+inline std::vector<cplx<R>> column_twiddles(int L) {
+  return kmajor_twiddles<R>(L, 64, cols_maxr(L, sizeof(R) == 4), false);
 }
 
 // Columns per workgroup of a strided (column) transform: 8 adjacent columns (128-byte

@@ -6,7 +6,7 @@ namespace sav {
 
 // Columns handled per workgroup for a column (strided) transform of length L.
 template <typename R, int L>
-using ColCfg = ColTile<R, L>;
+using ColCfg = ColTile<R, L, 8, cols_maxr(L, sizeof(R) == 4)>;
 
 // Rows handled per workgroup for a contiguous transform of length L.
 template <typename R, int L>
@@ -36,7 +36,7 @@ struct ColGeom {
 template <typename R, int L, bool INV>
 __global__ __launch_bounds__((ColCfg<R, L>::NT)) void k_cols(SpecJobs jobs, ColGeom cg,
                                                           const cplx<R>* __restrict__ tw, KTables kt) {
-  using F = TeamFFT<R, L, INV>;
+  using F = TeamFFT<R, L, INV, (sizeof(R) == 8 ? 128 : 64), 0, default_twm<R>(), 0, cols_maxr(L, sizeof(R) == 4)>;
   constexpr int C = ColCfg<R, L>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];

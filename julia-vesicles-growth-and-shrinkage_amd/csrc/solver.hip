@@ -159,8 +159,8 @@ class Solver final : public ISolver {
     g.tw_x = upload(default_twiddles<R>(Nx));
     g.tw_x2 = upload(twiddles<R>(Nx));
     g.tw_xh = upload(default_twiddles<R>(Nx / 2));
-    g.tw_y = upload(default_twiddles<R>(Ny));
-    g.tw_z = nd == 3 ? upload(default_twiddles<R>(Nz)) : g.tw_y;
+    g.tw_y = upload(column_twiddles<R>(Ny));
+    g.tw_z = nd == 3 ? upload(column_twiddles<R>(Nz)) : g.tw_y;
     g.k = KTables{d_kx, d_ky, d_kz, d_kx0};
     // states
     const long long nphys = (4LL * N + nd + 1) * P, nhalf = (4LL * N + nd + 1) * H;
