@@ -8,12 +8,14 @@ namespace sav {
 template <typename R, int L>
 using ColCfg = ColTile<R, L, 8, cols_maxr(L, sizeof(R) == 4)>;
 
-// Rows handled per workgroup for a contiguous transform of length L.
+// Rows handled per workgroup for a contiguous transform of length L: one wavefront per
+// workgroup (measured faster than four rows per 256-thread workgroup), or one row when
+// a row's team is larger than a wavefront.
 template <typename R, int L>
 struct RowCfg {
   using F = TeamFFT<R, L, false>;
   static constexpr int TT = F::TT;
-  static constexpr int ROWS = TT >= 256 ? 1 : 256 / TT;
+  static constexpr int ROWS = TT >= 64 ? 1 : 64 / TT;
   static constexpr int NT = ROWS * TT;
   static constexpr int LDSN = ROWS * (L + 1);
 };
