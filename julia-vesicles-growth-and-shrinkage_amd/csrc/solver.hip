@@ -91,6 +91,13 @@ class Solver final : public ISolver {
   double dv;  // dx*dy(*dz) in the reference's multiplication order is applied explicitly
   hipStream_t s = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Step 4 reads only the present/old state and the extrapolations of step 1's first
+  // kernels, so an eager step runs it on a second stream beside the psi solves (it fills
+  // the solves' single-workgroup finishes and launch gaps); its transforms use their own
+  // scratch (tmp4).  Joined before step 5.
+  hipStream_t s4 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool fork_step4 = false;
   FftGeom<R> g{};
   std::vector<void*> allocs;
   // tables
@@ -109,7 +116,7 @@ class Solver final : public ISolver {
   // work
   R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_s2, *w_du, *w_dmn, *w_conv,
       *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy;
-  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp;
+  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp, *tmp4;
   // Krylov
   KrylovCtx<R> kc{};
   KScal* hks = nullptr;
@@ -140,8 +147,11 @@ class Solver final : public ISolver {
     c.psi_out_v = hc.psi_out.data();
     SAV_HIP(hipSetDevice(c.device));
     SAV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    SAV_HIP(hipStreamCreateWithFlags(&s4, hipStreamNonBlocking));
     SAV_HIP(hipEventCreate(&ev0));
     SAV_HIP(hipEventCreate(&ev1));
+    SAV_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    SAV_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     // wavenumbers (SpectralUtils.jl:42-44; SpectralUtils_3d.jl)
     auto kx = rfftfreq_k(Nx, c.Lx), ky = fftfreq_k(Ny, c.Ly), kx0 = fftfreq_k(Nx, c.Lx);
     std::vector<double> kz = nd == 3 ? fftfreq_k(Nz, c.Lz) : std::vector<double>{0.0};
@@ -209,6 +219,9 @@ class Solver final : public ISolver {
     w_rhs2h = dalloc<C>(3 * H);
     w_uth = dalloc<C>(6 * H);
     tmp = dalloc<C>((16LL * N + 16) * H);
+    // step 4's transforms: nd^2 velocity derivatives + 2 nd N derivatives of mu, nu, and
+    // the 2 nd velocity predictors
+    tmp4 = dalloc<C>((long long)std::max(nd * nd + 2 * nd * N, 2 * nd) * H);
     // Krylov workspace (2D)
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks), 4 * sizeof(KScal), hipHostMallocDefault));
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hst), sizeof(DStateScal), hipHostMallocDefault));
@@ -266,6 +279,12 @@ class Solver final : public ISolver {
     for (auto& gr : graphs) hipGraphExecDestroy(gr.exec);
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    if (s4) {
+      hipStreamSynchronize(s4);
+      hipStreamDestroy(s4);
+    }
     if (s) hipStreamDestroy(s);
   }
 
@@ -997,6 +1016,22 @@ class Solver final : public ISolver {
       M[i] = T(0.1) * (T(1.0) - (T)M0 * (t * t));
     });
     foreach_n(s, (long long)nd * P, [=] __device__(long long i) { us[i] = T(2.0) * uN[i] - uO[i]; });
+    if (fork_step4) {
+      SAV_HIP(hipEventRecord(ev_fork, s));
+      SAV_HIP(hipStreamWaitEvent(s4, ev_fork, 0));
+      std::swap(s, s4);
+      std::swap(tmp, tmp4);
+      try {
+        step4();
+      } catch (...) {
+        std::swap(s, s4);
+        std::swap(tmp, tmp4);
+        throw;
+      }
+      std::swap(s, s4);
+      std::swap(tmp, tmp4);
+      SAV_HIP(hipEventRecord(ev_join, s4));
+    }
     if (nd == 2) {
       // M_max = maximum(M_psi) (Solvers.jl:27)
       max_reduce(M, NP, &ds->Mmax);
@@ -1947,10 +1982,17 @@ class Solver final : public ISolver {
       if (step_graph(monitor_on, st, rc)) return rc;
       set_bdf(a, b, cc, dt);  // psi iteration budget exceeded: repeat the step eagerly
     }
-    step1();
+    fork_step4 = true;
+    try {
+      step1();
+    } catch (...) {
+      fork_step4 = false;
+      throw;
+    }
+    fork_step4 = false;
     step2();
     step3();
-    step4();
+    SAV_HIP(hipStreamWaitEvent(s, ev_join, 0));
     step5();
     step67();
     // The back-transform and the monitor write only the new state's buffers, so they are
