@@ -91,12 +91,13 @@ class Solver final : public ISolver {
   double dv;  // dx*dy(*dz) in the reference's multiplication order is applied explicitly
   hipStream_t s = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // Step 4 reads only the present/old state and the extrapolations of step 1's first
-  // kernels, so an eager step runs it on a second stream beside the psi solves (it fills
-  // the solves' single-workgroup finishes and launch gaps); its transforms use their own
-  // scratch (tmp4).  Joined before step 5.
+  // Step 1's advection terms and step 4 read only the present/old state and the
+  // extrapolations of step 1's first kernels, so an eager step runs them on a second
+  // stream beside the functionals and the psi solves (they fill the solves'
+  // single-workgroup finishes and launch gaps); their transforms use their own scratch
+  // (tmp4, w_grad).  The psi solves wait for the advection terms, step 5 for step 4.
   hipStream_t s4 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_adv = nullptr, ev_join = nullptr;
   bool fork_step4 = false;
   FftGeom<R> g{};
   std::vector<void*> allocs;
@@ -115,7 +116,7 @@ class Solver final : public ISolver {
   double* red = nullptr;  // reduction outputs (512 doubles)
   // work
   R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_s2, *w_du, *w_dmn, *w_conv,
-      *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy;
+      *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy, *w_grad;
   C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp, *tmp4;
   // Krylov
   KrylovCtx<R> kc{};
@@ -151,6 +152,7 @@ class Solver final : public ISolver {
     SAV_HIP(hipEventCreate(&ev0));
     SAV_HIP(hipEventCreate(&ev1));
     SAV_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    SAV_HIP(hipEventCreateWithFlags(&ev_adv, hipEventDisableTiming));
     SAV_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     // wavenumbers (SpectralUtils.jl:42-44; SpectralUtils_3d.jl)
     auto kx = rfftfreq_k(Nx, c.Lx), ky = fftfreq_k(Ny, c.Ly), kx0 = fftfreq_k(Nx, c.Lx);
@@ -208,6 +210,7 @@ class Solver final : public ISolver {
     w_s5 = dalloc<R>(4 * NP);
     w_mon = dalloc<R>(6 * NP + 2 * P);
     w_gxy = dalloc<R>(2 * NP);
+    w_grad = dalloc<R>(2LL * nd * NP);
     w_phat = dalloc<C>(NH);
     w_omh = dalloc<C>(NH);
     w_Hh = dalloc<C>(4 * NH);
@@ -280,6 +283,7 @@ class Solver final : public ISolver {
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
     if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_adv) hipEventDestroy(ev_adv);
     if (ev_join) hipEventDestroy(ev_join);
     if (s4) {
       hipStreamSynchronize(s4);
@@ -999,6 +1003,59 @@ class Solver final : public ISolver {
     });
   }
 
+  // Step 1's advection terms (Solvers.jl:117-131): phi_21 = -rfft(u* . grad phi*) / lhs_phi,
+  // mu_21, and the right-hand side of psi_21, with grad phi*, grad psi* from the state spectra
+  // 2 X^n - X^{n-1}.  Reads only the state and u* (step 1's extrapolation); writes w_adv,
+  // w_advh, the 21 slots of the cache and the third psi right-hand side.
+  void step1_advection() {
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    const C *phhN = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
+    const R* us = w_us;
+    C* cch = cache;
+    const SpecC sc = spc();
+    const KIdx ki{Nxh, Ny, Nz};
+    const DScal* dsl = ds;
+    {
+      std::vector<SpecJob> jl;
+      for (int d = 0; d < nd; ++d) jl.push_back(job(phhN, w_grad + d * NP, N, DX(d), phhO));
+      for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, w_grad + (nd + d) * NP, N, DX(d), pshO));
+      irfft_v(jl);
+      R* adv = w_adv;
+      const R* gr = w_grad;
+      const int ndd = nd;
+      const long long Pl = P;
+      foreach_n(s, NP, [=] __device__(long long i) {
+        const long long sp = i % Pl;
+        R a = us[sp] * gr[i];
+        R b = us[sp] * gr[ndd * NP + i];
+        for (int d = 1; d < ndd; ++d) {
+          a = a + us[d * Pl + sp] * gr[d * NP + i];
+          b = b + us[d * Pl + sp] * gr[(ndd + d) * NP + i];
+        }
+        adv[i] = a;
+        adv[NP + i] = b;
+      });
+      rfft({job(w_adv, w_advh, 2 * N)});
+      const C* ah = w_advh;
+      C* rhs21 = w_rhs + 2 * NH;
+      foreach_n(s, NH, [=] __device__(long long i) {
+        int ix, iy, iz, f;
+        ki(i, ix, iy, iz, f);
+        const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double Lphi = sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3;
+        const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
+        const T Lp = (T)Lphi, lh = (T)lhs;
+        const CD t = ah[i];
+        const CD p21 = CD{(-t.x) / lh, (-t.y) / lh};
+        cch[PHI21 * NH + i] = p21;
+        cch[MU21 * NH + i] = CD{Lp * p21.x, Lp * p21.y};
+        const CD t2 = ah[NH + i];
+        rhs21[i] = CD{-t2.x, -t2.y};
+      });
+    }
+  }
+
   // ==========================================================================
   // Step 1 (Solvers.jl:14-137; Solvers_3d.jl:14-92)
   // ==========================================================================
@@ -1022,6 +1079,8 @@ class Solver final : public ISolver {
       std::swap(s, s4);
       std::swap(tmp, tmp4);
       try {
+        step1_advection();
+        SAV_HIP(hipEventRecord(ev_adv, s));
         step4();
       } catch (...) {
         std::swap(s, s4);
@@ -1113,46 +1172,11 @@ class Solver final : public ISolver {
         rhs12[i] = acc;
       });
     }
-    // 21: u* . grad(phi*), u* . grad(psi*) from state spectra (Solvers.jl:117-131)
-    {
-      std::vector<SpecJob> jl;
-      for (int d = 0; d < nd; ++d) jl.push_back(job(phhN, w_d4 + d * NP, N, DX(d), phhO));
-      for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, w_d4 + (nd + d) * NP, N, DX(d), pshO));
-      irfft_v(jl);
-      R* adv = w_adv;
-      const R* gr = w_d4;
-      const int ndd = nd;
-      const long long Pl = P;
-      foreach_n(s, NP, [=] __device__(long long i) {
-        const long long sp = i % Pl;
-        R a = us[sp] * gr[i];
-        R b = us[sp] * gr[ndd * NP + i];
-        for (int d = 1; d < ndd; ++d) {
-          a = a + us[d * Pl + sp] * gr[d * NP + i];
-          b = b + us[d * Pl + sp] * gr[(ndd + d) * NP + i];
-        }
-        adv[i] = a;
-        adv[NP + i] = b;
-      });
-      rfft({job(w_adv, w_advh, 2 * N)});
-      const C* ah = w_advh;
-      C* rhs21 = w_rhs + 2 * NH;
-      foreach_n(s, NH, [=] __device__(long long i) {
-        int ix, iy, iz, f;
-        ki(i, ix, iy, iz, f);
-        const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        const double Lphi = sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3;
-        const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
-        const T Lp = (T)Lphi, lh = (T)lhs;
-        const CD t = ah[i];
-        const CD p21 = CD{(-t.x) / lh, (-t.y) / lh};
-        cch[PHI21 * NH + i] = p21;
-        cch[MU21 * NH + i] = CD{Lp * p21.x, Lp * p21.y};
-        const CD t2 = ah[NH + i];
-        rhs21[i] = CD{-t2.x, -t2.y};
-      });
-    }
+    // 21: u* . grad(phi*), u* . grad(psi*) (forked with step 4 in an eager step)
+    if (fork_step4)
+      SAV_HIP(hipStreamWaitEvent(s, ev_adv, 0));
+    else
+      step1_advection();
     // psi solves
     if (nd == 2) {
       if (!use_learned && !collecting) {
