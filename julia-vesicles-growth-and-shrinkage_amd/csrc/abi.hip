@@ -12,6 +12,12 @@ struct sav_handle {
 
 static thread_local std::string g_create_error;
 
+namespace sav {
+void set_step_fork(sav_handle_t* h, bool on) {
+  if (h && h->s) h->s->set_step_fork(on);
+}
+}  // namespace sav
+
 #define SAV_GUARD(h, body)                                  \
   try {                                                     \
     body;                                                   \

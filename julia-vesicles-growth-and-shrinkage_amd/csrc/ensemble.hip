@@ -39,6 +39,7 @@ int sav_ensemble_create(const sav_config_t* cfgs, int n, sav_ensemble_t** out) {
       delete e;
       return rc;
     }
+    sav::set_step_fork(h, false);  // members overlap with each other: one stream each
     e->members.push_back(h);
     const long long pts = (long long)cfgs[i].Nx * cfgs[i].Ny * (cfgs[i].ndim == 3 ? cfgs[i].Nz : 1);
     e->work.push_back(pts * cfgs[i].N);

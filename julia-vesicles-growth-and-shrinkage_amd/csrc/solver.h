@@ -78,6 +78,9 @@ class ISolver {
   virtual int set_column_tile(int cw) = 0;
   virtual void column_tile(int* cw, double* ms8, double* ms16) = 0;
   virtual int set_graph(int on) = 0;
+  // Eager steps run step 1's advection terms and step 4 on a second stream (on by default);
+  // ensemble members, which overlap with each other, keep one stream each.
+  virtual void set_step_fork(bool on) = 0;
   virtual void graph_info(long long* replays, long long* repeats, long long* graphs) = 0;
   virtual int precond_train(int iters, int batch, double lr, uint64_t seed, double* loss_first,
                             double* loss_last) = 0;
@@ -87,5 +90,8 @@ class ISolver {
 };
 
 std::unique_ptr<ISolver> make_solver(const HostCfg& cfg);
+
+// Library-internal: the step fork of a handle's solver (see ISolver::set_step_fork).
+void set_step_fork(sav_handle_t* h, bool on);
 
 }  // namespace sav

@@ -98,7 +98,24 @@ class Solver final : public ISolver {
   // (tmp4, w_grad).  The psi solves wait for the advection terms, step 5 for step 4.
   hipStream_t s4 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_adv = nullptr, ev_join = nullptr;
-  bool fork_step4 = false;
+  bool fork_step4 = false, fork_enabled = true;
+  void set_step_fork(bool on) override { fork_enabled = on; }
+  // The second stream and its scratch are made at the first step that forks: HIP maps
+  // streams to hardware queues round-robin as they are created, so a stream made with
+  // every handle would leave the handles' main streams on half of the queues (ensemble
+  // members, which do not fork, would then serialise on them).
+  void ensure_fork() {
+    if (s4) return;
+    SAV_HIP(hipStreamCreateWithFlags(&s4, hipStreamNonBlocking));
+    SAV_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    SAV_HIP(hipEventCreateWithFlags(&ev_adv, hipEventDisableTiming));
+    SAV_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    // step 4's transforms: nd^2 velocity derivatives + 2 nd N derivatives of mu, nu, and
+    // the 2 nd velocity predictors; the advection terms: 2 nd N gradients
+    tmp4 = dalloc<C>((long long)std::max(nd * nd + 2 * nd * N, 2 * nd) * H);
+    w_grad = dalloc<R>(2LL * nd * N * P);
+    SAV_HIP(hipStreamSynchronize(s));
+  }
   FftGeom<R> g{};
   std::vector<void*> allocs;
   // tables
@@ -116,8 +133,11 @@ class Solver final : public ISolver {
   double* red = nullptr;  // reduction outputs (512 doubles)
   // work
   R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_s2, *w_du, *w_dmn, *w_conv,
-      *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy, *w_grad;
-  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp, *tmp4;
+      *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy;
+  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp;
+  // scratch of the forked branch (made by ensure_fork)
+  R* w_grad = nullptr;
+  C* tmp4 = nullptr;
   // Krylov
   KrylovCtx<R> kc{};
   KScal* hks = nullptr;
@@ -148,12 +168,8 @@ class Solver final : public ISolver {
     c.psi_out_v = hc.psi_out.data();
     SAV_HIP(hipSetDevice(c.device));
     SAV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    SAV_HIP(hipStreamCreateWithFlags(&s4, hipStreamNonBlocking));
     SAV_HIP(hipEventCreate(&ev0));
     SAV_HIP(hipEventCreate(&ev1));
-    SAV_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    SAV_HIP(hipEventCreateWithFlags(&ev_adv, hipEventDisableTiming));
-    SAV_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     // wavenumbers (SpectralUtils.jl:42-44; SpectralUtils_3d.jl)
     auto kx = rfftfreq_k(Nx, c.Lx), ky = fftfreq_k(Ny, c.Ly), kx0 = fftfreq_k(Nx, c.Lx);
     std::vector<double> kz = nd == 3 ? fftfreq_k(Nz, c.Lz) : std::vector<double>{0.0};
@@ -210,7 +226,6 @@ class Solver final : public ISolver {
     w_s5 = dalloc<R>(4 * NP);
     w_mon = dalloc<R>(6 * NP + 2 * P);
     w_gxy = dalloc<R>(2 * NP);
-    w_grad = dalloc<R>(2LL * nd * NP);
     w_phat = dalloc<C>(NH);
     w_omh = dalloc<C>(NH);
     w_Hh = dalloc<C>(4 * NH);
@@ -222,9 +237,6 @@ class Solver final : public ISolver {
     w_rhs2h = dalloc<C>(3 * H);
     w_uth = dalloc<C>(6 * H);
     tmp = dalloc<C>((16LL * N + 16) * H);
-    // step 4's transforms: nd^2 velocity derivatives + 2 nd N derivatives of mu, nu, and
-    // the 2 nd velocity predictors
-    tmp4 = dalloc<C>((long long)std::max(nd * nd + 2 * nd * N, 2 * nd) * H);
     // Krylov workspace (2D)
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks), 4 * sizeof(KScal), hipHostMallocDefault));
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hst), sizeof(DStateScal), hipHostMallocDefault));
@@ -1006,8 +1018,9 @@ class Solver final : public ISolver {
   // Step 1's advection terms (Solvers.jl:117-131): phi_21 = -rfft(u* . grad phi*) / lhs_phi,
   // mu_21, and the right-hand side of psi_21, with grad phi*, grad psi* from the state spectra
   // 2 X^n - X^{n-1}.  Reads only the state and u* (step 1's extrapolation); writes w_adv,
-  // w_advh, the 21 slots of the cache and the third psi right-hand side.
-  void step1_advection() {
+  // w_advh, the 21 slots of the cache and the third psi right-hand side.  `grad` holds the
+  // 2 nd N gradient fields (w_d4 in the serial order, where it is free at this point).
+  void step1_advection(R* grad) {
     const long long NP = (long long)N * P, NH = (long long)N * H;
     const C *phhN = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
     const R* us = w_us;
@@ -1017,11 +1030,11 @@ class Solver final : public ISolver {
     const DScal* dsl = ds;
     {
       std::vector<SpecJob> jl;
-      for (int d = 0; d < nd; ++d) jl.push_back(job(phhN, w_grad + d * NP, N, DX(d), phhO));
-      for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, w_grad + (nd + d) * NP, N, DX(d), pshO));
+      for (int d = 0; d < nd; ++d) jl.push_back(job(phhN, grad + d * NP, N, DX(d), phhO));
+      for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, grad + (nd + d) * NP, N, DX(d), pshO));
       irfft_v(jl);
       R* adv = w_adv;
-      const R* gr = w_grad;
+      const R* gr = grad;
       const int ndd = nd;
       const long long Pl = P;
       foreach_n(s, NP, [=] __device__(long long i) {
@@ -1079,7 +1092,7 @@ class Solver final : public ISolver {
       std::swap(s, s4);
       std::swap(tmp, tmp4);
       try {
-        step1_advection();
+        step1_advection(w_grad);
         SAV_HIP(hipEventRecord(ev_adv, s));
         step4();
       } catch (...) {
@@ -1176,7 +1189,7 @@ class Solver final : public ISolver {
     if (fork_step4)
       SAV_HIP(hipStreamWaitEvent(s, ev_adv, 0));
     else
-      step1_advection();
+      step1_advection(w_d4);
     // psi solves
     if (nd == 2) {
       if (!use_learned && !collecting) {
@@ -2006,7 +2019,9 @@ class Solver final : public ISolver {
       if (step_graph(monitor_on, st, rc)) return rc;
       set_bdf(a, b, cc, dt);  // psi iteration budget exceeded: repeat the step eagerly
     }
-    fork_step4 = true;
+    const bool forked = fork_enabled;
+    if (forked) ensure_fork();
+    fork_step4 = forked;
     try {
       step1();
     } catch (...) {
@@ -2016,7 +2031,10 @@ class Solver final : public ISolver {
     fork_step4 = false;
     step2();
     step3();
-    SAV_HIP(hipStreamWaitEvent(s, ev_join, 0));
+    if (forked)
+      SAV_HIP(hipStreamWaitEvent(s, ev_join, 0));
+    else
+      step4();
     step5();
     step67();
     // The back-transform and the monitor write only the new state's buffers, so they are
