@@ -69,8 +69,16 @@ __device__ __forceinline__ double pinv_diag(double a_dt, double Mmax, double S4,
 // ---------------------------------------------------------------------------
 // K1: column inverse FFTs of D1f_x*(S4 x) and D1f_y*(S4 x)
 // ---------------------------------------------------------------------------
-template <typename R, int L, int CW>
+// UPD selects the vector x the matvec is applied to, formed here from BiCGSTAB vectors as
+// each element is loaded (the vector update then needs no pass of its own):
+//   0: x = X;
+//   1: x = s = r - alpha v   (X = r, V = v; s is written to Xo);
+//   2: x = p = r + beta (p - omega v)   (X = r, V = v, Xo = p, updated in place).
+// The updates' arithmetic is that of k_kry_upd1s / k_kry_upd3.
+template <typename R, int L, int CW, int UPD>
 __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) void k_mv_cols_inv(const cplx<R>* __restrict__ X,
+                                                                    const cplx<R>* __restrict__ V,
+                                                                    cplx<R>* Xo,
                                                                     cplx<R>* __restrict__ Bx,
                                                                     cplx<R>* __restrict__ By, int Nx, double S4,
                                                                     const double* __restrict__ kx0,
@@ -85,6 +93,14 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) 
   X += z * vs;
   Bx += z * vs;
   By += z * vs;
+  if (UPD != 0) {
+    V += z * vs;
+    Xo += z * vs;
+  }
+  // UPD 1: -alpha; UPD 2: -omega and beta (rounded to R as in the separate update kernels)
+  const cplx<R> c1 = UPD == 1 ? cplx<R>{(R)-ks->ar, (R)-ks->ai} : cplx<R>{(R)-ks->wr, (R)-ks->wi};
+  const cplx<R> c2 = cplx<R>{(R)ks->br, (R)ks->bi};
+  const cplx<R> one = cplx<R>{R(1), R(0)};
   using F = KColFFT<R, L, true>;
   constexpr int C = KColCfg<R, L, CW>::C;
   constexpr int E = F::E;
@@ -100,7 +116,20 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) 
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int pos = F::in_pos(t, e);
-    cplx<R> a = act ? X[base + (long long)pos * Nx] : cplx<R>{R(0), R(0)};
+    const long long idx = base + (long long)pos * Nx;
+    cplx<R> a = cplx<R>{R(0), R(0)};
+    if (act) {
+      if (UPD == 0) {
+        a = X[idx];
+      } else if (UPD == 1) {
+        a = X[idx] + c1 * V[idx];
+        Xo[idx] = a;
+      } else {
+        const cplx<R> pa = Xo[idx] + c1 * V[idx];
+        a = one * X[idx] + c2 * pa;
+        Xo[idx] = a;
+      }
+    }
     b1[e] = cplx<R>{s4 * a.x, s4 * a.y};
     v[e] = cplx<R>{-kxv * b1[e].y, kxv * b1[e].x};
   }
@@ -438,8 +467,9 @@ __global__ __launch_bounds__(256) void k_kry_upd1(const cplx<R>* __restrict__ r,
   }
 }
 
-// s = r - alpha v (the iterate's update x += alpha p waits for k_kry_upd2, which reads
-// x once per iteration; `done` changes only after upd2, so both run or neither).
+// s = r - alpha v, in its own pass (FP32; FP64 forms s inside the matvec, see kry_iteration).
+// The iterate's update x += alpha p waits for k_kry_upd2, which reads x once per iteration;
+// `done` changes only after upd2, so both run or neither.
 template <typename R>
 __global__ __launch_bounds__(256) void k_kry_upd1s(const cplx<R>* __restrict__ r, const cplx<R>* __restrict__ v,
                                                    cplx<R>* __restrict__ s, long long n,
@@ -571,21 +601,38 @@ static void tev_collect(KrylovCtx<R>& k, const int* iters, int nsys) {
   k.ev_used = 0;
 }
 
+// upd: the vector update the kernel forms on load (k_mv_cols_inv's UPD, FP64 only); 1 and 2
+// take r and v from the context and write s (1) or p (2).
 template <typename R, int CW>
-static void launch_cols_inv(KrylovCtx<R>& k, const cplx<R>* y, const KScal* ks, int nsys, hipEvent_t e0,
+static void launch_cols_inv(KrylovCtx<R>& k, const cplx<R>* y, int upd, const KScal* ks, int nsys, hipEvent_t e0,
                             hipEvent_t e1) {
+  if (upd != 0 && sizeof(R) != 8) throw HipError("launch_cols_inv: the fused vector update is built for FP64 only", -1);
   const long long vs = k.vs;
+  const cplx<R>* xin = upd == 0 ? y : k.r;
+  cplx<R>* xo = upd == 1 ? k.s : upd == 2 ? k.p : nullptr;
+#define KL(L, U)                                                                                                  \
+  hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW, U>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, xin, k.v, xo, k.Bx, \
+                        k.By, k.Nx, k.S4, k.kx0, k.ky, k.tw_ci, ks, vs)
   switch (k.Ny) {
-#define X(L)                                                                                                     \
-  case L: {                                                                                                      \
-    using CC = KColCfg<R, L, CW>;                                                                                \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);                                                            \
-    hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, y, k.Bx, k.By, k.Nx, \
-                          k.S4, k.kx0, k.ky, k.tw_ci, ks, vs);                                                    \
-    break;                                                                                                       \
+#define X(L)                                                  \
+  case L: {                                                   \
+    using CC = KColCfg<R, L, CW>;                             \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);         \
+    if constexpr (sizeof(R) == 8) {                           \
+      if (upd == 1)                                           \
+        KL(L, 1);                                             \
+      else if (upd == 2)                                      \
+        KL(L, 2);                                             \
+      else                                                    \
+        KL(L, 0);                                             \
+    } else {                                                  \
+      KL(L, 0);                                               \
+    }                                                         \
+    break;                                                    \
   }
     SAV_KRY_SIZES(X)
 #undef X
+#undef KL
     default: throw HipError("unsupported Ny for the psi solve: " + std::to_string(k.Ny), -1);
   }
   SAV_LAUNCH_CHECK();
@@ -626,19 +673,21 @@ static int launch_cols_fwd(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int 
 
 // One matvec on nsys systems (vectors k.vs apart, KScal slots slot .. slot+nsys-1).
 // mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
+// upd: 0 = apply to y; 1 = y is s = r - alpha v, 2 = y is p = r + beta (p - omega v), formed
+// (and stored) by the first kernel (launch_cols_inv).
 template <typename R>
 static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
-                         const cplx<R>* partner = nullptr, int nsys = 1) {
+                         const cplx<R>* partner = nullptr, int nsys = 1, int upd = 0) {
   const KScal* ks = k.ks + slot;
   const long long vs = k.vs;
   const double sn = (double)sizeof(R) * (double)k.nfull;
   hipEvent_t e0, e1;
   tev_pre(k, 1, e0, e1);
   if (k.colw == 16)
-    launch_cols_inv<R, 16>(k, y, ks, nsys, e0, e1);
+    launch_cols_inv<R, 16>(k, y, upd, ks, nsys, e0, e1);
   else
-    launch_cols_inv<R, 8>(k, y, ks, nsys, e0, e1);
-  tev_post(k, e0, 6.0 * sn);
+    launch_cols_inv<R, 8>(k, y, upd, ks, nsys, e0, e1);
+  tev_post(k, e0, (upd == 0 ? 6.0 : upd == 1 ? 10.0 : 12.0) * sn);
   tev_pre(k, 2, e0, e1);
   const long long nrh = (long long)k.N * k.Ny;
   const double scale = 1.0 / ((double)k.Nx * k.Ny);
@@ -751,26 +800,33 @@ static void kry_start(KrylovCtx<R>& k, int slot, int nsys) {
 
 // One BiCGSTAB iteration of every system still running (a system whose `done` flag is set
 // skips every kernel), queued on k.st with no host synchronisation.
+// In FP64 the updates of s and of p run inside the first kernel of the matvec that consumes
+// them (launch_cols_inv's upd), so an iteration is two matvecs, one vector kernel and three
+// scalar finishes; `first` marks the iteration after kry_start, whose p is r as initialised.
+// In FP32 that kernel holds 32 elements per lane at two wavefronts per SIMD, too few to hide
+// the update's extra loads, and the separate update passes are faster (1024^2, two vesicles,
+// per launch over 2-3 systems: s 66 us fused against 32 + 24 us, p 88 against 32 + 27; in
+// FP64 99 against 62 + 48 and 113 against 62 + 55).
 template <typename R>
-static void kry_iteration(KrylovCtx<R>& k, int slot, int nsys) {
+static void kry_iteration(KrylovCtx<R>& k, int slot, int nsys, bool first) {
+  constexpr bool fuse = sizeof(R) == 8;
   const long long n = k.nfull;
   const int nbv = vec_blocks(n);
   const long long vs = k.vs, ps = k.ps;
   KScal* ks = k.ks + slot;
-  // v = P^{-1} A p ; alpha = rho / <c, v>
-  int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b, nsys);
+  // [p = r + beta (p - omega v)] ; v = P^{-1} A p ; alpha = rho / <c, v>
+  int nb3 = matvec_launch(k, k.p, k.v, 1, slot, k.b, nsys, (fuse && !first) ? 2 : 0);
   fin_launch(k, nb3, slot, 1, nsys);
-  // s = r - alpha v
-  hipLaunchKernelGGL((k_kry_upd1s<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.s, n, ks, vs);
-  // t = P^{-1} A s ; omega = <t, s> / <t, t>
-  nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s, nsys);
+  // s = r - alpha v ; t = P^{-1} A s ; omega = <t, s> / <t, t>
+  if (!fuse) hipLaunchKernelGGL((k_kry_upd1s<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.s, n, ks, vs);
+  nb3 = matvec_launch(k, k.s, k.t, 2, slot, k.s, nsys, fuse ? 1 : 0);
   fin_launch(k, nb3, slot, 2, nsys);
   // x += alpha p, x += omega s ; r = s - omega t ; <c, r>, ||r||
   hipLaunchKernelGGL((k_kry_upd2<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.s, k.t, k.b, k.p, k.x, k.r, n, ks,
                      k.partials, vs, ps);
   fin_launch(k, nbv, slot, 3, nsys);
-  // p = r + beta (p - omega v)
-  hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks, vs);
+  // p = r + beta (p - omega v), for the next iteration's matvec
+  if (!fuse) hipLaunchKernelGGL((k_kry_upd3<R>), dim3(nbv, nsys), dim3(256), 0, k.st, k.r, k.v, k.p, n, ks, vs);
   SAV_LAUNCH_CHECK();
 }
 
@@ -778,7 +834,7 @@ template <typename R>
 void bicgstab_queue(KrylovCtx<R>& k, int slot, int nsys, int iters) {
   if (k.colw == 0) throw HipError("bicgstab_queue: the column tile of the matvec is not chosen yet", -1);
   kry_start(k, slot, nsys);
-  for (int it = 0; it < iters && it < k.itmax; ++it) kry_iteration(k, slot, nsys);
+  for (int it = 0; it < iters && it < k.itmax; ++it) kry_iteration(k, slot, nsys, it == 0);
 }
 
 template <typename R>
@@ -797,7 +853,7 @@ int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
     const int nb = batch < k.itmax - queued ? batch : k.itmax - queued;
     for (int it = 0; it < nb; ++it) {
       k.cur_iter = queued + it + 1;
-      kry_iteration(k, slot, nsys);
+      kry_iteration(k, slot, nsys, queued + it == 0);
     }
     queued += nb;
     SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, nsys * sizeof(KScal), hipMemcpyDeviceToHost, k.st));
