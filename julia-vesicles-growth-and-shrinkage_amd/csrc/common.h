@@ -103,6 +103,31 @@ __device__ __forceinline__ void block_sum(double (&acc)[K], double* scratch) {
 
 template <typename R> __device__ __forceinline__ double to_d(R v) { return (double)v; }
 
+// Strided partial sums of a single-workgroup finish kernel: for k < K, acc[k] += p[i * K + k]
+// over i = t, t + NT, t + 2 NT, ... < nb, added in that order.  The loads are issued in
+// groups of 8 before any of them is added, so that they are in flight together: the
+// partials were written by the previous kernel on every XCD and come from beyond this
+// XCD's L2, and one load at a time made such a kernel latency-bound.
+template <int K, int NT>
+__device__ __forceinline__ void strided_sums(const double* __restrict__ p, int nb, int t, double (&acc)[K]) {
+  constexpr int U = 8;
+  for (int i0 = t; i0 < nb; i0 += U * NT) {
+    double v[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * NT;
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[u][k] = i < nb ? p[(long long)i * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * NT < nb) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += v[u][k];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // XCD-aware workgroup numbering.  The workgroups of a launch reach the 8 XCDs of the
 // chip round-robin by linear id, and each XCD has its own L2.  Tiles that share cache

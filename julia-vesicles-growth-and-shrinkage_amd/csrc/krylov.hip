@@ -355,19 +355,6 @@ __global__ __launch_bounds__(256) void k_kry_init(const cplx<R>* __restrict__ b,
   }
 }
 
-__device__ __forceinline__ void sum_partials(const double* partials, int nb, double* out3) {
-  // single-thread, fixed order
-  double a = 0, b = 0, c = 0;
-  for (int i = 0; i < nb; ++i) {
-    a += partials[i * 3 + 0];
-    b += partials[i * 3 + 1];
-    c += partials[i * 3 + 2];
-  }
-  out3[0] = a;
-  out3[1] = b;
-  out3[2] = c;
-}
-
 // Finalise a 3-wide partial array with one block (deterministic order).
 __global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ partials, int nb, KScal* __restrict__ ks,
                                                  int stage, double atol, double rtol, int itmax, long long ps) {
@@ -377,11 +364,7 @@ __global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ part
   partials += blockIdx.x * ps;
   if (stage != 0 && ks->done) return;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < nb; i += 256) {
-    acc[0] += partials[i * 3 + 0];
-    acc[1] += partials[i * 3 + 1];
-    acc[2] += partials[i * 3 + 2];
-  }
+  strided_sums<3, 256>(partials, nb, threadIdx.x, acc);
   block_sum<3>(acc, scratch);
   if (threadIdx.x != 0) return;
   KScal k = *ks;

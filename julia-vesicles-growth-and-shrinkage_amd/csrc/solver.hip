@@ -24,7 +24,19 @@ __global__ __launch_bounds__(256) void k_red_fin(const double* __restrict__ part
   __shared__ double sc[4];
   const int k = blockIdx.x, g = blockIdx.y;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) s += partials[((long long)g * nb + b) * K + k];
+  {
+    // partials of group g: [b * K + k], b < nb; summed in the order b = t, t + 256, ...
+    const double* pg = partials + (long long)g * nb * K + k;
+    constexpr int U = 8;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += U * 256) {
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = b0 + u * 256 < nb ? pg[(long long)(b0 + u * 256) * K] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (b0 + u * 256 < nb) s += v[u];
+    }
+  }
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = s;
   __syncthreads();
