@@ -139,7 +139,7 @@ constexpr unsigned kNumXcd = 8;
 
 // Bijection on [0, total): the ids one XCD receives (lin = x, x + 8, x + 16, ...) map to a
 // contiguous range of logical ids, in dispatch order.
-__device__ __forceinline__ unsigned xcd_spread(unsigned lin, unsigned total) {
+__host__ __device__ __forceinline__ unsigned xcd_spread(unsigned lin, unsigned total) {
   const unsigned q = total / kNumXcd, r = total % kNumXcd, x = lin % kNumXcd, k = lin / kNumXcd;
   return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
@@ -148,16 +148,20 @@ struct BlockIdx3 {
   int x, y, z;
 };
 
-// Logical block index of the calling workgroup.  With a multiple of 8 blocks along x the
-// renumbering stays within each (y, z) row of tiles (every row keeps all 8 XCDs, so rows
-// that exit early, such as converged systems of a batched solve, idle no XCD); otherwise
-// it runs over the whole grid.
-__device__ __forceinline__ BlockIdx3 xcd_block() {
-  const unsigned gx = gridDim.x, gy = gridDim.y;
-  if (gx % kNumXcd == 0) return BlockIdx3{(int)xcd_spread(blockIdx.x, gx), (int)blockIdx.y, (int)blockIdx.z};
-  const unsigned total = gx * gy * gridDim.z;
-  const unsigned l = xcd_spread(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), total);
+// Logical block index of block (bx, by, bz) in a (gx, gy, gz) grid.  With a multiple of 8
+// blocks along x the renumbering stays within each (y, z) row of tiles (every row keeps all
+// 8 XCDs, so rows that exit early, such as converged systems of a batched solve, idle no
+// XCD); otherwise it runs over the whole grid.
+__host__ __device__ __forceinline__ BlockIdx3 xcd_map(unsigned bx, unsigned by, unsigned bz, unsigned gx, unsigned gy,
+                                                      unsigned gz) {
+  if (gx % kNumXcd == 0) return BlockIdx3{(int)xcd_spread(bx, gx), (int)by, (int)bz};
+  const unsigned l = xcd_spread(bx + gx * (by + gy * bz), gx * gy * gz);
   return BlockIdx3{(int)(l % gx), (int)((l / gx) % gy), (int)(l / (gx * gy))};
+}
+
+// Logical block index of the calling workgroup.
+__device__ __forceinline__ BlockIdx3 xcd_block() {
+  return xcd_map(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.y, gridDim.z);
 }
 
 }  // namespace sav
