@@ -8,7 +8,7 @@ from .config import (Config, BDFCoeff, bdf_coeff, set_para_base, stabilisers, Sp
 from .sav import (FieldState, Operators, Ensemble, config_c, build_operators, to_spectral, to_physical, update_state,
                   generate_initial_condition, state_from_physical, run_simulation, time_nodes, load_library,
                   save_steps, snapshot_name, write_snapshot, read_snapshot, write_checkpoint, read_checkpoint,
-                  resume_simulation, SNAPSHOT_KEYS, initial_fields_3d,
+                  resume_simulation, SNAPSHOT_KEYS, initial_fields_3d, initial_fields_2d,
                   SavError, WCNegativeError, EXPORTS, LIB_PATH,
                   SAV_OK, SAV_W_KRYLOV_NOT_CONVERGED, SAV_W_THETA2_NONNEG, SAV_E_ARG, SAV_E_HIP,
                   SAV_E_WC_NEGATIVE, SAV_E_NONFINITE, SAV_E_UNSUPPORTED, SAV_PREC_FP64, SAV_PREC_FP32,

@@ -117,12 +117,17 @@ int sav_init_sav(sav_handle_t* h) {
   return rc;
 }
 
+// Error text of a step status shared by sav_step and sav_run (and so the ensemble).
+static void set_step_error(sav_handle_t* h, int rc) {
+  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0; step not committed";
+  if (rc == SAV_E_NONFINITE) h->err = "non-finite SAV scalars; step not committed";
+}
+
 int sav_step(sav_handle_t* h, double dt, int n, sav_step_stats_t* stats) {
   if (!h || n < 1 || !(dt > 0)) return SAV_E_ARG;
   int rc = 0;
   SAV_GUARD(h, rc = h->s->step(dt, n, stats));
-  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0; step not committed";
-  if (rc == SAV_E_NONFINITE) h->err = "non-finite SAV scalars; step not committed";
+  set_step_error(h, rc);
   return rc;
 }
 
@@ -131,7 +136,7 @@ int sav_run(sav_handle_t* h, const double* t_nodes, int n_nodes, int n0, double*
   if (!h || !t_nodes || n_nodes < 1 || n0 < 1) return SAV_E_ARG;
   int rc = 0;
   SAV_GUARD(h, rc = h->s->run(t_nodes, n_nodes, n0, energy_hist, area_hist, krylov_iters_total));
-  if (rc == SAV_E_WC_NEGATIVE) h->err = "FATAL: W + C < 0; step not committed";
+  set_step_error(h, rc);
   return rc;
 }
 

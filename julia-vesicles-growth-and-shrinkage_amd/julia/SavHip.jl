@@ -64,10 +64,17 @@ end
 
 destroy(h) = ccall((:sav_destroy, LIB), Cint, (Ptr{Cvoid},), h)
 
+"""Vesicle count N of a FieldState: the last dimension of phi, (Nx, Ny, N) or (Nx, Ny, Nz, N)."""
+nvesicles(s) = size(s.phi, ndims(s.phi))
+
 """Copies a FieldState (src/Types.jl:167-201) into the handle (which = SAV_PRESENT/SAV_OLD)."""
 function set_state!(h, which, s)
     sc = Float64[s.Q, s.R1, s.R2, s.R3]
-    A0 = collect(Float64, s.A0)
+    # the library reads N values of A0; the reference's FieldState holds A0 = [0.0]
+    # (Init.jl:215, Types.jl:228) until the SAV initialisation, so a short vector is padded
+    N = nvesicles(s)
+    A0 = zeros(N)
+    copyto!(A0, 1, s.A0, 1, min(N, length(s.A0)))
     GC.@preserve s sc A0 begin
         rc = ccall((:sav_set_state, LIB), Cint,
                    (Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{ComplexF64}, Ptr{Float64}, Ptr{ComplexF64},
@@ -84,7 +91,7 @@ end
 """Copies the handle's state into `s` (same shapes as FieldState(Nx, Ny, N))."""
 function get_state!(h, which, s)
     sc = zeros(4)
-    A0 = zeros(length(s.A0))
+    A0 = zeros(nvesicles(s))  # the library writes N values
     dt = Ref{Float64}(0.0)
     GC.@preserve s sc A0 begin
         rc = ccall((:sav_get_state, LIB), Cint,

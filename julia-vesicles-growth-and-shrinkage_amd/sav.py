@@ -580,23 +580,25 @@ def initial_fields_3d(conf: Config, state_type: int):
     z = np.linspace(0, Lz - Lz / Nz, Nz)[None, None, :]
     s2e = math.sqrt(2) * eps
     phi = np.zeros((Nx, Ny, Nz, N), order="F")
+    # Init_3d.jl:96-123: the branches that the reference leaves without a shape (N = 1 with
+    # state_type <= 6 other than 1, N = 2 with state_type >= 8, N >= 3) keep phi = 0
     if N == 1:
-        if state_type != 1:
-            raise ValueError(f"state_type {state_type} has no 3D shape for N = 1 in the reference")
-        R = 0.2 * Lx
-        dist = np.sqrt((x - Lx / 2) ** 2 / 2 + (y - Ly / 2) ** 2 / 2 + (z - Lz / 2) ** 2)
-        phi[..., 0] = np.tanh((R - dist) / s2e)
+        if state_type > 6:
+            raise ValueError(f"state_type {state_type} invalid for N = 1 (Init_3d.jl:98)")
+        if state_type == 1:
+            R = 0.2 * Lx
+            dist = np.sqrt((x - Lx / 2) ** 2 / 2 + (y - Ly / 2) ** 2 / 2 + (z - Lz / 2) ** 2)
+            phi[..., 0] = np.tanh((R - dist) / s2e)
     elif N == 2:
-        if state_type != 7:
-            raise ValueError(f"state_type {state_type} invalid for N = 2 (the reference defines 7)")
-        if conf.goal == "s":
-            raise ValueError("state_type 7 needs goal g (Init_3d.jl)")
-        for n, cxn in enumerate((0.73, 1.27)):
-            r = np.sqrt((x - cxn) ** 2 + (y - Ly / 2) ** 2 + (z - Lz / 2) ** 2)
-            th = np.arctan2(x - cxn, y - Ly / 2)
-            phi[..., n] = np.tanh((0.2 + 0.02 * np.cos(10 * th) - r) / s2e)
-    else:
-        raise ValueError(f"N = {N} has no 3D shape in the reference")
+        if state_type <= 6:
+            raise ValueError(f"state_type {state_type} invalid for N = 2 (Init_3d.jl:108)")
+        if state_type == 7:
+            if conf.goal == "s":
+                raise ValueError("state_type 7 needs goal g (Init_3d.jl:114)")
+            for n, cxn in enumerate((0.73, 1.27)):
+                r = np.sqrt((x - cxn) ** 2 + (y - Ly / 2) ** 2 + (z - Lz / 2) ** 2)
+                th = np.arctan2(x - cxn, y - Ly / 2)
+                phi[..., n] = np.tanh((0.2 + 0.02 * np.cos(10 * th) - r) / s2e)
     psi = np.zeros_like(phi, order="F")
     for n in range(N):
         psi[..., n] = (-0.1 * phi[..., n] + 0.7) if conf.goal == "s" else (-0.35 * phi[..., n] + 0.45)
@@ -604,10 +606,18 @@ def initial_fields_3d(conf: Config, state_type: int):
 
 
 def generate_initial_condition(conf: Config, ops: Optional[Operators], state_type: int, seed: int = 1234):
-    """generate_initial_condition of Init.jl:74-217 (2D) and Init_3d.jl:67-153 (3D)."""
+    """generate_initial_condition of Init.jl:74-217 (2D) and Init_3d.jl:67-153 (3D): the
+    physical fields of initial_fields_2d / initial_fields_3d and their spectra."""
     if conf.ndim == 3:
         phi, psi = initial_fields_3d(conf, state_type)
-        return state_from_physical(conf, ops, phi, psi)
+    else:
+        phi, psi = initial_fields_2d(conf, state_type, seed)
+    return state_from_physical(conf, ops, phi, psi)
+
+
+def initial_fields_2d(conf: Config, state_type: int, seed: int = 1234):
+    """(phi, psi) in Julia shape (Nx, Ny, N) of Init.jl:74-198; state_type 4 draws its
+    mode amplitudes and phases from SplitMix64(seed) in the order of Init.jl:151-152."""
     Nx, Ny, N = conf.Nx, conf.Ny, conf.N
     Lx, Ly, eps = conf.Lx, conf.Ly, conf.epsilon
     x = np.linspace(0, Lx - conf.dx, Nx)
@@ -618,14 +628,19 @@ def generate_initial_condition(conf: Config, ops: Optional[Operators], state_typ
     if N == 1:
         cx, cy = [Lx / 2], [Ly / 2]
     elif N == 2:
+        if conf.goal not in ("g", "s"):
+            raise ValueError(f"what's the goal? now it is {conf.goal} (Init.jl:91)")
         cx = [0.75, 1.25] if conf.goal == "g" else [0.65, 1.35]
         cy = [Ly / 2, Ly / 2]
     else:
         cx = [Lx / 4, Lx / 2, 3 * Lx / 4]
         cy = [Ly / 2] * 3
+    # Init.jl:108-187: N = 1 raises for state_type > 6 and leaves phi = 0 for the types
+    # without a shape (<= 0, 5, 6); N >= 2 raises for state_type <= 6, for 7 with goal s and
+    # 8 with goal g, and leaves phi = 0 for state_type > 8
     if N == 1:
         if state_type > 6:
-            raise ValueError(f"state_type {state_type} invalid for N = 1")
+            raise ValueError(f"state_type {state_type} invalid for N = 1 (Init.jl:109)")
         cxn, cyn = cx[0], cy[0]
         if state_type == 1:
             R = 0.2 * Lx
@@ -655,12 +670,16 @@ def generate_initial_condition(conf: Config, ops: Optional[Operators], state_typ
                 pn = rng.uniform() * 2 * math.pi
                 Rt = Rt + an * R0 * np.cos(k * th + pn)
             phi[:, :, 0] = np.tanh((Rt - r) / s2e)
-        else:
-            raise ValueError(f"state_type {state_type} has no shape in the reference")
     else:
         if state_type <= 6:
-            raise ValueError(f"state_type {state_type} invalid for N = {N}")
-        for n in range(N):
+            raise ValueError(f"state_type {state_type} invalid for N = {N} (Init.jl:157)")
+        if state_type == 7 and conf.goal == "s":
+            raise ValueError("state_type 7 needs goal g (Init.jl:163)")
+        if state_type == 8 and conf.goal == "g":
+            raise ValueError("state_type 8 needs goal s (Init.jl:174)")
+        if N > 3:
+            raise ValueError(f"N = {N}: Init.jl defines vesicle centres for N <= 3 only (Init.jl:86-100)")
+        for n in range(N if state_type in (7, 8) else 0):
             R0, amp = (0.2, 0.02) if state_type == 7 else (0.3, 0.01)
             r = np.sqrt((X - cx[n]) ** 2 + (Y - cy[n]) ** 2)
             th = np.arctan2(X - cx[n], Y - cy[n])
@@ -668,7 +687,7 @@ def generate_initial_condition(conf: Config, ops: Optional[Operators], state_typ
     psi = np.zeros_like(phi, order="F")
     for n in range(N):
         psi[:, :, n] = (-0.1 * phi[:, :, n] + 0.7) if conf.goal == "s" else (-0.35 * phi[:, :, n] + 0.45)
-    return state_from_physical(conf, ops, phi, psi)
+    return phi, psi
 
 
 def state_from_physical(conf: Config, ops: Optional[Operators], phi: np.ndarray, psi: np.ndarray) -> FieldState:

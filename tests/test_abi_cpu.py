@@ -101,3 +101,18 @@ def test_splitmix64_known_values(jv):
     r = jv.SplitMix64(0)
     assert r.next_u64() == 0xE220A8397B1DCDAF
     assert r.next_u64() == 0x6E789E6AA1B965F4
+
+
+def test_julia_shim_sizes_A0_by_the_vesicle_count():
+    # the library copies N values of A0 in and out (sav_set_state / sav_get_state), while the
+    # reference's FieldState holds A0 = [0.0] (Types.jl:228): the binding must size its
+    # buffers by N, never by length(s.A0)
+    src = (ROOT / "julia-vesicles-growth-and-shrinkage_amd" / "julia" / "SavHip.jl").read_text()
+    body = lambda name: src.split(f"function {name}(")[1].split("\nend\n")[0]
+    for fn in ("set_state!", "get_state!"):
+        b = body(fn)
+        assert "nvesicles(s)" in b, fn
+        assert "length(s.A0))" not in b.replace("min(N, length(s.A0))", ""), fn
+    assert "nvesicles(s) = size(s.phi, ndims(s.phi))" in src
+    doc = (ROOT / "INTEGRATION.md").read_text()
+    assert "sc, s.A0, s.dt)" not in doc
