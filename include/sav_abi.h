@@ -38,6 +38,11 @@ extern "C" {
 
 #define SAV_PREC_FP64 0
 #define SAV_PREC_FP32 1 /* FP32 fields/FFTs, FP64 reductions, 3x3 solve and Krylov scalars */
+/* FP64 fields and every stage in FP64 except the 2D psi BiCGSTAB's inner iterations: an
+ * FP32 BiCGSTAB on the residual equation inside an FP64 defect correction that stops on the
+ * reference's preconditioned residual test (||P(b - A x)|| <= atol + tol ||P b||,
+ * src/Solvers.jl:73-75) taken on the FP64 true residual.  3D steps are those of FP64. */
+#define SAV_PREC_MIXED 2
 
 #define SAV_PRECOND_DIAG 0
 #define SAV_PRECOND_LEARNED 1

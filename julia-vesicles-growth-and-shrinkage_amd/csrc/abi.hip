@@ -45,7 +45,7 @@ int sav_create(const sav_config_t* cfg, sav_handle_t** out) {
       throw std::invalid_argument("grid sizes must be 2^a 3^b, Nx even, Nx/2, Ny, Nz <= 1024");
     if (cfg->ndim == 2 && cfg->Ny < 4) throw std::invalid_argument("Ny must be >= 4");
     if (!cfg->psi_in_v || !cfg->psi_out_v) throw std::invalid_argument("psi_in_v/psi_out_v required");
-    if (cfg->precision != SAV_PREC_FP64 && cfg->precision != SAV_PREC_FP32)
+    if (cfg->precision != SAV_PREC_FP64 && cfg->precision != SAV_PREC_FP32 && cfg->precision != SAV_PREC_MIXED)
       throw std::invalid_argument("unknown precision");
     if (cfg->precond != SAV_PRECOND_DIAG && cfg->precond != SAV_PRECOND_LEARNED)
       throw std::invalid_argument("unknown preconditioner");

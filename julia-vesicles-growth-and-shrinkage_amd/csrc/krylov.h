@@ -52,6 +52,9 @@ struct KrylovCtx {
   // preconditioned directions of the flexible solver
   cplx<R>* Bp = nullptr;
   cplx<R>* Bs = nullptr;
+  // inner solve of a mixed-precision solve: system z is skipped (done, zero right-hand
+  // side) when outer[slot + z] is done
+  const KScal* outer = nullptr;
 };
 
 // Solves A x = b with b already in ctx.b; x in ctx.x on return.  Returns the
@@ -78,6 +81,16 @@ void matvec_prec_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot);
 // out = A y without the preconditioner (queued, no synchronisation).
 template <typename R>
 void matvec_plain_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot);
+
+// Mixed-precision solve of nsys systems (SAV_PREC_MIXED): FP64 iterate k.x, right-hand
+// side k.b and residual, FP32 inner BiCGSTAB (k32, its own rtol and itmax) on the residual
+// equation A e = b - A x, x += e, until the diagonally preconditioned FP64 residual
+// ||P^{-1}(b - A x)|| <= k.atol + k.rtol ||P^{-1} b|| (the reference's stopping test,
+// Solvers.jl:73-75, taken on the true residual) or max_rounds rounds.  k32 shares the
+// operator tables, k.ds and the partials buffer; k32.M holds M_psi in FP32.  k.hks[slot+z]
+// returns: iter = inner iterations of all rounds, rounds, rnorm = final true residual,
+// solved.  Returns the largest inner iteration total.
+int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int nsys, int max_rounds);
 
 // Flexible right-preconditioned BiCGSTAB with preconditioner `prec` (see krylov.hip).
 template <typename R>

@@ -357,12 +357,24 @@ __global__ __launch_bounds__(256) void k_kry_init(const cplx<R>* __restrict__ b,
 
 // Finalise a 3-wide partial array with one block (deterministic order).
 __global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ partials, int nb, KScal* __restrict__ ks,
-                                                 int stage, double atol, double rtol, int itmax, long long ps) {
+                                                 int stage, double atol, double rtol, int itmax, long long ps,
+                                                 const KScal* __restrict__ outer) {
   __shared__ double scratch[3 * 4];
   // one block per system of a batched solve
   ks += blockIdx.x;
   partials += blockIdx.x * ps;
   if (stage != 0 && ks->done) return;
+  if (stage == 0 && outer != nullptr && outer[blockIdx.x].done) {
+    // inner solve of a mixed-precision system whose outer solve has converged: nothing to do
+    if (threadIdx.x == 0) {
+      KScal k{};
+      k.done = 1;
+      k.solved = 1;
+      k.status = 4;
+      *ks = k;
+    }
+    return;
+  }
   double acc[3] = {0.0, 0.0, 0.0};
   strided_sums<3, 256>(partials, nb, threadIdx.x, acc);
   block_sum<3>(acc, scratch);
@@ -765,7 +777,7 @@ void matvec_plain_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot) 
 template <typename R>
 static void fin_launch(KrylovCtx<R>& k, int nb, int slot, int stage, int nsys) {
   hipLaunchKernelGGL(k_kry_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nb, k.ks + slot, stage, k.atol, k.rtol,
-                     k.itmax, k.ps);
+                     k.itmax, k.ps, k.outer ? k.outer + slot : nullptr);
 }
 
 // x = 0, r = p = P^{-1} b, rho, the stopping threshold: KScal slots slot .. slot + nsys - 1
@@ -866,6 +878,174 @@ int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
 template <typename R>
 int bicgstab_2d(KrylovCtx<R>& k, int slot) {
   return bicgstab_batch(k, slot, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solve: FP64 defect correction around an FP32 inner BiCGSTAB
+// ---------------------------------------------------------------------------
+// Round j: the inner solve reduces the preconditioned residual of A e = r_j by its own
+// rtol in FP32, then x += e and r_{j+1} = b - A x in FP64.  Only e, whose size is that of
+// the residual, is ever rounded to FP32, so the solution's accuracy is set by the FP64
+// residual test, not by FP32 rounding of x.
+
+// x = 0, r32 = b (FP32), partial ||P^{-1} b||^2.
+__global__ __launch_bounds__(256) void k_ref_start(const cplx<double>* __restrict__ b, cplx<double>* __restrict__ x,
+                                                   cplx<float>* __restrict__ r32, long long n, int Nx, int Ny,
+                                                   double S4, const double* __restrict__ kx0,
+                                                   const double* __restrict__ ky, const DScal* __restrict__ ds,
+                                                   double* __restrict__ partials, long long vs, long long ps) {
+  __shared__ double scratch[3 * 4];
+  const int z = blockIdx.y;
+  b += z * vs;
+  x += z * vs;
+  r32 += z * vs;
+  partials += z * ps;
+  const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const int ix = (int)(i % Nx);
+    const int iy = (int)((i / Nx) % Ny);
+    const double pv = pinv_diag(a_dt, Mmax, S4, kx0[ix], ky[iy]);
+    const cplx<double> bb = b[i];
+    x[i] = cplx<double>{0.0, 0.0};
+    r32[i] = cplx<float>{(float)bb.x, (float)bb.y};
+    acc[0] += pv * pv * (bb.x * bb.x + bb.y * bb.y);
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 3 + 0] = acc[0];
+    partials[blockIdx.x * 3 + 1] = acc[1];
+    partials[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+// x += e (the inner solution) for the systems still iterating.
+__global__ __launch_bounds__(256) void k_ref_update(cplx<double>* __restrict__ x, const cplx<float>* __restrict__ e,
+                                                    long long n, const KScal* __restrict__ ks, long long vs) {
+  const int z = blockIdx.y;
+  if (ks[z].done) return;
+  x += z * vs;
+  e += z * vs;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const cplx<float> d = e[i];
+    const cplx<double> xi = x[i];
+    x[i] = cplx<double>{xi.x + (double)d.x, xi.y + (double)d.y};
+  }
+}
+
+// r = b - Ax (ax = A x, unpreconditioned), r32 = r in FP32, partial ||P^{-1} r||^2.
+__global__ __launch_bounds__(256) void k_ref_resid(const cplx<double>* __restrict__ b,
+                                                   const cplx<double>* __restrict__ ax, cplx<float>* __restrict__ r32,
+                                                   long long n, int Nx, int Ny, double S4,
+                                                   const double* __restrict__ kx0, const double* __restrict__ ky,
+                                                   const DScal* __restrict__ ds, const KScal* __restrict__ ks,
+                                                   double* __restrict__ partials, long long vs, long long ps) {
+  __shared__ double scratch[3 * 4];
+  const int z = blockIdx.y;
+  if (ks[z].done) return;
+  b += z * vs;
+  ax += z * vs;
+  r32 += z * vs;
+  partials += z * ps;
+  const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const int ix = (int)(i % Nx);
+    const int iy = (int)((i / Nx) % Ny);
+    const double pv = pinv_diag(a_dt, Mmax, S4, kx0[ix], ky[iy]);
+    const cplx<double> bb = b[i], aa = ax[i];
+    const double rx = bb.x - aa.x, ry = bb.y - aa.y;
+    r32[i] = cplx<float>{(float)rx, (float)ry};
+    acc[0] += pv * pv * (rx * rx + ry * ry);
+  }
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 3 + 0] = acc[0];
+    partials[blockIdx.x * 3 + 1] = acc[1];
+    partials[blockIdx.x * 3 + 2] = acc[2];
+  }
+}
+
+// Outer scalars, one block per system.  stage 0: ||P^{-1} b||, the threshold
+// atol + rtol ||P^{-1} b||; stage 1 (after a round): the true residual norm, the inner
+// iterations of the round, the stopping tests.
+__global__ __launch_bounds__(256) void k_ref_fin(const double* __restrict__ partials, int nb, KScal* __restrict__ ks,
+                                                 const KScal* __restrict__ inner, int stage, double atol, double rtol,
+                                                 int max_rounds, long long ps) {
+  __shared__ double scratch[3 * 4];
+  ks += blockIdx.x;
+  partials += blockIdx.x * ps;
+  if (stage != 0 && ks->done) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  strided_sums<3, 256>(partials, nb, threadIdx.x, acc);
+  block_sum<3>(acc, scratch);
+  if (threadIdx.x != 0) return;
+  KScal k = *ks;
+  const double rn = sqrt(acc[0]);
+  if (stage == 0) {
+    k = KScal{};
+    k.rnorm = rn;
+    k.eps = atol + rtol * rn;
+    if (rn == 0.0) {
+      k.done = 1;
+      k.solved = 1;
+      k.status = 4;
+    } else if (rn <= k.eps) {
+      k.done = 1;
+      k.solved = 1;
+      k.status = 1;
+    }
+  } else {
+    k.rounds += 1;
+    k.iter += inner[blockIdx.x].iter;
+    k.rnorm = rn;
+    if (rn <= k.eps) {
+      k.done = 1;
+      k.solved = 1;
+      k.status = 1;
+    } else if (k.rounds >= max_rounds || rn != rn) {
+      k.done = 1;
+      k.status = 2;
+    }
+  }
+  *ks = k;
+}
+
+int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int nsys, int max_rounds) {
+  if (k32.vs != k.vs || k32.nfull != k.nfull || k32.st != k.st)
+    throw HipError("bicgstab_refined: the FP32 context does not match the FP64 one", -1);
+  autotune_columns(k);
+  autotune_columns(k32);
+  const long long n = k.nfull, vs = k.vs, ps = k.ps;
+  const int nbv = vec_blocks(n);
+  KScal* ks = k.ks + slot;
+  k32.outer = k.ks;
+  hipLaunchKernelGGL(k_ref_start, dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.x, k32.b, n, k.Nx, k.Ny, k.S4, k.kx0,
+                     k.ky, k.ds, k.partials, vs, ps);
+  hipLaunchKernelGGL(k_ref_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nbv, ks, k32.ks + slot, 0, k.atol,
+                     k.rtol, max_rounds, ps);
+  SAV_LAUNCH_CHECK();
+  for (int round = 0; round < max_rounds; ++round) {
+    // inner FP32 solve of A e = r on the systems still iterating (the others start done)
+    bicgstab_batch<float>(k32, slot, nsys);
+    hipLaunchKernelGGL(k_ref_update, dim3(nbv, nsys), dim3(256), 0, k.st, k.x, k32.x, n, ks, vs);
+    // r = b - A x in FP64: A x into k.t (unpreconditioned, no dots), skipping converged systems
+    matvec_launch<double>(k, k.x, k.t, 4, slot, nullptr, nsys);
+    hipLaunchKernelGGL(k_ref_resid, dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.t, k32.b, n, k.Nx, k.Ny, k.S4,
+                       k.kx0, k.ky, k.ds, ks, k.partials, vs, ps);
+    hipLaunchKernelGGL(k_ref_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nbv, ks, k32.ks + slot, 1, k.atol,
+                       k.rtol, max_rounds, ps);
+    SAV_LAUNCH_CHECK();
+    SAV_HIP(hipMemcpyAsync(k.hks + slot, ks, nsys * sizeof(KScal), hipMemcpyDeviceToHost, k.st));
+    SAV_HIP(hipStreamSynchronize(k.st));
+    bool all_done = true;
+    for (int z = 0; z < nsys; ++z) all_done = all_done && k.hks[slot + z].done;
+    if (all_done) break;
+  }
+  k32.outer = nullptr;
+  int maxit = 0;
+  for (int z = 0; z < nsys; ++z) maxit = std::max(maxit, k.hks[slot + z].iter);
+  return maxit;
 }
 
 // Flexible (right-preconditioned) BiCGSTAB with a possibly nonlinear preconditioner

@@ -41,6 +41,8 @@ struct KScal {
   double br, bi;  // beta
   double rnorm, eps;
   int iter, done, solved, status;  // status: 0 running, 1 solved, 2 itmax, 3 breakdown, 4 zero rhs, 5 bc=0
+  int rounds;                      // defect-correction rounds of a mixed-precision solve
+  int pad;
 };
 
 struct HostCfg {

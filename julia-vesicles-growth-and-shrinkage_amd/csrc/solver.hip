@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "krylov.h"
 #include "launch.h"
@@ -152,6 +153,17 @@ class Solver final : public ISolver {
   C* tmp4 = nullptr;
   // Krylov
   KrylovCtx<R> kc{};
+  // SAV_PREC_MIXED (2D; FP64 storage): every stage in FP64 except the psi BiCGSTAB's inner
+  // iterations, which run in FP32 on the residual equation of an FP64 defect correction
+  // (bicgstab_refined); the FP64 context kc holds the iterate, right-hand side and residual.
+  bool mixed = false;
+  KrylovCtx<float> kc32{};
+  KScal* hks32 = nullptr;
+  float* d_M32 = nullptr;  // M_psi of the inner iterations
+  static constexpr int MIXED_ROUNDS = 8;
+  // inner relative tolerance: at 1024^2 (two vesicles, 100 steps) 1e-5 gave 175.8 SAV steps/s,
+  // 1e-4 159.4, 3e-4 152.7, 1e-6 163.7 (fewer rounds against more inner iterations)
+  static constexpr double MIXED_INNER_RTOL = 1e-5;
   KScal* hks = nullptr;
   DStateScal* hst = nullptr;  // pinned: scalars of the state a step produced
   C* w_bphys_c = nullptr;
@@ -292,6 +304,48 @@ class Solver final : public ISolver {
       kc.hks = hks;
       kc.st = s;
       kc.guess[0] = kc.guess[1] = kc.guess[2] = 8;
+      if constexpr (std::is_same<R, double>::value) {
+        mixed = c.precision == SAV_PREC_MIXED;
+        if (mixed) {
+          kc32.Nx = Nx;
+          kc32.Ny = Ny;
+          kc32.N = N;
+          kc32.nfull = nf;
+          kc32.kx0 = d_kx0;
+          kc32.ky = d_ky;
+          kc32.tw_ri = upload(kmajor_twiddles<float>(Nx, 64, row_plan_maxr(Nx), false));
+          kc32.tw_rf = upload(kmajor_twiddles<float>(Nx, 64, row_plan_maxr(Nx), true));
+          kc32.tw_ci = upload(
+              kmajor_twiddles<float>(Ny, col_plan_ttmax<float>(Ny, false), col_plan_maxr<float>(Ny, false), false));
+          kc32.tw_cf = upload(
+              kmajor_twiddles<float>(Ny, col_plan_ttmax<float>(Ny, true), col_plan_maxr<float>(Ny, true), false));
+          d_M32 = dalloc<float>(nf);
+          kc32.M = d_M32;
+          kc32.ds = ds;
+          kc32.S4 = c.S4;
+          kc32.atol = 0.0;
+          kc32.rtol = MIXED_INNER_RTOL;
+          kc32.itmax = c.itmax;
+          kc32.vs = nf;
+          kc32.ps = kc.ps;
+          kc32.x = dalloc<cplx<float>>(3 * nf);
+          kc32.r = dalloc<cplx<float>>(3 * nf);
+          kc32.p = dalloc<cplx<float>>(3 * nf);
+          kc32.v = dalloc<cplx<float>>(3 * nf);
+          kc32.s = dalloc<cplx<float>>(3 * nf);
+          kc32.t = dalloc<cplx<float>>(3 * nf);
+          kc32.b = dalloc<cplx<float>>(3 * nf);
+          kc32.Bx = dalloc<cplx<float>>(3 * nf);
+          kc32.By = dalloc<cplx<float>>(3 * nf);
+          kc32.partials = partials;
+          kc32.ks = dalloc<KScal>(4);
+          SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks32), 4 * sizeof(KScal), hipHostMallocDefault));
+          std::memset(hks32, 0, 4 * sizeof(KScal));
+          kc32.hks = hks32;
+          kc32.st = s;
+          kc32.guess[0] = kc32.guess[1] = kc32.guess[2] = 4;
+        }
+      }
     }
     SAV_HIP(hipStreamSynchronize(s));
   }
@@ -302,6 +356,7 @@ class Solver final : public ISolver {
     for (void* p : allocs) hipFree(p);
     if (hs) hipHostFree(hs);
     if (hks) hipHostFree(hks);
+    if (hks32) hipHostFree(hks32);
     if (hst) hipHostFree(hst);
     for (auto& gr : graphs) hipGraphExecDestroy(gr.exec);
     if (ev0) hipEventDestroy(ev0);
@@ -550,7 +605,10 @@ class Solver final : public ISolver {
     foreach_n(s, NP, [=] __device__(long long i) { b[i] = CD{rphys[i], T(0)}; });
     fft_full_2d<R>(g, b, N, false, s);
     int it;
-    if (use_learned && !collecting) {
+    if (mixed) {
+      mobility_fp32();
+      it = refined_solve(slot, 1);
+    } else if (use_learned && !collecting) {
       if (stencil_stale) refresh_stencil();
       it = fbicgstab_2d<R>(kc, slot, [this](const C* in, C* o) { learned_apply(in, o); });
       if (!hks[slot].solved) {
@@ -579,6 +637,20 @@ class Solver final : public ISolver {
     return it;
   }
 
+  // M_psi in FP32 for the inner iterations of the mixed-precision solve.
+  void mobility_fp32() {
+    const R* M = w_M;
+    float* m32 = d_M32;
+    foreach_n(s, (long long)N * P, [=] __device__(long long i) { m32[i] = (float)M[i]; });
+  }
+  int refined_solve(int slot, int nsys) {
+    if constexpr (std::is_same<R, double>::value) {
+      return bicgstab_refined(kc, kc32, slot, nsys, MIXED_ROUNDS);
+    } else {
+      throw HipError("the mixed-precision solve needs FP64 storage", SAV_E_UNSUPPORTED);
+    }
+  }
+
   // The three psi solves of step 1 (rhs: 3 consecutive half spectra of N fields) as one
   // batched BiCGSTAB: systems z = 0, 1, 2 use KScal slots 0, 1, 2 and the vectors at
   // offset z * kc.vs.  Each system's iterates equal those of its own solve.
@@ -593,10 +665,14 @@ class Solver final : public ISolver {
       b[z * vs + j] = CD{rphys[i], T(0)};
     });
     fft_full_2d<R>(g, b, 3 * N, false, s);  // vs == nf: the three systems are contiguous
-    if (capture_iters > 0)
+    if (mixed) {
+      mobility_fp32();
+      refined_solve(0, 3);
+    } else if (capture_iters > 0) {
       bicgstab_queue<R>(kc, 0, 3, capture_iters);
-    else
+    } else {
       bicgstab_batch<R>(kc, 0, 3);
+    }
     const C* x = kc.x;
     const int Nxl = Nx, Nxhl = Nxh;
     const long long nh = (long long)N * H;
@@ -857,6 +933,7 @@ class Solver final : public ISolver {
   }
 
   int set_precond(int mode) override {
+    if (mixed && mode != SAV_PRECOND_DIAG) return SAV_E_UNSUPPORTED;  // diagonal inner solves only
     if (mode != SAV_PRECOND_DIAG && mode != SAV_PRECOND_LEARNED && mode != SAV_PRECOND_STENCIL) return SAV_E_ARG;
     if (mode != SAV_PRECOND_DIAG) ensure_net();
     use_learned = mode != SAV_PRECOND_DIAG;
@@ -885,6 +962,7 @@ class Solver final : public ISolver {
   }
 
   int precond_collect(int on, int per_snapshot, uint64_t seed) override {
+    if (mixed) return SAV_E_UNSUPPORTED;
     ensure_net();
     if (!on) {
       collecting = false;
@@ -1921,7 +1999,8 @@ class Solver final : public ISolver {
   long long graph_replays = 0, graph_repeats = 0;
 
   bool graph_usable() const {
-    return graph_on && kc.timed == 0 && (nd == 3 || (!use_learned && !collecting && kc.colw != 0));
+    // a mixed-precision step decides its rounds on the host: eager only
+    return graph_on && kc.timed == 0 && (nd == 3 || (!mixed && !use_learned && !collecting && kc.colw != 0));
   }
   void drop_graphs() {
     if (!graphs.empty()) SAV_HIP(hipStreamSynchronize(s));
@@ -2110,11 +2189,23 @@ class Solver final : public ISolver {
     kc.tk_count = 0;
     kc.tk_ms = 0.0;
     kc.tk_bytes = 0.0;
+    if (mixed) {  // the FP32 inner iterations carry the matvec kernels of a mixed solve
+      kc32.ev = tev.data();
+      kc32.ev_cap = 256;
+      kc32.timed = id;
+      kc32.ev_used = 0;
+      kc32.tk_count = 0;
+      kc32.tk_ms = 0.0;
+      kc32.tk_bytes = 0.0;
+      kc.timed = 0;
+    }
   }
   void kernel_timing_get(long long* n, double* ms, double* bytes) override {
-    if (n) *n = kc.tk_count;
-    if (ms) *ms = kc.tk_ms;
-    if (bytes) *bytes = kc.tk_count ? kc.tk_bytes / (double)kc.tk_count : 0.0;
+    const long long cnt = mixed ? kc32.tk_count : kc.tk_count;
+    const double tms = mixed ? kc32.tk_ms : kc.tk_ms, tb = mixed ? kc32.tk_bytes : kc.tk_bytes;
+    if (n) *n = cnt;
+    if (ms) *ms = tms;
+    if (bytes) *bytes = cnt ? tb / (double)cnt : 0.0;
   }
 
   void rfft_host(const double* in, double* out, int count) override {
@@ -2249,6 +2340,7 @@ void Solver<R>::reduce_max_impl(const R* x, long long n, double* out) {
 
 std::unique_ptr<ISolver> make_solver(const HostCfg& cfg) {
   if (cfg.c.precision == SAV_PREC_FP32) return std::unique_ptr<ISolver>(new Solver<float>(cfg));
+  // SAV_PREC_FP64 and SAV_PREC_MIXED (FP64 storage; FP32 inner psi iterations in 2D)
   return std::unique_ptr<ISolver>(new Solver<double>(cfg));
 }
 

@@ -36,6 +36,7 @@ SAV_E_NONFINITE = -4
 SAV_E_UNSUPPORTED = -5
 SAV_PREC_FP64 = 0
 SAV_PREC_FP32 = 1
+SAV_PREC_MIXED = 2
 SAV_PRECOND_DIAG = 0
 SAV_PRECOND_LEARNED = 1
 SAV_PRECOND_STENCIL = 2
@@ -237,8 +238,10 @@ def config_c(conf: Config, precision: int, itmax: int = 20, atol: float = 1e-14,
 class Operators:
     """Owns one libsav_hip handle: plans, twiddles, multipliers, device state.
 
-    ``precision`` is SAV_PREC_FP64 (the reference's arithmetic) or SAV_PREC_FP32
-    (FP32 fields/FFTs with FP64 reductions, 3x3 solve and Krylov scalars)."""
+    ``precision`` is SAV_PREC_FP64 (the reference's arithmetic), SAV_PREC_FP32
+    (FP32 fields/FFTs with FP64 reductions, 3x3 solve and Krylov scalars) or
+    SAV_PREC_MIXED (FP64 throughout except the inner iterations of the 2D psi solve,
+    an FP32 BiCGSTAB inside an FP64 defect correction with the reference's stopping test)."""
 
     def __init__(self, conf: Config, precision: int = SAV_PREC_FP64, itmax: int = 20, atol: float = 1e-14,
                  device: int = 0, precond: int = SAV_PRECOND_DIAG, _borrow=None):
