@@ -2,10 +2,12 @@
 """SAV steps/s of the seven-step pseudo-spectral timestep on one MI355X.
 
 Default workload: two seeded vesicles on 1024^2 (eps = 0.015, dt = 2e-5, BiCGSTAB
-rtol 1e-10, itmax 20), FP32 fields with FP64 reductions.  The same line reports
-the FP64 form at 1024^2, both forms at 128^3 (two ellipsoids), the psi-solve
-iteration counts, the FP32-vs-FP64 field error, the roofline of the dominant
-kernel and the CPU oracle timed on the host cores.
+rtol 1e-10, itmax 20) in FP64, the reference's arithmetic.  The same line reports the
+mixed form (FP32 inner psi iterations inside an FP64 defect correction) and the FP32-fields
+form with their field errors against FP64, the FP64 and FP32 forms at 128^3 (two
+ellipsoids), the psi-solve iteration counts, the roofline of the dominant FP64 kernel,
+graph replay, the learned preconditioner, the 64-instance ensemble and the CPU oracle
+timed on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -24,15 +26,16 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = ROOT / "profiles" / "traffic_k_mv_rows_two_vesicles_1024.md"
+TRAFFIC_PROFILE = ROOT / "profiles" / "traffic_psi_matvec_fp64_two_vesicles_1024.md"
 
 
-def _traffic_ratio():
-    """HBM bytes / algorithmic bytes of k_mv_rows<float, 1024>, measured with rocprofv3
-    FETCH_SIZE/WRITE_SIZE counters scaled by a plain copy (the committed profile)."""
+def _traffic_ratio(kernel):
+    """HBM bytes / algorithmic bytes of one FP64 psi-matvec kernel at 1024^2, measured with
+    rocprofv3 FETCH_SIZE/WRITE_SIZE counters scaled by a plain copy (the committed profile
+    holds one `| traffic_over_algorithmic <kernel> | ratio |` row per kernel)."""
     try:
         for line in TRAFFIC_PROFILE.read_text().splitlines():
-            if line.startswith("| traffic_over_algorithmic |"):
+            if line.startswith(f"| traffic_over_algorithmic {kernel} |"):
                 return float(line.split("|")[2])
     except (OSError, ValueError):
         pass
@@ -226,30 +229,35 @@ def _rel(a, b):
 
 
 def run_configs(jv, cpu=True):
-    """One row per benchmark configuration at its full step count: SAV steps/s in FP64
-    and in FP32 (fields FP32, reductions FP64), psi iterations per step, the max-norm
-    relative error of FP32 against FP64 for phi, psi, u and the modified energy (over
-    every step), algorithmic bytes per step and the rate they imply.  The single-vesicle
-    128^2 case also runs the FP64 CPU oracle over all its steps and reports the GPU's
-    FP64 error against it."""
+    """One row per benchmark configuration at its full step count: SAV steps/s in FP64,
+    in the mixed form (2D: FP32 inner psi iterations in an FP64 defect correction) and in
+    the FP32-fields form, psi iterations per step, the max-norm relative error of each form
+    against FP64 for phi, psi, u and the modified energy (over every step), whether each
+    energy history is non-increasing, algorithmic bytes per step and the rate they imply.
+    The single-vesicle 128^2 case also runs the FP64 CPU oracle over all its steps and
+    reports the GPU's FP64 error against it."""
     rows = {}
     for wl in (jv.workload_c1(), jv.workload_c2(), jv.workload_c3(), jv.workload_c4()):
         conf = wl.conf
-        r64, s64 = _run_full(jv, wl, jv.SAV_PREC_FP64)
-        r32, s32 = _run_full(jv, wl, jv.SAV_PREC_FP32)
+        runs = {"fp64": _run_full(jv, wl, jv.SAV_PREC_FP64)}
+        if conf.ndim == 2:
+            runs["mixed"] = _run_full(jv, wl, jv.SAV_PREC_MIXED)
+        runs["fp32"] = _run_full(jv, wl, jv.SAV_PREC_FP32)
+        r64, s64 = runs["fp64"]
         row = dict(grid=f"{conf.Nx}^{conf.ndim}" if conf.ndim == 3 else f"{conf.Nx}x{conf.Ny}", N=conf.N,
-                   steps=wl.steps, epsilon=conf.epsilon, dt=conf.dt, status=[r64["rc"], r32["rc"]])
-        for name, r, s in (("fp64", r64, 8), ("fp32", r32, 4)):
+                   steps=wl.steps, epsilon=conf.epsilon, dt=conf.dt, S2=conf.S2,
+                   status={k: r["rc"] for k, (r, _) in runs.items()})
+        for name, (r, st) in runs.items():
+            sb = 4 if name == "fp32" else 8
             k = r["krylov_iters"] / wl.steps
-            B = algorithmic_bytes(conf, s, k)
+            B = algorithmic_bytes(conf, sb, k)
             row[name] = dict(steps_per_s=round(r["steps_per_s"], 2), psi_iters_per_step=round(k, 3),
                              algorithmic_gb_per_step=round(B / 1e9, 4),
-                             algorithmic_tb_per_s=round(B * r["steps_per_s"] / 1e12, 3))
-        row["fp32_vs_fp64_max_rel_err"] = {
-            "phi": _rel(s32.phi, s64.phi), "psi": _rel(s32.psi, s64.psi), "u": _rel(s32.u, s64.u),
-            "energy": _rel(r32["E"], r64["E"])}
-        row["energy_monotone"] = {"fp64": bool(np.all(np.diff(r64["E"]) <= 0)),
-                                  "fp32": bool(np.all(np.diff(r32["E"]) <= 0))}
+                             algorithmic_tb_per_s=round(B * r["steps_per_s"] / 1e12, 3),
+                             energy_monotone=bool(np.all(np.diff(r["E"]) <= 0)))
+            if name != "fp64":
+                row[name]["max_rel_err_vs_fp64"] = {"phi": _rel(st.phi, s64.phi), "psi": _rel(st.psi, s64.psi),
+                                                    "u": _rel(st.u, s64.u), "energy": _rel(r["E"], r64["E"])}
         if cpu and wl.name == "single_vesicle_128":
             from oracle import sav_oracle as so
             phi, psi = wl.initial_fields()
@@ -263,15 +271,38 @@ def run_configs(jv, cpu=True):
                                      gpu_fp64_vs_oracle_max_rel_err={
                                          "phi": _rel(s64.phi.T, pres.phi), "psi": _rel(s64.psi.T, pres.psi)})
         rows[wl.name] = row
-        print(f"config {wl.name}: fp64 {row['fp64']['steps_per_s']} fp32 {row['fp32']['steps_per_s']} steps/s",
+        print(f"config {wl.name}: " + " ".join(f"{k} {row[k]['steps_per_s']}" for k in runs) + " steps/s",
               file=sys.stderr, flush=True)
-    ens = run_ensemble(jv, 500, jv.SAV_PREC_FP32, threads_list=(1, 16))
-    rows["ensemble_64"] = dict(steps=500, precision="fp32",
+    ens = run_ensemble(jv, 500, jv.SAV_PREC_FP64, threads_list=(1, 16))
+    rows["ensemble_64"] = dict(steps=500, precision="fp64",
                                sequential_steps_per_s=round(ens["by_threads"][1]["steps_per_s"], 2),
                                threads16_steps_per_s=round(ens["by_threads"][16]["steps_per_s"], 2),
                                speedup_vs_sequential=round(ens["speedup_vs_sequential"], 3),
                                status=[v["rc"] for v in ens["by_threads"].values()])
     return rows
+
+
+FIELD_KEYS = ("phi", "psi", "u", "mu", "nu", "p")
+
+
+def _errors(s, ref, E=None, Eref=None):
+    """Max-norm relative error of each field of `s` against `ref` (and of the energy history)."""
+    out = {k: _rel(getattr(s, k), getattr(ref, k)) for k in FIELD_KEYS}
+    if E is not None:
+        out["energy"] = _rel(np.asarray(E), np.asarray(Eref))
+    return out
+
+
+def _host_info():
+    info = {"nproc": os.cpu_count()}
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return info
 
 
 def main():
@@ -280,12 +311,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble"], default="all")
+    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble", "fp64-2d", "mixed-2d", "fp64-3d"],
+                    default="all", help="fp64-2d / mixed-2d / fp64-3d: one run of that form, for profiler runs")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--column-tile", type=int, default=0, choices=[0, 8, 16],
                     help="fix the psi matvec's columns per workgroup (0: autotuned; fixing it makes "
                          "separate profiler runs launch the same kernels)")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="steps of the CPU oracle sample")
     ap.add_argument("--configs", action="store_true",
                     help="one row per benchmark configuration at its full step count (a separate report, "
                          "not the headline line)")
@@ -299,9 +332,9 @@ def main():
         sys.exit(2)
     global COLUMN_TILE
     COLUMN_TILE = a.column_tile
+    import torch  # noqa: F401  (device-wide synchronisation around the timed region)
     import __graft_entry__ as ge
     jv = ge.load_package()
-    import torch  # noqa: F401  (device-wide synchronisation around the timed region)
     if a.copy_calibration:
         src = torch.empty(64 * 2 ** 20, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
@@ -314,32 +347,46 @@ def main():
         print(json.dumps(run_configs(jv, cpu=not a.no_cpu)), flush=True)
         return
     if a.only == "ensemble":
-        print(json.dumps(run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32)), flush=True)
+        print(json.dumps(run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64)), flush=True)
+        return
+    wl3 = jv.workload_c3(n=1024, steps=a.steps)
+    wl4 = jv.workload_c4(n=128, steps=a.steps)
+    if a.only in ("fp64-2d", "mixed-2d", "fp64-3d"):
+        wl, prec = {"fp64-2d": (wl3, jv.SAV_PREC_FP64), "mixed-2d": (wl3, jv.SAV_PREC_MIXED),
+                    "fp64-3d": (wl4, jv.SAV_PREC_FP64)}[a.only]
+        r, _ = run_case(jv, wl, prec, a.steps, a.warmup)
+        print(json.dumps({"only": a.only, "steps_per_s": r["steps_per_s"], "ms_per_step": r["ms_per_step"],
+                          "krylov_iters": r["krylov_iters"]}), flush=True)
         return
     out = {}
-    wl3 = jv.workload_c3(n=1024, steps=a.steps)
     if a.only in ("all", "2d"):
-        r32, s32 = run_case(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup, timed_kernel=2)
         r64, s64 = run_case(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup)
-        err = {k: float(np.max(np.abs(getattr(s32, k) - getattr(s64, k))) / np.max(np.abs(getattr(s64, k))))
-               for k in ("phi", "psi", "mu", "nu")}
-        err["u"] = float(np.max(np.abs(s32.u - s64.u)) / max(np.max(np.abs(s64.u)), 1e-300))
-        out["2d"] = dict(fp32=r32, fp64=r64, fp32_vs_fp64_max_rel_err=err)
-        # the same FP32 run with each step replayed as one captured HIP graph
-        out["2d"]["fp32_graph"], _ = run_case(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup, graph=True)
+        # each psi-matvec kernel timed live in a run of its own (the events delay the next dispatch)
+        timed = {}
+        for kid in (1, 2, 3):
+            rk, _ = run_case(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup, timed_kernel=kid)
+            timed[kid] = rk["kernel"]
+        rmx, smx = run_case(jv, wl3, jv.SAV_PREC_MIXED, a.steps, a.warmup)
+        r32, s32 = run_case(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup)
+        rg, sg = run_case(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup, graph=True)
+        out["2d"] = dict(fp64=r64, timed=timed, mixed=rmx, fp32=r32, graph=rg,
+                         mixed_err=_errors(smx, s64), fp32_err=_errors(s32, s64),
+                         graph_equal=all(np.array_equal(getattr(sg, k), getattr(s64, k)) for k in FIELD_KEYS))
         if not a.no_learned:
-            out["2d"]["fp32_learned"] = run_case_learned(jv, wl3, jv.SAV_PREC_FP32, a.steps, a.warmup)
+            out["2d"]["learned"] = run_case_learned(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup)
     if a.only in ("all", "3d"):
-        wl4 = jv.workload_c4(n=128, steps=a.steps)
-        q32, t32 = run_case(jv, wl4, jv.SAV_PREC_FP32, a.steps, a.warmup)
         q64, t64 = run_case(jv, wl4, jv.SAV_PREC_FP64, a.steps, a.warmup)
-        err3 = {k: float(np.max(np.abs(getattr(t32, k) - getattr(t64, k))) / np.max(np.abs(getattr(t64, k))))
-                for k in ("phi", "psi")}
-        out["3d"] = dict(fp32=q32, fp64=q64, fp32_vs_fp64_max_rel_err=err3)
+        q32, t32 = run_case(jv, wl4, jv.SAV_PREC_FP32, a.steps, a.warmup)
+        out["3d"] = dict(fp64=q64, fp32=q32, fp32_err=_errors(t32, t64))
 
-    head = out["2d"]["fp32"] if "2d" in out else out["3d"]["fp32"]
+    if "2d" in out:
+        head, wl_head, grid = out["2d"]["fp64"], wl3, "1024x1024"
+    else:
+        head, wl_head, grid = out["3d"]["fp64"], wl4, "128^3"
+    conf = wl_head.conf
     line = {
-        "metric": "SAV steps/s at 1024^2 (two vesicles, FP32 fields + FP64 reductions, diagonal psi preconditioner)",
+        "metric": "SAV steps/s at 1024^2 (two vesicles, FP64, diagonal psi preconditioner)" if "2d" in out
+                  else "SAV steps/s at 128^3 (two ellipsoids, FP64)",
         "value": round(head["steps_per_s"], 3),
         "unit": "SAV steps/s",
         "n_gpus": 1,
@@ -349,49 +396,69 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64-reductions",
+        "dtype": "f64",
         "data": "synthetic (seeded initial vesicle fields)",
-        "config": {"workload": wl3.name if "2d" in out else "two_ellipsoids_128cubed",
-                   "grid": "1024x1024" if "2d" in out else "128^3", "N": 2,
-                   "epsilon": 0.015, "dt": 2e-5, "krylov_rtol": 1e-10, "itmax": 20,
-                   "precond": "diag", "parallelism": "single GPU"},
+        "config": {"workload": wl_head.name, "grid": grid, "N": conf.N, "epsilon": conf.epsilon, "dt": conf.dt,
+                   "S2": round(conf.S2, 6), "krylov_rtol": conf.tol, "itmax": 20, "precond": "diag",
+                   "parallelism": "single GPU"},
     }
     if "2d" in out:
-        k = out["2d"]["fp32"]["kernel"]
+        o = out["2d"]
+        # the dominant psi-matvec kernel of the FP64 step: the longest average launch (each of
+        # the three runs twice per BiCGSTAB iteration)
+        kid = max(o["timed"], key=lambda i: o["timed"][i]["avg_ms"])
+        k = o["timed"][kid]
         achieved = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9
-        ratio = _traffic_ratio()
-        line["roofline"] = {"bound": "hbm", "kernel": k["name"] + "<float, 1024>", "achieved": round(achieved, 1),
+        ratio = _traffic_ratio(k["name"])
+        line["roofline"] = {"bound": "hbm", "kernel": k["name"] + "<double, 1024>", "achieved": round(achieved, 1),
                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                             "traffic": round(ratio * k["bytes"]) if ratio else None,
                             "traffic_over_algorithmic": ratio,
                             "avg_launch_ms": round(k["avg_ms"], 5), "launches": k["launches"],
                             "bytes_per_launch": round(k["bytes"])}
-        line["psi_iters_per_step"] = {"fp32": out["2d"]["fp32"]["krylov_iters"] / a.steps,
-                                      "fp64": out["2d"]["fp64"]["krylov_iters"] / a.steps}
-        line["fp64_steps_per_s_1024"] = round(out["2d"]["fp64"]["steps_per_s"], 3)
-        G = out["2d"]["fp32_graph"]
-        line["fp32_graph_replay_1024"] = {"steps_per_s": round(G["steps_per_s"], 3), **G["graph"]}
-        line["psi_matvec_column_tile_1024"] = {"fp32": out["2d"]["fp32"]["autotune"],
-                                               "fp64": out["2d"]["fp64"]["autotune"]}
-        if "fp32_learned" in out["2d"]:
-            L = out["2d"]["fp32_learned"]
+        line["psi_matvec_kernels_fp64_1024"] = {
+            v["name"]: {"avg_launch_ms": round(v["avg_ms"], 5), "bytes_per_launch": round(v["bytes"]),
+                        "achieved_gb_s": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)}
+            for v in o["timed"].values()}
+        line["psi_iters_per_step_1024"] = {"fp64": o["fp64"]["krylov_iters"] / a.steps,
+                                           "mixed_inner_fp32": o["mixed"]["krylov_iters"] / a.steps}
+        line["energy_monotone_1024"] = {"fp64": o["fp64"]["energy_monotone"], "mixed": o["mixed"]["energy_monotone"],
+                                        "fp32": o["fp32"]["energy_monotone"]}
+        line["mixed_1024"] = {"steps_per_s": round(o["mixed"]["steps_per_s"], 3),
+                              "speedup_vs_fp64": round(o["mixed"]["steps_per_s"] / o["fp64"]["steps_per_s"], 3),
+                              "dtype": "f64 state and stages, f32 inner psi iterations (f64 defect correction)",
+                              "max_rel_err_vs_fp64": o["mixed_err"]}
+        line["fp32_fields_1024"] = {"steps_per_s": round(o["fp32"]["steps_per_s"], 3),
+                                    "dtype": "f32 fields and transforms, f64 reductions",
+                                    "max_rel_err_vs_fp64": o["fp32_err"]}
+        line["fp64_graph_replay_1024"] = {"steps_per_s": round(o["graph"]["steps_per_s"], 3),
+                                          "bit_identical_to_eager": o["graph_equal"], **o["graph"]["graph"]}
+        line["psi_matvec_column_tile_1024"] = o["fp64"]["autotune"]
+        if "learned" in o:
+            L = o["learned"]
             line["learned_precond_1024"] = {
-                "steps_per_s": round(L["steps_per_s"], 3), "psi_iters_per_step": L["psi_iters_per_step"],
+                "dtype": "f64 solve, bf16 network", "steps_per_s": round(L["steps_per_s"], 3),
+                "psi_iters_per_step": L["psi_iters_per_step"],
                 "analytic_stencil_steps_per_s": round(L["stencil_steps_per_s"], 3),
                 "analytic_stencil_psi_iters_per_step": L["stencil_psi_iters_per_step"],
-                "diag_psi_iters_per_step": out["2d"]["fp32"]["krylov_iters"] / a.steps,
-                "diag_fallback_solves": L["diag_fallback_solves"], "windows": L["windows"], "train_iters": L["train_iters"], "train_seconds": L["train_seconds"],
+                "diag_psi_iters_per_step": o["fp64"]["krylov_iters"] / a.steps,
+                "diag_fallback_solves": L["diag_fallback_solves"], "windows": L["windows"],
+                "train_iters": L["train_iters"], "train_seconds": L["train_seconds"],
                 "loss_first": L["loss_first"], "loss_last": L["loss_last"]}
-        line["fp32_vs_fp64_max_rel_err_1024"] = out["2d"]["fp32_vs_fp64_max_rel_err"]
-        line["energy_monotone_1024"] = out["2d"]["fp64"]["energy_monotone"]
     if "3d" in out:
-        line["fp32_steps_per_s_128cubed"] = round(out["3d"]["fp32"]["steps_per_s"], 3)
-        line["fp64_steps_per_s_128cubed"] = round(out["3d"]["fp64"]["steps_per_s"], 3)
-        line["fp32_vs_fp64_max_rel_err_128cubed"] = out["3d"]["fp32_vs_fp64_max_rel_err"]
+        o = out["3d"]
+        line["fp64_steps_per_s_128cubed"] = round(o["fp64"]["steps_per_s"], 3)
+        line["mixed_128cubed"] = {"steps_per_s": round(o["fp64"]["steps_per_s"], 3),
+                                  "note": "the 3D psi solve is a spectral divide: the mixed form has no FP32 "
+                                          "stage in 3D and runs the FP64 step"}
+        line["fp32_fields_128cubed"] = {"steps_per_s": round(o["fp32"]["steps_per_s"], 3),
+                                        "max_rel_err_vs_fp64": o["fp32_err"]}
+        line["energy_monotone_128cubed"] = {"fp64": o["fp64"]["energy_monotone"],
+                                            "fp32": o["fp32"]["energy_monotone"]}
     if a.only == "all":
-        ens = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32)
-        ensg = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP32, threads_list=(1, 16), graph=True)
-        line["ensemble_64_fp32"] = {"instances": ens["instances"], "steps": ens["steps"],
+        ens = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64)
+        ensg = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64, threads_list=(1, 16), graph=True)
+        line["ensemble_64_fp64"] = {"instances": ens["instances"], "steps": ens["steps"],
                                     "sequential_steps_per_s": round(ens["by_threads"][1]["steps_per_s"], 2),
                                     "best_steps_per_s": round(ens["by_threads"][ens["best_threads"]]["steps_per_s"], 2),
                                     "best_threads": ens["best_threads"],
@@ -400,7 +467,9 @@ def main():
                                     "graph_replay_by_threads": {k: round(v["steps_per_s"], 2)
                                                                 for k, v in ensg["by_threads"].items()}}
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(jv, wl3 if "2d" in out else jv.workload_c4(n=128, steps=1))
+        line["cpu_baseline"] = cpu_baseline(jv, wl_head if "2d" in out else jv.workload_c4(n=128, steps=1),
+                                            nsteps=a.cpu_steps if "2d" in out else 2)
+        line["cpu_baseline"]["host"] = _host_info()
     print(json.dumps(line), flush=True)
 
 

@@ -56,3 +56,46 @@ def test_two_vesicles_1024_match_oracle(jv, so):
 
 def test_two_ellipsoids_128cubed_match_oracle(jv, so):
     _compare_two_steps(jv, so, jv.workload_c4())
+
+
+def test_four_vesicles_512_match_oracle(jv, so):
+    _compare_two_steps(jv, so, jv.workload_c2())
+
+
+def test_ensemble_members_match_oracle(jv, so):
+    # one member of each grid size of the 64-instance workload (384^2 is a radix-3 length),
+    # stepped inside an Ensemble from host threads, against the oracle of each member
+    wls = jv.workload_c5(steps=2)
+    picks = []
+    for n in (128, 256, 384, 512):
+        size = [w for w in wls if w.conf.Nx == n]
+        # the member with the most vesicles (and, among those, the stiffest bending)
+        picks.append(max(size, key=lambda w: (w.conf.N, w.conf.gamma_bend)))
+    assert any(w.conf.S2 > 0 for w in wls)
+    ens = jv.Ensemble([w.conf for w in picks], precision=jv.SAV_PREC_FP64)
+    try:
+        for m, wl in zip(ens.members, picks):
+            phi, psi = wl.initial_fields()
+            m.set_state(jv.SAV_PRESENT, device_state(jv, m, wl.conf, phi, psi))
+            m.init_sav()
+        t = np.arange(3) * picks[0].conf.dt
+        rc, E, A, kit, st = ens.run(t, 1, threads=4)
+        assert rc == 0 and not st.any()
+        got = [m.get_state(jv.SAV_PRESENT) for m in ens.members]
+    finally:
+        ens.close()
+    for i, wl in enumerate(picks):
+        conf = wl.conf
+        phi, psi = wl.initial_fields()
+        orc = so.Oracle(conf, workers=min(os.cpu_count() or 1, 16))
+        pres, old = orc.init_sav(so.state_from_physical(phi, psi, orc.ops, 2))
+        for n in (1, 2):
+            out = orc.step(pres, old, conf.dt, n)
+        errs = {name: relerr(getattr(got[i], name).T, getattr(pres, name))
+                for name in ("phi", "psi", "mu", "nu", "u", "p")}
+        print(f"member {conf.Nx}^2 N={conf.N} S2={conf.S2:.3g}: " +
+              " ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+        for name in ("phi", "psi", "mu", "nu"):
+            assert errs[name] < FP64_TOL, (conf.Nx, name, errs[name])
+        assert errs["u"] < U_TOL and errs["p"] < U_TOL
+        assert E[i][-1] == pytest.approx(out["energy"], rel=0, abs=FP64_TOL * abs(conf.lamda) * (conf.C1 + conf.C2 + conf.C3))
