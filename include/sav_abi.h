@@ -137,6 +137,19 @@ int sav_rfft(sav_handle_t* h, const double* in, double* out_hat, int count);
 int sav_irfft(sav_handle_t* h, const double* in_hat, double* out, int count);
 int sav_fft_full(sav_handle_t* h, const double* in, double* out, int count, int inverse);
 
+/* Allocation-free irfft on DEVICE buffers.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same transform as sav_irfft (to_physical!,
+ * src/SpectralUtils.jl:144-148, input untouched), on the handle's stream, returning when
+ * out_dev is written.  in_hat_dev: count half spectra, out_dev: count fields, both in the
+ * handle's storage type (double for SAV_PREC_FP64 / SAV_PREC_MIXED, float for SAV_PREC_FP32).
+ * The column pass's intermediate lives in a file-scope device array of
+ * SAV_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); the call makes no
+ * hipMalloc/hipFree.  SAV_E_ARG when count * (Nx/2+1) * Ny [* Nz] complex values exceed it.
+ * Calls that overlap on one GPU (several handles or threads) share that array and race. */
+#define SAV_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sav_irfft_noalloc(sav_handle_t* h, const void* in_hat_dev, void* out_dev, int count);
+
 /* Functionals at (phi, psi): H1, H2, H3, G (each N fields), W[3], area[N]
  * (get_H1/get_H2/get_H3/get_MG, get_W1..3, calculate_area; src/Utils.jl:66-249).
  * A0 is taken from the present state.  Returns SAV_E_WC_NEGATIVE like the reference. */

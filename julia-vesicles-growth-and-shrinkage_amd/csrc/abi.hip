@@ -152,6 +152,20 @@ int sav_irfft(sav_handle_t* h, const double* in_hat, double* out, int count) {
   return SAV_OK;
 }
 
+int sav_irfft_noalloc(sav_handle_t* h, const void* in_hat_dev, void* out_dev, int count) {
+  if (!h) return SAV_E_ARG;
+  if (!in_hat_dev || !out_dev || count < 1) {
+    h->err = "sav_irfft_noalloc: null buffer or count < 1";
+    return SAV_E_ARG;
+  }
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->irfft_noalloc(in_hat_dev, out_dev, count));
+  if (rc == SAV_E_ARG)
+    h->err = "sav_irfft_noalloc: count half spectra exceed the " + std::to_string(SAV_NOALLOC_SCRATCH_BYTES >> 20) +
+             " MiB scratch (count * (Nx/2+1) * Ny [* Nz] * complex size)";
+  return rc;
+}
+
 int sav_fft_full(sav_handle_t* h, const double* in, double* out, int count, int inverse) {
   if (!h || !in || !out || count < 1) return SAV_E_ARG;
   if (h->ndim != 2) {

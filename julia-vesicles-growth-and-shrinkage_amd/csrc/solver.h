@@ -66,6 +66,9 @@ class ISolver {
   virtual int psi_solve_host(const double* phi_star, double a, double dt, const double* rhs, double* out,
                              int32_t* iters, int32_t* solved, double* rnorm) = 0;
   virtual int modified_energy(double* E, double* ar) = 0;
+  // irfft of device buffers with the file-scope scratch array as the column-pass
+  // intermediate (no allocation); returns SAV_E_ARG when count fields exceed the scratch
+  virtual int irfft_noalloc(const void* in_hat, void* out, int count) = 0;
   virtual double last_elapsed_ms() = 0;
   virtual void kernel_timing(int id) = 0;
   virtual void kernel_timing_get(long long* n, double* ms, double* bytes) = 0;

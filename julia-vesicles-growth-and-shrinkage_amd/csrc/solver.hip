@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "krylov.h"
@@ -90,6 +91,27 @@ struct KIdx {
 
 enum { F_PHI = 0, F_PSI = 1, F_MU = 2, F_NU = 3 };
 
+// Scratch of the allocation-free entry point (sav_irfft_noalloc): a file-scope device array,
+// one copy per GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in symbol): it takes the per-device address
+// from hipGetSymbolAddress, looked up once per device when a handle is created there.
+static __device__ __align__(256) unsigned char noalloc_scratch[SAV_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[64] = {};
+
+// Device address of noalloc_scratch on `device` (which must be current).
+static void* noalloc_scratch_address(int device) {
+  if (device < 0 || device >= 64) throw HipError("device index out of range for the scratch table", SAV_E_ARG);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    SAV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned a null scratch address", SAV_E_HIP);
+    g_noalloc_ptr[device] = p;
+  }
+  return g_noalloc_ptr[device];
+}
+
 template <typename R>
 class Solver final : public ISolver {
  public:
@@ -153,6 +175,7 @@ class Solver final : public ISolver {
   C* tmp4 = nullptr;
   // Krylov
   KrylovCtx<R> kc{};
+  void* scratch = nullptr;  // this device's noalloc_scratch (sav_irfft_noalloc)
   // SAV_PREC_MIXED (2D; FP64 storage): every stage in FP64 except the psi BiCGSTAB's inner
   // iterations, which run in FP32 on the residual equation of an FP64 defect correction
   // (bicgstab_refined); the FP64 context kc holds the iterate, right-hand side and residual.
@@ -191,6 +214,7 @@ class Solver final : public ISolver {
     c.psi_in_v = hc.psi_in.data();
     c.psi_out_v = hc.psi_out.data();
     SAV_HIP(hipSetDevice(c.device));
+    scratch = noalloc_scratch_address(c.device);
     SAV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     SAV_HIP(hipEventCreate(&ev0));
     SAV_HIP(hipEventCreate(&ev1));
@@ -2294,6 +2318,20 @@ class Solver final : public ISolver {
     if (solved) *solved = hks[0].solved;
     if (rnorm) *rnorm = hks[0].rnorm;
     return hks[0].solved ? 0 : SAV_W_KRYLOV_NOT_CONVERGED;
+  }
+
+  // irfft of `count` device half spectra (storage type R) into device fields, with the
+  // intermediate of the column pass in this device's noalloc_scratch: launches only, no
+  // allocation.  Every element of the intermediate is written before it is read, so the
+  // scratch needs no clearing; calls that overlap on one GPU share it and race.
+  int irfft_noalloc(const void* in_hat, void* out, int count) override {
+    const long long need = (long long)count * H * (long long)sizeof(C);
+    if (count < 1 || need > (long long)SAV_NOALLOC_SCRATCH_BYTES) return SAV_E_ARG;
+    SpecJobs js{};
+    js.j[js.n++] = job(in_hat, out, count);
+    irfft_jobs<R>(g, js, reinterpret_cast<C*>(scratch), s);
+    SAV_HIP(hipStreamSynchronize(s));
+    return 0;
   }
 
   int modified_energy(double* E, double* ar) override {

@@ -70,7 +70,7 @@ class SavStepStatsC(ct.Structure):
 
 
 EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sav_set_state",
-           "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft",
+           "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft", "sav_irfft_noalloc",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
            "sav_set_column_tile", "sav_autotune_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
@@ -113,6 +113,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_rfft.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int]
     lib.sav_irfft.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int]
     lib.sav_fft_full.argtypes = [ct.c_void_p, _dp, _dp, ct.c_int, ct.c_int]
+    lib.sav_irfft_noalloc.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int]
     lib.sav_eval_functionals.argtypes = [ct.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.sav_psi_solve.argtypes = [ct.c_void_p, _dp, ct.c_double, ct.c_double, _dp, _dp,
                                   ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), _dp]
