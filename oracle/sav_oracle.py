@@ -37,13 +37,21 @@ What pins each function (tests/test_oracle_kat.py, tests/test_golden.py):
 * ``f_surf``/``calculate_area``, ``f_bend``/``get_omega``: closed forms (tanh stripe,
   constant phi).
 * ``get_W2``/``get_H2``, ``get_W3``/``get_H3``/``get_MG``: Gateaux derivatives
-  (H2 R2 = dW2/dphi exactly when gamma_surf = 0; H3, G exact derivatives of W3).
-* ``get_H1``/``get_W1``: constant-phi closed form only; H1 is not an exact derivative
-  (Utils.jl:198-206), so its general formula is parity unpinned beyond restating it.
+  (H2 R2 = dW2/dphi exactly when gamma_surf = 0; H3, G exact derivatives of W3), and
+  the whole of H2 and W2 (surface and bending terms) in closed form on a tanh stripe of
+  width delta != eps.
+* ``get_H1``/``get_W1``: H1 is not an exact derivative (Utils.jl:198-206); it is pinned by
+  its closed form on the same stripe with A0 != A (omega = (phi^3 - phi)(1/eps -
+  eps/delta^2), area per interface delta/(2 eps) + sqrt2 eps/(3 delta)), and by the
+  constant-phi case.
 * ``bicgstab``: dense ``numpy.linalg.solve`` of the assembled operator on 8x8 grids,
   the constant-mobility case (a spectral divide), a zero right-hand side, itmax
-  reporting; the exact iterate of Krylov.jl at ``itmax`` is parity unpinned (it needs
-  the Julia package).
+  reporting, and at every iteration (also when stopped at itmax) the identities that
+  define the recurrence: r_k = M(b - A x_k), <c, s_k> = 0, <t_k, r_k> = 0,
+  rho_{k+1} = <c, r_k>.  Krylov.jl's own floating-point operation order is not
+  reproduced bit for bit (the Julia package is absent).
+* ``AltOps``: the same transforms by a second route, for the round-off floor between
+  two FFT implementations (tests/test_oracle_fft_floor.py).
 * ``solve_step1`` .. ``solve_step7``, ``step``: invariants (spectral div u = 0,
   modified energy non-increasing, mean psi conserved, BDF1 first step); end-to-end
   trajectories of the Julia code are parity unpinned (no Julia here, no golden data in
@@ -131,6 +139,31 @@ class Ops:
         if self.ndim == 3:
             v *= conf.dz
         return v
+
+
+class AltOps(Ops):
+    """The same transforms by a second route (2D), for measuring the round-off floor that
+    separates two correct FFT implementations: r2c through the full complex FFT, c2r
+    through a c2c over y and a Hermitian extension along x inverted by a complex FFT (Im of
+    the DC and Nyquist bins dropped, the c2r semantics), full complex transforms through
+    the conjugation identity."""
+
+    def rfft(self, u):
+        return sfft.fftn(np.asarray(u, dtype=complex), axes=self.axes, workers=self.workers)[..., : self.Nx // 2 + 1]
+
+    def irfft(self, u_hat):
+        nx = self.Nx
+        y = sfft.ifftn(u_hat, axes=self.axes[:-1], workers=self.workers).copy()
+        y[..., 0] = y[..., 0].real
+        y[..., nx // 2] = y[..., nx // 2].real
+        full = np.concatenate([y, np.conj(y[..., 1: nx // 2][..., ::-1])], axis=-1)
+        return sfft.ifft(full, axis=-1, workers=self.workers).real
+
+    def fft(self, u):
+        return np.conj(sfft.ifftn(np.conj(u), axes=self.axes, workers=self.workers, norm="forward"))
+
+    def ifft(self, u_hat):
+        return np.conj(sfft.fftn(np.conj(u_hat), axes=self.axes, workers=self.workers, norm="forward"))
 
 
 def _bcast_n(v, ndim):
@@ -380,11 +413,13 @@ def _knrm2(x):
     return float(np.linalg.norm(x))
 
 
-def bicgstab(A, b, M, atol, rtol, itmax, history=True):
+def bicgstab(A, b, M, atol, rtol, itmax, history=True, callback=None):
     """BiCGSTAB of van der Vorst (1992) in the form Krylov.jl 0.10.6 implements.
 
     r0 = M b, p = r0, c = b, rho = <c, r>, eps = atol + rtol*||r0||.
     Stops on ||r|| <= eps (or ||r|| + 1 <= 1), iter >= itmax, or alpha == 0/NaN.
+    ``callback(k, dict)``, if given, sees every iteration's vectors (x, r, s, t, v, p) and
+    scalars (alpha, omega, rho, next_rho) after the iteration, for the identity tests.
     """
     x = np.zeros_like(b)
     r = M(b)
@@ -424,6 +459,9 @@ def bicgstab(A, b, M, atol, rtol, itmax, history=True):
         p = p - omega * v
         p = r + beta * p
         rnorm = _knrm2(r)
+        if callback is not None:
+            callback(it, dict(x=x, r=r, s=s, t=t, v=v, p=p, c=c, alpha=alpha, omega=omega, rho=rho,
+                              next_rho=next_rho))
         if history:
             stats.residuals.append(rnorm)
             stats.alphas.append(alpha)

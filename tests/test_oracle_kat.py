@@ -390,3 +390,108 @@ def test_conv_oracle_conv_is_a_periodic_correlation():
     assert out[0, 0, i, j] == pytest.approx(ref, rel=1e-14)
     assert co.gelu(np.array([0.0]))[0] == 0.0
     assert co.gelu(np.array([30.0]))[0] == pytest.approx(30.0)
+
+
+# --- stripe closed forms for H1 and the whole of H2 -------------------------
+def _stripe(jv, so, delta, **kw):
+    """phi = tanh((x - x1)/(sqrt2 delta)) - tanh((x - x2)/(sqrt2 delta)) - 1: two straight
+    interfaces pi apart, each within exp(-pi/(sqrt2 delta)) of the single-interface profile
+    (delta small enough that the slope mismatch of the tails at x = 0 = 2 pi is ~1e-16)."""
+    conf = _conf(jv, n=512, eps=0.08, **kw)
+    o = so.Ops(conf)
+    x = np.arange(conf.Nx) * conf.dx
+    s2d = math.sqrt(2) * delta
+    x1, x2 = math.pi / 2, 3 * math.pi / 2
+    prof = np.tanh((x - x1) / s2d) - np.tanh((x - x2) / s2d) - 1.0
+    phi = np.broadcast_to(prof[None, :], (conf.Ny, conf.Nx))[None].copy()
+    return conf, o, phi
+
+
+def test_H1_closed_form_on_a_stripe_with_A0_not_A(jv, so):
+    """On phi = tanh(x/(sqrt2 delta)), delta != eps: phi'' = (phi^3 - phi)/delta^2, so
+    omega = (phi^3 - phi)(1/eps - eps/delta^2), and the surface functional per interface
+    and unit length is delta/(2 eps) + sqrt2 eps/(3 delta).  H1 (Utils.jl:198-206) is then
+    gamma_area (A - A0)(3 sqrt2/4) omega / sqrt(W1 + C1), W1 = gamma_area (A - A0)^2 / 2."""
+    delta = 0.06
+    conf, o, phi = _stripe(jv, so, delta, gamma_area=1000, C1=3.0)
+    eps = conf.epsilon
+    A = 2 * conf.Ly * (delta / (2 * eps) + math.sqrt(2) * eps / (3 * delta))
+    assert so.calculate_area(phi, o, conf)[0] == pytest.approx(A, rel=1e-9)
+    A0 = np.array([0.97 * A])
+    W1 = 0.5 * conf.gamma_area * (A - A0[0]) ** 2
+    assert so.get_W1(phi, o, conf, A0) == pytest.approx(W1, rel=1e-8)
+    omega = (phi ** 3 - phi) * (1 / eps - eps / delta ** 2)
+    np.testing.assert_allclose(so.get_omega(phi, o, conf), omega, rtol=0, atol=1e-9 * np.max(np.abs(omega)))
+    H1 = conf.gamma_area * (A - A0[0]) * (3 * math.sqrt(2) / 4) * omega / math.sqrt(W1 + conf.C1)
+    got = so.get_H1(phi, o, conf, A0)
+    assert np.max(np.abs(got - H1)) <= 1e-8 * np.max(np.abs(H1))
+    # the sign follows A - A0: an area above its target gives H1 of omega's sign
+    assert np.sum(got * omega) > 0
+
+
+def test_H2_surface_and_bending_closed_form_on_a_stripe(jv, so):
+    """H2 (Utils.jl:208-226) with gamma_s, gamma_b != 0 and S1..S3 = 0 on the stripe:
+    lap omega = c phi (1 - phi^2)(4 - 6 phi^2)/delta^2 with c = 1/eps - eps/delta^2, and
+    int omega^2 dx per interface = c^2 sqrt2 delta 4/15, which fixes W2 too."""
+    delta = 0.06
+    conf, o, phi = _stripe(jv, so, delta, gamma_surf=1.3, gamma_bend=0.2, S1=0.0, S2=0.0, S3=0.0, C2=50.0)
+    eps = conf.epsilon
+    c = 1 / eps - eps / delta ** 2
+    A = 2 * conf.Ly * (delta / (2 * eps) + math.sqrt(2) * eps / (3 * delta))
+    Fb = (3 * math.sqrt(2) / (16 * eps)) * 2 * conf.Ly * c ** 2 * math.sqrt(2) * delta * 4 / 15
+    W2 = conf.gamma_surf * A + conf.gamma_bend * Fb
+    assert so.get_W2(phi, o, conf) == pytest.approx(W2, rel=1e-8)
+    omega = c * (phi ** 3 - phi)
+    lap_omega = c * phi * (1 - phi ** 2) * (4 - 6 * phi ** 2) / delta ** 2
+    H2 = (conf.gamma_surf * (3 * math.sqrt(2) / 4) * omega
+          + conf.gamma_bend * (3 * math.sqrt(2) / (8 * eps)) * (omega * (3 * phi ** 2 - 1) / eps - eps * lap_omega)
+          ) / math.sqrt(W2 + conf.C2)
+    got = so.get_H2(phi, o, conf)
+    assert np.max(np.abs(got - H2)) <= 1e-8 * np.max(np.abs(H2))
+    # the surface term alone (gamma_b = 0): a wrong 3 sqrt2/4 or sign would fail here
+    conf_s = conf.with_(gamma_bend=0.0)
+    H2s = conf.gamma_surf * (3 * math.sqrt(2) / 4) * omega / math.sqrt(conf.gamma_surf * A + conf.C2)
+    assert np.max(np.abs(so.get_H2(phi, o, conf_s) - H2s)) <= 1e-8 * np.max(np.abs(H2s))
+
+
+# --- BiCGSTAB identities at every iteration ---------------------------------
+def test_bicgstab_identities_every_iteration(jv, so):
+    """On the 8x8 psi operator, after every iteration k (also at the itmax stop):
+    r_k = M (b - A x_k) (the recurrence residual is the preconditioned true residual),
+    <c, s_k> = 0 (alpha = rho / <c, v>), <t_k, r_k> = 0 (omega = <t, s>/<t, t>, first
+    argument conjugated), rho_{k+1} = <c, r_k>."""
+    conf, orc, phi = _psi_setup(jv, so, n=8)
+    o = orc.ops
+    Mpsi = so.get_M_psi(phi, conf)
+    a, dt = 1.5, conf.dt
+    shape = phi.shape
+    D1f = o.D1_full
+
+    def A(xv):
+        xh = xv.reshape(shape)
+        acc = sum(D1f[d] * o.fft(Mpsi * o.ifft(D1f[d] * (conf.S4 * xh))) for d in range(2))
+        return ((a / dt) * xh - acc).reshape(-1)
+    Pd = (1.0 / (a / dt - float(Mpsi.max()) * o.Laplacian_full * conf.S4)).reshape(-1)
+    M = lambda v: Pd * v
+    rng = np.random.default_rng(17)
+    b = o.fft(rng.standard_normal(shape)).reshape(-1)
+    seen = []
+
+    def cb(k, d):
+        x, r, s, t, c = d["x"], d["r"], d["s"], d["t"], d["c"]
+        scale = np.linalg.norm(M(b))
+        assert np.linalg.norm(r - M(b - A(x))) <= 1e-12 * scale, k
+        assert abs(np.vdot(c, s)) <= 1e-12 * np.linalg.norm(c) * np.linalg.norm(s), k
+        assert abs(np.vdot(t, r)) <= 1e-12 * np.linalg.norm(t) * max(np.linalg.norm(r), 1e-300), k
+        assert abs(np.vdot(c, r) - d["next_rho"]) <= 1e-12 * abs(d["next_rho"]) + 1e-300, k
+        seen.append(k)
+
+    x, st = so.bicgstab(A, b, M, 1e-14, 1e-10, 20, callback=cb)
+    assert st.solved and seen == list(range(1, st.niter + 1)) and st.niter >= 3
+    # stopped at itmax = 2: the same identities hold for the committed iterate
+    seen.clear()
+    x2, st2 = so.bicgstab(A, b, M, 1e-14, 1e-10, 2, callback=cb)
+    assert st2.niter == 2 and not st2.solved and seen == [1, 2]
+    # and that iterate is the first two iterates of the converging run
+    x_a, _ = so.bicgstab(A, b, M, 1e-14, 1e-10, 2)
+    assert np.array_equal(x_a, x2)
