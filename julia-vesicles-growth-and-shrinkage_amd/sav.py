@@ -790,13 +790,14 @@ def read_checkpoint(path: os.PathLike):
 def run_simulation(dt_val: float, T_val: float, state_type: int, save_path: Optional[str] = None,
                    save_interval: float = 0.001, conf: Optional[Config] = None,
                    state: Optional[FieldState] = None, t_nodes: Optional[np.ndarray] = None,
-                   precision: int = SAV_PREC_FP64, checkpoint: bool = False):
+                   precision: int = SAV_PREC_FP64, checkpoint: bool = False, plots: bool = True):
     """Returns (present, energy_history, area_ratio_history, Dt) like simulation.jl:154.
 
     With ``conf``/``state``/``t_nodes`` left out it runs the reference's own set-up
     (set_para_base, generate_initial_condition, two-phase Dt schedule).  With
     ``save_path`` it writes the reference's snapshots (``save_path/MAT/case3/``, the
-    schedule of simulation.jl:113-114; no plots) and, with ``checkpoint``, a resumable
+    schedule of simulation.jl:113-114), with ``plots`` the 2D phi image of each snapshot
+    (``save_path/phi_t<t>.png``, simulation.jl:130-135) and, with ``checkpoint``, a resumable
     checkpoint next to each snapshot."""
     conf = conf or set_para_base(dt_val, T_val)
     ops = build_operators(conf, precision=precision)
@@ -822,6 +823,12 @@ def run_simulation(dt_val: float, T_val: float, state_type: int, save_path: Opti
             if n in saves:
                 st = ops.get_state(SAV_PRESENT)
                 write_snapshot(mat_folder / snapshot_name(conf, float(Dt[n - 1])), st)
+                if plots and conf.ndim == 2:
+                    from .plots import plot_field, phi_sum, _plt
+                    t_str = f"{float(Dt[n - 1]):.2e}"
+                    plot_field(phi_sum(st.phi, conf.N), conf, title=f"phi  (t = {t_str})",
+                               filename=str(Path(save_path) / f"phi_t{t_str}.png"))
+                    _plt().close("all")
                 if checkpoint:
                     write_checkpoint(mat_folder / (snapshot_name(conf, float(Dt[n - 1]))[:-4] + ".ckpt.npz"),
                                      st, ops.get_state(SAV_OLD), n + 1, float(Dt[n]))
