@@ -102,7 +102,8 @@ def run_case_learned(jv, wl, precision, steps, warmup, collect_steps=4, train_it
     Adam training on the GPU, then `warmup` + `steps` steps with the network (timed)."""
     conf = wl.conf
     ops = _setup(jv, wl, precision)
-    ops.precond_collect(True, patches_per_snapshot=32, seed=17)
+    # about 20k windows: 4 steps x ~19 diagonal iterations x 256 windows per iterate
+    ops.precond_collect(True, patches_per_snapshot=256, seed=17)
     t = np.arange(collect_steps + warmup + steps + 1) * conf.dt
     rc, E0, A0, kit_diag = ops.run(t[: collect_steps + 1], 1)
     ops.precond_collect(False)

@@ -22,6 +22,8 @@ def test_run_simulation_matches_oracle_over_the_reference_schedule(jv, so, tmp_p
     present, E, A, Dt_out = jv.run_simulation(conf.dt, T, 1, conf=conf, save_path=str(tmp_path),
                                               save_interval=5e-4)
     assert np.array_equal(Dt_out, Dt)
+    # the phi image of every 2D snapshot (simulation.jl:130-135)
+    assert len(list(tmp_path.glob("phi_t*.png"))) == len(jv.save_steps(Dt, 5e-4))
     phi, psi = jv.initial_fields_2d(conf, 1)
     orc = so.Oracle(conf)
     pres, old = orc.init_sav(so.state_from_physical(phi.T, psi.T, orc.ops, 2))
