@@ -89,8 +89,10 @@ void matvec_plain_2d(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int slot);
 // Solvers.jl:73-75, taken on the true residual) or max_rounds rounds.  k32 shares the
 // operator tables, k.ds and the partials buffer; k32.M holds M_psi in FP32.  k.hks[slot+z]
 // returns: iter = inner iterations of all rounds, rounds, rnorm = final true residual,
-// solved.  Returns the largest inner iteration total.
-int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int nsys, int max_rounds);
+// solved.  With `warm` the solve starts from the iterate k.x already holds (the previous
+// solve's solution) instead of 0: one more FP64 matvec, fewer inner iterations; the stopping
+// test is the same.  Returns the largest inner iteration total.
+int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int nsys, int max_rounds, bool warm);
 
 // Flexible right-preconditioned BiCGSTAB with preconditioner `prec` (see krylov.hip).
 template <typename R>

@@ -394,6 +394,9 @@ __global__ __launch_bounds__(256) void k_kry_fin(const double* __restrict__ part
       k.status = 4;
     } else {
       k.eps = atol + rtol * k.rnorm;
+      // inner solve of a mixed-precision system: no need to go below half the outer
+      // threshold (the round that reaches it is the last one)
+      if (outer != nullptr) k.eps = fmax(k.eps, 0.5 * outer[blockIdx.x].eps);
       k.nr = acc[0];
       k.ni = acc[1];
       if (k.nr == 0.0 && k.ni == 0.0) {
@@ -957,6 +960,7 @@ __global__ __launch_bounds__(256) void k_ref_resid(const cplx<double>* __restric
     const double rx = bb.x - aa.x, ry = bb.y - aa.y;
     r32[i] = cplx<float>{(float)rx, (float)ry};
     acc[0] += pv * pv * (rx * rx + ry * ry);
+    acc[1] += pv * pv * (bb.x * bb.x + bb.y * bb.y);
   }
   block_sum<3>(acc, scratch);
   if (threadIdx.x == 0) {
@@ -968,21 +972,35 @@ __global__ __launch_bounds__(256) void k_ref_resid(const cplx<double>* __restric
 
 // Outer scalars, one block per system.  stage 0: ||P^{-1} b||, the threshold
 // atol + rtol ||P^{-1} b||; stage 1 (after a round): the true residual norm, the inner
-// iterations of the round, the stopping tests.
+// iterations of the round, the stopping tests; stage 2 (warm start, after k_ref_resid on
+// x0): ||P^{-1}(b - A x0)|| and the threshold from ||P^{-1} b||; stage -1: reset.
 __global__ __launch_bounds__(256) void k_ref_fin(const double* __restrict__ partials, int nb, KScal* __restrict__ ks,
                                                  const KScal* __restrict__ inner, int stage, double atol, double rtol,
                                                  int max_rounds, long long ps) {
   __shared__ double scratch[3 * 4];
   ks += blockIdx.x;
   partials += blockIdx.x * ps;
-  if (stage != 0 && ks->done) return;
+  if (stage < 0) {  // reset: every system iterates (before a warm-started residual)
+    if (threadIdx.x == 0) *ks = KScal{};
+    return;
+  }
+  if (stage == 1 && ks->done) return;
   double acc[3] = {0.0, 0.0, 0.0};
   strided_sums<3, 256>(partials, nb, threadIdx.x, acc);
   block_sum<3>(acc, scratch);
   if (threadIdx.x != 0) return;
   KScal k = *ks;
   const double rn = sqrt(acc[0]);
-  if (stage == 0) {
+  if (stage == 2) {  // warm start: rn = ||P (b - A x0)||, the threshold from ||P b||
+    k = KScal{};
+    k.rnorm = rn;
+    k.eps = atol + rtol * sqrt(acc[1]);
+    if (rn <= k.eps) {
+      k.done = 1;
+      k.solved = 1;
+      k.status = rn == 0.0 ? 4 : 1;
+    }
+  } else if (stage == 0) {
     k = KScal{};
     k.rnorm = rn;
     k.eps = atol + rtol * rn;
@@ -1011,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_ref_fin(const double* __restrict__ part
   *ks = k;
 }
 
-int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int nsys, int max_rounds) {
+int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int nsys, int max_rounds, bool warm) {
   if (k32.vs != k.vs || k32.nfull != k.nfull || k32.st != k.st)
     throw HipError("bicgstab_refined: the FP32 context does not match the FP64 one", -1);
   autotune_columns(k);
@@ -1020,10 +1038,21 @@ int bicgstab_refined(KrylovCtx<double>& k, KrylovCtx<float>& k32, int slot, int 
   const int nbv = vec_blocks(n);
   KScal* ks = k.ks + slot;
   k32.outer = k.ks;
-  hipLaunchKernelGGL(k_ref_start, dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.x, k32.b, n, k.Nx, k.Ny, k.S4, k.kx0,
-                     k.ky, k.ds, k.partials, vs, ps);
-  hipLaunchKernelGGL(k_ref_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nbv, ks, k32.ks + slot, 0, k.atol,
-                     k.rtol, max_rounds, ps);
+  if (warm) {
+    // x0 = the iterate k.x holds (the previous solve's solution): r = b - A x0 in FP64
+    hipLaunchKernelGGL(k_ref_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, 0, ks, k32.ks + slot, -1, k.atol,
+                       k.rtol, max_rounds, ps);
+    matvec_launch<double>(k, k.x, k.t, 4, slot, nullptr, nsys);
+    hipLaunchKernelGGL(k_ref_resid, dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.t, k32.b, n, k.Nx, k.Ny, k.S4,
+                       k.kx0, k.ky, k.ds, ks, k.partials, vs, ps);
+    hipLaunchKernelGGL(k_ref_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nbv, ks, k32.ks + slot, 2, k.atol,
+                       k.rtol, max_rounds, ps);
+  } else {
+    hipLaunchKernelGGL(k_ref_start, dim3(nbv, nsys), dim3(256), 0, k.st, k.b, k.x, k32.b, n, k.Nx, k.Ny, k.S4,
+                       k.kx0, k.ky, k.ds, k.partials, vs, ps);
+    hipLaunchKernelGGL(k_ref_fin, dim3(nsys), dim3(256), 0, k.st, k.partials, nbv, ks, k32.ks + slot, 0, k.atol,
+                       k.rtol, max_rounds, ps);
+  }
   SAV_LAUNCH_CHECK();
   for (int round = 0; round < max_rounds; ++round) {
     // inner FP32 solve of A e = r on the systems still iterating (the others start done)

@@ -344,6 +344,8 @@ class Solver final : public ISolver {
           kc32.tw_cf = upload(
               kmajor_twiddles<float>(Ny, col_plan_ttmax<float>(Ny, true), col_plan_maxr<float>(Ny, true), false));
           d_M32 = dalloc<float>(nf);
+          x_prev = dalloc<C>(3 * nf);
+          x_prev2 = dalloc<C>(3 * nf);
           kc32.M = d_M32;
           kc32.ds = ds;
           kc32.S4 = c.S4;
@@ -667,9 +669,46 @@ class Solver final : public ISolver {
     float* m32 = d_M32;
     foreach_n(s, (long long)N * P, [=] __device__(long long i) { m32[i] = (float)M[i]; });
   }
+  // The step's three solves start from the solutions of the previous steps (kc.x holds the
+  // last ones x_n, x_prev x_{n-1}, x_prev2 x_{n-2}): from x_n after one step, from 2 x_n - x_{n-1}
+  // after two, from 3 x_n - 3 x_{n-1} + x_{n-2} after three.  A single-system solve
+  // (sav_psi_solve) starts from 0 and drops the history.
+  int mixed_hist = 0;
+  C* x_prev = nullptr;
+  C* x_prev2 = nullptr;
   int refined_solve(int slot, int nsys) {
     if constexpr (std::is_same<R, double>::value) {
-      return bicgstab_refined(kc, kc32, slot, nsys, MIXED_ROUNDS);
+      const long long n3 = 3 * kc.nfull;
+      if (nsys == 3 && mixed_hist >= 3) {
+        C* x2 = x_prev2;
+        const C *xc = kc.x, *x1 = x_prev;
+        foreach_n(s, n3, [=] __device__(long long i) {
+          const CD a = xc[i], b = x1[i], c0 = x2[i];
+          x2[i] = CD{3.0 * (a.x - b.x) + c0.x, 3.0 * (a.y - b.y) + c0.y};
+        });
+        // kc.x: the extrapolation; x_prev: x_n; x_prev2: x_{n-1}
+        C* e = x_prev2;
+        x_prev2 = x_prev;
+        x_prev = kc.x;
+        kc.x = e;
+      } else if (nsys == 3 && mixed_hist == 2) {
+        C* x2 = x_prev2;
+        const C *xc = kc.x, *x1 = x_prev;
+        foreach_n(s, n3, [=] __device__(long long i) {
+          const CD a = xc[i], b = x1[i];
+          x2[i] = CD{2.0 * a.x - b.x, 2.0 * a.y - b.y};
+        });
+        C* e = x_prev2;
+        x_prev2 = x_prev;
+        x_prev = kc.x;
+        kc.x = e;
+      } else if (nsys == 3 && mixed_hist == 1) {
+        SAV_HIP(hipMemcpyAsync(x_prev, kc.x, n3 * sizeof(C), hipMemcpyDeviceToDevice, s));
+      }
+      const bool warm = nsys == 3 && mixed_hist >= 1;
+      const int it = bicgstab_refined(kc, kc32, slot, nsys, MIXED_ROUNDS, warm);
+      mixed_hist = nsys == 3 ? std::min(mixed_hist + 1, 3) : 0;
+      return it;
     } else {
       throw HipError("the mixed-precision solve needs FP64 storage", SAV_E_UNSUPPORTED);
     }
