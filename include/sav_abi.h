@@ -173,6 +173,16 @@ int sav_modified_energy(sav_handle_t* h, double* energy, double* area_ratio);
 int sav_set_column_tile(sav_handle_t* h, int columns);
 int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16);
 
+/* Autotuner of every other column (strided-axis) transform (no reference counterpart): at a
+ * handle's first step an irfft and an rfft of the present phi are timed with 4, 8 and 16
+ * columns per workgroup (widths that resolve to the same tile for this grid are timed once)
+ * and the fastest is kept; all widths compute bit-identical results.
+ * sav_set_transform_tile(h, 4 | 8 | 16) fixes the width, 0 retunes at the next step.
+ * sav_transform_tile_info returns the width in use (0 before the first step) and the measured
+ * milliseconds of each width (-1: same tile as a smaller width, 0: not timed yet). */
+int sav_set_transform_tile(sav_handle_t* h, int columns);
+int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16);
+
 /* Graph replay of steps (no reference counterpart; the loop body of src/simulation.jl:73-110
  * as one HIP graph).  With on = 1, sav_step and sav_run capture a whole step (steps 1-7,
  * the back-transform of the new state, the monitor) once per buffer rotation and psi

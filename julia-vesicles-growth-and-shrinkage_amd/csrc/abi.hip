@@ -235,6 +235,21 @@ int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* m
   SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16); if (columns) *columns = cw; return SAV_OK);
 }
 
+int sav_set_transform_tile(sav_handle_t* h, int columns) {
+  if (!h) return SAV_E_ARG;
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->set_transform_tile(columns));
+  return rc;
+}
+
+int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16) {
+  if (!h) return SAV_E_ARG;
+  int cw = 0;
+  SAV_GUARD(h, h->s->transform_tile(&cw, ms_4, ms_8, ms_16));
+  if (columns) *columns = cw;
+  return SAV_OK;
+}
+
 int sav_set_graph(sav_handle_t* h, int on) {
   if (!h) return SAV_E_ARG;
   SAV_GUARD(h, return h->s->set_graph(on));

@@ -487,7 +487,12 @@ struct FftGeom {
   const cplx<R>* tw_y;    // table for L = Ny
   const cplx<R>* tw_z;    // table for L = Nz
   KTables k;
+  int colw = 0;           // columns per workgroup of the column transforms: 0 = default rule, 4 / 8 / 16
 };
+
+// Columns per workgroup that width cw (0, 4, 8, 16) resolves to for a column transform of
+// length L with real_bytes-byte values (the LDS and thread limits can cap it).
+int col_tile_columns(int L, int real_bytes, int cw);
 
 // rfft over the spatial axes: dst_j = rfft(src_j), count_j fields each.
 template <typename R>

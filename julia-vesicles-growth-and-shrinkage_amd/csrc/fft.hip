@@ -9,8 +9,25 @@ namespace sav {
 // workgroup per CU; with 4 columns two workgroups share a CU and overlap their load, transform
 // and store phases (1024^2: 60.8 -> 58.9 us inverse, 28.6 -> 24.5 us forward, per launch).
 // FP32 keeps 8 (4 columns: 35.7 -> 39.0 us inverse).
+// The width is also tunable at run time (FftGeom::colw, 4 / 8 / 16; ColCfgW), since each
+// column's transform is the same whichever tile it shares: the results are bit-identical.
+template <typename R, int L, int CW = 0>
+using ColCfgW = ColTile<R, L, CW != 0 ? CW : ((sizeof(R) == 8 && L >= 512) ? 4 : 8), cols_maxr(L, sizeof(R) == 4)>;
 template <typename R, int L>
-using ColCfg = ColTile<R, L, (sizeof(R) == 8 && L >= 512) ? 4 : 8, cols_maxr(L, sizeof(R) == 4)>;
+using ColCfg = ColCfgW<R, L, 0>;
+
+int col_tile_columns(int L, int real_bytes, int cw) {
+  switch (L) {
+#define X(LL)                                                                                              \
+  case LL:                                                                                                 \
+    if (real_bytes == 8)                                                                                   \
+      return cw == 4 ? ColCfgW<double, LL, 4>::C : cw == 16 ? ColCfgW<double, LL, 16>::C : cw == 8 ? ColCfgW<double, LL, 8>::C : ColCfgW<double, LL, 0>::C; \
+    return cw == 4 ? ColCfgW<float, LL, 4>::C : cw == 16 ? ColCfgW<float, LL, 16>::C : cw == 8 ? ColCfgW<float, LL, 8>::C : ColCfgW<float, LL, 0>::C;
+    X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+#undef X
+    default: return 0;
+  }
+}
 
 // Rows handled per workgroup for a contiguous transform of length L: one wavefront per
 // workgroup (measured faster than four rows per 256-thread workgroup), or one row when
@@ -39,11 +56,11 @@ struct ColGeom {
 // ---------------------------------------------------------------------------
 // Column transform (strided axis), several adjacent columns per workgroup.
 // ---------------------------------------------------------------------------
-template <typename R, int L, bool INV>
-__global__ __launch_bounds__((ColCfg<R, L>::NT)) void k_cols(SpecJobs jobs, ColGeom cg,
-                                                          const cplx<R>* __restrict__ tw, KTables kt) {
+template <typename R, int L, bool INV, int CW>
+__global__ __launch_bounds__((ColCfgW<R, L, CW>::NT)) void k_cols(SpecJobs jobs, ColGeom cg,
+                                                               const cplx<R>* __restrict__ tw, KTables kt) {
   using F = TeamFFT<R, L, INV, (sizeof(R) == 8 ? 128 : 64), 0, default_twm<R>(), 0, cols_maxr(L, sizeof(R) == 4)>;
-  constexpr int C = ColCfg<R, L>::C;
+  constexpr int C = ColCfgW<R, L, CW>::C;
   constexpr int E = F::E;
   __shared__ cplx<R> lds[C * L];
   const BlockIdx3 bk = xcd_block();
@@ -412,16 +429,16 @@ static int max_count(const SpecJobs& jobs) {
   return m;
 }
 
-template <typename R, bool INV>
-static void launch_cols(int L, const SpecJobs& jobs, const ColGeom& cg, const cplx<R>* tw, const KTables& kt,
-                        hipStream_t s) {
+template <typename R, bool INV, int CW>
+static void launch_cols_w(int L, const SpecJobs& jobs, const ColGeom& cg, const cplx<R>* tw, const KTables& kt,
+                          hipStream_t s) {
   const int mc = max_count(jobs);
   switch (L) {
 #define X(LL)                                                                                 \
   case LL: {                                                                                  \
-    using CC = ColCfg<R, LL>;                                                                 \
+    using CC = ColCfgW<R, LL, CW>;                                                            \
     dim3 grid((cg.inner + CC::C - 1) / CC::C, mc * cg.nsub, jobs.n);                          \
-    hipLaunchKernelGGL((k_cols<R, LL, INV>), grid, dim3(CC::NT), 0, s, jobs, cg, tw, kt);     \
+    hipLaunchKernelGGL((k_cols<R, LL, INV, CW>), grid, dim3(CC::NT), 0, s, jobs, cg, tw, kt); \
     break;                                                                                    \
   }
     SAV_FFT_SIZES(X)
@@ -429,6 +446,20 @@ static void launch_cols(int L, const SpecJobs& jobs, const ColGeom& cg, const cp
     default: throw HipError("unsupported FFT length " + std::to_string(L), -1);
   }
   SAV_LAUNCH_CHECK();
+}
+
+// Column transforms with the tile width chosen for the handle (0: the default rule).
+template <typename R, bool INV>
+static void launch_cols(int L, const SpecJobs& jobs, const ColGeom& cg, const cplx<R>* tw, const KTables& kt,
+                        hipStream_t s, int colw) {
+  if (colw == 4)
+    launch_cols_w<R, INV, 4>(L, jobs, cg, tw, kt, s);
+  else if (colw == 8)
+    launch_cols_w<R, INV, 8>(L, jobs, cg, tw, kt, s);
+  else if (colw == 16)
+    launch_cols_w<R, INV, 16>(L, jobs, cg, tw, kt, s);
+  else
+    launch_cols_w<R, INV, 0>(L, jobs, cg, tw, kt, s);
 }
 
 template <typename R>
@@ -560,18 +591,18 @@ void rfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs, hipStream_t s) {
     launch_plane<R>(g, jobs, false, nullptr, 1.0, s);
     ColGeom cz = geom_z(g);
     cz.use_src = 0;
-    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s);
+    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s, g.colw);
     return;
   }
   const int rows = g.Ny * (g.ndim == 3 ? g.Nz : 1);
   launch_r2c<R>(g.Nx / 2, jobs, rows, g.tw_xh, g.tw_x2, s);
   ColGeom cy = geom_y(g);
   cy.use_src = 0;
-  launch_cols<R, false>(g.Ny, jobs, cy, g.tw_y, g.k, s);
+  launch_cols<R, false>(g.Ny, jobs, cy, g.tw_y, g.k, s, g.colw);
   if (g.ndim == 3) {
     ColGeom cz = geom_z(g);
     cz.use_src = 0;
-    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s);
+    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s, g.colw);
   }
 }
 
@@ -591,18 +622,18 @@ void irfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs_in, cplx<R>* tmp, hipS
   if (g.ndim == 2) {
     ColGeom cy = geom_y(g);
     cy.use_src = 1;
-    launch_cols<R, true>(g.Ny, tj, cy, g.tw_y, g.k, s);
+    launch_cols<R, true>(g.Ny, tj, cy, g.tw_y, g.k, s, g.colw);
   } else {
     ColGeom cz = geom_z(g);
     cz.use_src = 1;
-    launch_cols<R, true>(g.Nz, tj, cz, g.tw_z, g.k, s);
+    launch_cols<R, true>(g.Nz, tj, cz, g.tw_z, g.k, s, g.colw);
     if (plane_path(g)) {
       launch_plane<R>(g, jobs, true, tmp, 1.0 / ((double)g.Nx * g.Ny * g.Nz), s);
       return;
     }
     ColGeom cy = geom_y(g);
     cy.use_src = 0;
-    launch_cols<R, true>(g.Ny, tj, cy, g.tw_y, g.k, s);
+    launch_cols<R, true>(g.Ny, tj, cy, g.tw_y, g.k, s, g.colw);
   }
   const int rows = g.Ny * (g.ndim == 3 ? g.Nz : 1);
   const double scale = 1.0 / ((double)g.Nx * g.Ny * (g.ndim == 3 ? g.Nz : 1));
@@ -626,9 +657,9 @@ void fft_full_2d(const FftGeom<R>& g, cplx<R>* data, int count, bool inverse, hi
   c.use_src = 0;
   if (!inverse) {
     launch_rows_c2c<R, false>(g.Nx, data, nrows, 1.0, g.tw_x, s);
-    launch_cols<R, false>(g.Ny, jobs, c, g.tw_y, g.k, s);
+    launch_cols<R, false>(g.Ny, jobs, c, g.tw_y, g.k, s, g.colw);
   } else {
-    launch_cols<R, true>(g.Ny, jobs, c, g.tw_y, g.k, s);
+    launch_cols<R, true>(g.Ny, jobs, c, g.tw_y, g.k, s, g.colw);
     launch_rows_c2c<R, true>(g.Nx, data, nrows, 1.0 / ((double)g.Nx * g.Ny), g.tw_x, s);
   }
 }

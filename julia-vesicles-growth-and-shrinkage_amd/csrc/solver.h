@@ -82,6 +82,9 @@ class ISolver {
   // psi matvec column-tile autotuner (2D)
   virtual int set_column_tile(int cw) = 0;
   virtual void column_tile(int* cw, double* ms8, double* ms16) = 0;
+  // column-transform tile autotuner (every transform outside the psi matvec)
+  virtual int set_transform_tile(int cw) = 0;
+  virtual void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) = 0;
   virtual int set_graph(int on) = 0;
   // Eager steps run step 1's advection terms and step 4 on a second stream (on by default);
   // ensemble members, which overlap with each other, keep one stream each.

@@ -1059,6 +1059,72 @@ class Solver final : public ISolver {
     return SAV_OK;
   }
 
+  // Autotuner of the column transforms outside the psi matvec (steps 1-7, back-transforms,
+  // monitor): at a handle's first step an irfft (with the Laplacian multiplier) and an rfft of
+  // the present phi are timed with 4, 8 and 16 columns per workgroup, and the fastest width is
+  // kept for every later transform of the handle.  Widths that resolve to the same tile for
+  // this grid's column lengths are timed once.  Each column is transformed by the same team
+  // code whatever tile it shares, so every width gives bit-identical results.
+  int tcolw = 0;
+  double tune_t_ms[3] = {0.0, 0.0, 0.0};
+  void autotune_transforms() {
+    if (tcolw != 0) return;
+    const int widths[3] = {4, 8, 16};
+    const int rb = (int)sizeof(R);
+    auto key = [&](int w) {
+      return col_tile_columns(Ny, rb, w) * 10000 + (nd == 3 ? col_tile_columns(Nz, rb, w) : 0);
+    };
+    hipEvent_t a, b;
+    SAV_HIP(hipEventCreate(&a));
+    SAV_HIP(hipEventCreate(&b));
+    double best = 1e300;
+    int best_w = 0;
+    std::vector<int> seen;
+    for (int i = 0; i < 3; ++i) {
+      const int w = widths[i];
+      const int kk = key(w);
+      if (std::find(seen.begin(), seen.end(), kk) != seen.end()) {
+        tune_t_ms[i] = -1.0;  // the same tile as a width already timed
+        continue;
+      }
+      seen.push_back(kk);
+      g.colw = w;
+      auto once = [&]() {
+        irfft({job(fhalf(ip, F_PHI), w_d4, N, MUL_LAP)});
+        rfft({job(fphys(ip, F_PHI), w_phat, N)});
+      };
+      once();  // warm-up
+      SAV_HIP(hipEventRecord(a, s));
+      for (int r = 0; r < 3; ++r) once();
+      SAV_HIP(hipEventRecord(b, s));
+      SAV_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      SAV_HIP(hipEventElapsedTime(&ms, a, b));
+      tune_t_ms[i] = ms / 3.0;
+      if (tune_t_ms[i] < best) {
+        best = tune_t_ms[i];
+        best_w = w;
+      }
+    }
+    SAV_HIP(hipEventDestroy(a));
+    SAV_HIP(hipEventDestroy(b));
+    tcolw = best_w;
+    g.colw = tcolw;
+  }
+  int set_transform_tile(int cw) override {
+    if (cw != 0 && cw != 4 && cw != 8 && cw != 16) return SAV_E_ARG;
+    if (cw != tcolw) drop_graphs();  // captured transforms use the previous width
+    tcolw = cw;
+    g.colw = cw;
+    return SAV_OK;
+  }
+  void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) override {
+    if (cw) *cw = tcolw;
+    if (ms4) *ms4 = tune_t_ms[0];
+    if (ms8) *ms8 = tune_t_ms[1];
+    if (ms16) *ms16 = tune_t_ms[2];
+  }
+
   void column_tile(int* cw, double* ms8, double* ms16) override {
     if (cw) *cw = kc.colw;
     if (ms8) *ms8 = kc.tune_ms[0];
@@ -2165,6 +2231,7 @@ class Solver final : public ISolver {
   }
 
   int step_impl(double dt, int n, sav_step_stats_t* st, bool monitor_on) {
+    autotune_transforms();
     const double a = n == 1 ? 1.0 : 1.5, b = n == 1 ? -1.0 : -2.0, cc = n == 1 ? 0.0 : 0.5;
     present_dt = dt;
     set_bdf(a, b, cc, dt);

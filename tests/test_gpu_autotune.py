@@ -28,3 +28,37 @@ def test_column_tile_choice_does_not_change_results(jv, so, precision):
     ops.close()
     assert np.array_equal(outs[8], outs[16])
     assert np.array_equal(auto, outs[t["columns"]])
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_transform_tile_widths_give_identical_steps(jv, case):
+    """The column transforms outside the psi matvec run with 4, 8 or 16 columns per
+    workgroup (sav_set_transform_tile): two steps give bit-identical states whatever the
+    width, and the autotuner (width 0) picks one of the timed widths."""
+    from helpers import device_state
+    wl = jv.workload_c1(n=96, steps=2) if case == "2d" else jv.workload_c4(n=16, steps=2)
+    conf = wl.conf
+    phi, psi = wl.initial_fields()
+    states = {}
+    for w in (0, 4, 8, 16):
+        ops = jv.Operators(conf)
+        ops.set_state(jv.SAV_PRESENT, device_state(jv, ops, conf, phi, psi))
+        ops.init_sav()
+        if w:
+            ops.set_transform_tile(w)
+        for n in (1, 2):
+            rc, st = ops.step(conf.dt, n)
+            assert rc == 0
+        info = ops.transform_tile_info()
+        states[w] = ops.get_state(jv.SAV_PRESENT)
+        ops.close()
+        if w == 0:
+            print(case, info)
+            assert info["columns"] in (4, 8, 16)
+            timed = [info[f"ms_{c}"] for c in (4, 8, 16)]
+            assert max(timed) > 0 and info[f"ms_{info['columns']}"] > 0
+        else:
+            assert info["columns"] == w
+    for w in (4, 8, 16):
+        for name in ("phi", "psi", "mu", "nu", "u", "p", "phi_hat", "psi_hat"):
+            assert np.array_equal(getattr(states[w], name), getattr(states[0], name)), (w, name)
