@@ -218,6 +218,48 @@ def algorithmic_bytes(conf, s, k):
     return (282 + 88 * k) * u + 94 * v
 
 
+def algorithmic_bytes_built(conf, s, k):
+    """The same model with the ψ iterations counted by the kernels this library launches,
+    per system-iteration and real N-field u: FP64 72u (k_mv_cols_inv with the s / p update
+    on load 10u + 12u, two k_mv_rows 18u, k_mv_cols_fwd 10u + 8u, k_kry_upd2 14u), FP32 76u
+    (separate updates: column inverse 6u + 6u, rows 18u, column forward 10u + 8u, s update 6u,
+    k_kry_upd2 14u, p update 8u), in place of the survey's 88u."""
+    pts = conf.Nx * conf.Ny * (conf.Nz if conf.ndim == 3 else 1)
+    u, v = s * pts * conf.N, s * pts
+    if conf.ndim == 3:
+        return 280 * u + 144 * v
+    return (282 + (72 if s == 8 else 76) * k) * u + 94 * v
+
+
+def cpu_report(jv, steps=20, warm=2):
+    """The FP64 CPU oracle on the host: `steps` steps after `warm` steps of the 1024^2 and
+    128^3 workloads, with scipy.fft on every host core, on 8 and on 1 (the monitor included,
+    as the reference loop runs it every step)."""
+    from oracle import sav_oracle as so
+    info = _host_info()
+    rows = {"host": info}
+    allc = min(os.cpu_count() or 1, 16)
+    for wl in (jv.workload_c3(), jv.workload_c4()):
+        conf = wl.conf
+        phi, psi = wl.initial_fields()
+        row = {}
+        for workers in sorted({allc, 8, 1}, reverse=True):
+            orc = so.Oracle(conf, workers=workers)
+            pres, old = orc.init_sav(so.state_from_physical(phi, psi, orc.ops, conf.ndim))
+            for n in range(1, warm + 1):
+                orc.step(pres, old, conf.dt, n)
+            t0 = time.perf_counter()
+            for n in range(warm + 1, warm + steps + 1):
+                orc.step(pres, old, conf.dt, n)
+                print(f"cpu {wl.name} workers {workers}: step {n} at {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
+            el = time.perf_counter() - t0
+            row[f"fft_workers_{workers}"] = dict(steps_per_s=round(steps / el, 4), seconds=round(el, 1))
+            print(f"cpu {wl.name} workers {workers}: {steps / el:.4f} steps/s", file=sys.stderr, flush=True)
+        rows[wl.name] = row
+    return rows
+
+
 def _run_full(jv, wl, precision):
     """The workload's whole step schedule (uniform dt, BDF1 then BDF2) after the SAV
     initialisation; the steps are timed together."""
@@ -266,6 +308,9 @@ def run_configs(jv, cpu=True):
                              algorithmic_gb_per_step=round(B / 1e9, 4),
                              algorithmic_tb_per_s=round(B * r["steps_per_s"] / 1e12, 3),
                              energy_monotone=bool(np.all(np.diff(r["E"]) <= 0)))
+            if name != "mixed":
+                Bb = algorithmic_bytes_built(conf, sb, k)
+                row[name]["built_kernels_tb_per_s"] = round(Bb * r["steps_per_s"] / 1e12, 3)
             if name != "fp64":
                 row[name]["max_rel_err_vs_fp64"] = {"phi": _rel(st.phi, s64.phi), "psi": _rel(st.psi, s64.psi),
                                                     "u": _rel(st.u, s64.u), "energy": _rel(r["E"], r64["E"])}
@@ -333,6 +378,9 @@ def main():
     ap.add_argument("--configs", action="store_true",
                     help="one row per benchmark configuration at its full step count (a separate report, "
                          "not the headline line)")
+    ap.add_argument("--cpu-report", action="store_true",
+                    help="time the FP64 CPU oracle over 20 steps of the 1024^2 and 128^3 workloads with "
+                         "all host cores, 8 and 1 (a separate report; minutes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain device copy of 256 MiB (x4), the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes")
@@ -356,6 +404,9 @@ def main():
 
     if a.configs:
         print(json.dumps(run_configs(jv, cpu=not a.no_cpu)), flush=True)
+        return
+    if a.cpu_report:
+        print(json.dumps(cpu_report(jv)), flush=True)
         return
     if a.only == "ensemble":
         print(json.dumps(run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64)), flush=True)

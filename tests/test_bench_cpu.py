@@ -30,3 +30,13 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stdout
+
+
+def test_built_kernel_byte_model(jv):
+    import bench
+    c3 = jv.workload_c3().conf
+    u = 8 * c3.Nx * c3.Ny * c3.N
+    # one more psi iteration adds 72 real N-fields in FP64 (76 in FP32) with the built kernels
+    assert bench.algorithmic_bytes_built(c3, 8, 25) - bench.algorithmic_bytes_built(c3, 8, 24) == 72 * u
+    assert bench.algorithmic_bytes_built(c3, 4, 25) - bench.algorithmic_bytes_built(c3, 4, 24) == 76 * u // 2
+    assert bench.algorithmic_bytes_built(c3, 8, 0) == bench.algorithmic_bytes(c3, 8, 0)
