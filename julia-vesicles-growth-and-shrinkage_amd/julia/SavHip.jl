@@ -21,6 +21,9 @@ const SAV_PRESENT = Cint(0)
 const SAV_OLD = Cint(1)
 const SAV_W_KRYLOV_NOT_CONVERGED = Cint(1)
 const SAV_E_WC_NEGATIVE = Cint(-3)
+const SAV_PREC_FP64 = Cint(0)      # the reference's arithmetic
+const SAV_PREC_FP32 = Cint(1)      # FP32 fields and transforms, FP64 reductions
+const SAV_PREC_MIXED = Cint(2)     # FP64 except the psi solve's inner iterations (FP32, FP64 defect correction)
 
 struct SavConfig                      # sav_config_t, field for field
     ndim::Int32; N::Int32; Nx::Int32; Ny::Int32; Nz::Int32
@@ -42,7 +45,8 @@ end
 last_error(h) = unsafe_string(ccall((:sav_last_error, LIB), Cstring, (Ptr{Cvoid},), h))
 
 """Device handle for `conf` (Config of src/Types.jl:27-74): sav_create.
-precision 0 = FP64 (the reference's arithmetic), 1 = FP32 fields with FP64 reductions."""
+precision SAV_PREC_FP64 (the reference's arithmetic), SAV_PREC_MIXED (FP64 except the psi
+solve's inner iterations, same stopping test) or SAV_PREC_FP32 (FP32 fields, FP64 reductions)."""
 function create(conf; precision::Integer=0, itmax::Integer=20, atol::Real=1e-14, device::Integer=0)
     pin = collect(Float64, conf.psi_in_v)
     pout = collect(Float64, conf.psi_out_v)
