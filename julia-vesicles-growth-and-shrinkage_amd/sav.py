@@ -8,7 +8,9 @@ Names follow the reference: ``Config`` (src/Types.jl:27), ``FieldState``
 order, so ``state.phi.shape == (Nx, Ny, N)`` and the memory handed to the C ABI
 is byte-identical to what Julia's ``ccall`` would pass.
 
-Everything numeric runs in the HIP library; this module only marshals arrays.
+Everything numeric in a time step runs in the HIP library; this module marshals arrays and
+prepares inputs on the host (the initial conditions of Init.jl, and the FP64 spectra of initial
+data for FP32-field handles, see ``Operators.rfft_accurate``).
 """
 from __future__ import annotations
 
@@ -366,11 +368,11 @@ class Operators:
         return out
 
     def rfft_accurate(self, u: np.ndarray) -> np.ndarray:
-        """Spectra of initial data: the device transform for an FP64 handle; for an
+        """Spectra of initial data: the device transform for an FP64 or mixed handle; for an
         FP32 handle the FP64 transform is taken on the host (numpy), because the
         FP32 FFT roundoff floor at high k is amplified by the S1 k^4 stabiliser.
         Used to prepare inputs only, never inside a time step."""
-        if self.precision == SAV_PREC_FP64:
+        if self.precision in (SAV_PREC_FP64, SAV_PREC_MIXED):
             return self.rfft(u)
         u = np.asarray(u, dtype=np.float64)
         nd = self.conf.ndim
