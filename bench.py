@@ -90,6 +90,7 @@ def run_case(jv, wl, precision, steps, warmup, timed_kernel=0, graph=False):
         n, ms, b = ops.kernel_timing_get()
         res["kernel"] = dict(name=ops.KERNELS[timed_kernel], launches=n, avg_ms=ms / max(n, 1), bytes=b)
     res["autotune"] = ops.autotune_info() if conf.ndim == 2 else None
+    res["plane_kernel"] = ops.plane_kernel_info() if conf.ndim == 3 else None
     if graph:
         res["graph"] = ops.graph_info()
     st = ops.get_state(jv.SAV_PRESENT)
@@ -520,6 +521,11 @@ def main():
     if "3d" in out:
         o = out["3d"]
         line["fp64_steps_per_s_128cubed"] = round(o["fp64"]["steps_per_s"], 3)
+        pk = o["fp64"]["plane_kernel"]
+        line["plane_kernel_128cubed"] = {
+            "fp64": {"kind": {1: "register columns", 2: "LDS plane"}.get(pk["kind"], pk["kind"]),
+                     "transform_pair_ms_register": round(pk["ms_register"], 4),
+                     "transform_pair_ms_lds": round(pk["ms_lds"], 4)}}
         line["mixed_128cubed"] = {"steps_per_s": round(o["fp64"]["steps_per_s"], 3),
                                   "note": "the 3D psi solve is a spectral divide: the mixed form has no FP32 "
                                           "stage in 3D and runs the FP64 step"}

@@ -183,6 +183,21 @@ int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* m
 int sav_set_transform_tile(sav_handle_t* h, int columns);
 int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16);
 
+/* (x, y) plane kernels of 3D grids with 128 x 128 planes (no reference counterpart; they
+ * carry SpectralUtils_3d.jl's rfft/irfft over x and y).  SAV_PLANE_REGISTER keeps the plane's
+ * columns in registers and stages a 64 KB half plane (FP64) through LDS; SAV_PLANE_LDS holds
+ * the whole half-spectrum plane in LDS.  Both compute the same transform (results agree to
+ * FP round-off, not bit for bit).  With SAV_PLANE_AUTO (the default) the next step times an
+ * irfft and an rfft of the present phi with each and keeps the faster.
+ * sav_plane_kernel_info returns the kind in use (SAV_PLANE_AUTO before it is chosen) and
+ * the measured milliseconds of each (0: not timed). Other grids use SAV_PLANE_LDS where the
+ * plane fits in LDS; sav_set_plane_kernel(h, SAV_PLANE_REGISTER) is SAV_E_UNSUPPORTED there. */
+#define SAV_PLANE_AUTO 0
+#define SAV_PLANE_REGISTER 1
+#define SAV_PLANE_LDS 2
+int sav_set_plane_kernel(sav_handle_t* h, int kind);
+int sav_plane_kernel_info(sav_handle_t* h, int32_t* kind, double* ms_register, double* ms_lds);
+
 /* Graph replay of steps (no reference counterpart; the loop body of src/simulation.jl:73-110
  * as one HIP graph).  With on = 1, sav_step and sav_run capture a whole step (steps 1-7,
  * the back-transform of the new state, the monitor) once per buffer rotation and psi

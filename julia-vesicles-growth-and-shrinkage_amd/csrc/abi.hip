@@ -250,6 +250,21 @@ int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, dou
   return SAV_OK;
 }
 
+int sav_set_plane_kernel(sav_handle_t* h, int kind) {
+  if (!h) return SAV_E_ARG;
+  int rc = 0;
+  SAV_GUARD(h, rc = h->s->set_plane_kernel(kind));
+  return rc;
+}
+
+int sav_plane_kernel_info(sav_handle_t* h, int32_t* kind, double* ms_register, double* ms_lds) {
+  if (!h) return SAV_E_ARG;
+  int k = 0;
+  SAV_GUARD(h, h->s->plane_kernel(&k, ms_register, ms_lds));
+  if (kind) *kind = k;
+  return SAV_OK;
+}
+
 int sav_set_graph(sav_handle_t* h, int on) {
   if (!h) return SAV_E_ARG;
   SAV_GUARD(h, return h->s->set_graph(on));

@@ -401,6 +401,30 @@ inline std::vector<cplx<R>> column_twiddles(int L) {
   return kmajor_twiddles<R>(L, 64, cols_maxr(L, sizeof(R) == 4), false);
 }
 
+// Host: the twiddle table of the register-column plane kernels for 128 x 128 planes
+// (k_hplane_r2c / k_hplane_c2r), forward sign, in three parts laid out so that the lanes
+// of a 16-lane LDS access group (two teams) read opposite halves of the 64 banks:
+//   [0, 192)    W_128^{t k1}, t < 8, k1 < 16, at t * 24 + k1    (y pass, 128 = 16 x 8)
+//   [192, 256)  W_64^{s k1},  s < 8, k1 < 8,  at 192 + k1 * 8 + s (x pass, 64 = 8 x 8)
+//   [256, 321)  W_128^k,      k <= 64,        at 256 + k          (the r2c / c2r split)
+constexpr int kHPlaneTw = 321;
+constexpr int kHPlaneTX = 192, kHPlaneT2 = 256;
+template <typename R>
+inline std::vector<cplx<R>> hplane_twiddles() {
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  std::vector<cplx<R>> w(kHPlaneTw, cplx<R>{R(0), R(0)});
+  auto put = [&](int i, int num, int den) {
+    const long double a = -two_pi * (long double)num / (long double)den;
+    w[i] = cplx<R>{(R)cosl(a), (R)sinl(a)};
+  };
+  for (int t = 0; t < 8; ++t)
+    for (int k1 = 0; k1 < 16; ++k1) put(t * 24 + k1, t * k1, 128);
+  for (int s = 0; s < 8; ++s)
+    for (int k1 = 0; k1 < 8; ++k1) put(kHPlaneTX + k1 * 8 + s, s * k1, 64);
+  for (int k = 0; k <= 64; ++k) put(kHPlaneT2 + k, k, 128);
+  return w;
+}
+
 // Columns per workgroup of a strided (column) transform: 8 adjacent columns (128-byte
 // row segments in FP64, 64-byte in FP32), at least 256 threads per workgroup, at most
 // 1024 threads and the 160 KB of LDS of a gfx950 CU.
@@ -488,6 +512,8 @@ struct FftGeom {
   const cplx<R>* tw_z;    // table for L = Nz
   KTables k;
   int colw = 0;           // columns per workgroup of the column transforms: 0 = default rule, 4 / 8 / 16
+  const cplx<R>* tw_hp = nullptr;  // hplane_twiddles() for 128 x 128 planes of 3D fields
+  int plane_kind = 0;     // 128 x 128 planes: 0 = register-column kernels, 1 = whole plane in LDS
 };
 
 // Columns per workgroup that width cw (0, 4, 8, 16) resolves to for a column transform of

@@ -408,6 +408,277 @@ __global__ __launch_bounds__((PlaneCfg<R, N>::NT)) void k_plane_c2r(SpecJobs job
 }
 
 // ---------------------------------------------------------------------------
+// Register-column (x, y) transforms of 128 x 128 planes.
+//
+// The LDS-plane kernels above hold a whole half-spectrum plane (65 x 128 complex: 133 KB
+// in FP64), so one FP64 workgroup fills a CU and its load, transform and store phases
+// cannot overlap with another's.  These kernels keep the columns in registers and use
+// LDS only as a 64 KB half plane (64 rows x 64 columns), so two FP64 workgroups (four
+// FP32 ones) share a CU.
+//
+// 512 threads = 8 wavefronts; lane l of wavefront w is (c, t) = (l & 7, l >> 3) in the
+// column phase and (s, rl) = (l & 7, l >> 3) in the row phase.
+//  * columns: thread (w, c, t) holds column x = 8w + c at rows y = t + 8m (m < 16) and
+//    runs the length-128 y transform as 16 x 8: a 16-point DFT over m in registers, the
+//    twiddles W_128^{t k1}, then for each k1 an 8-point DFT over t after an exchange
+//    through the wavefront's own 8 KB of LDS (two rounds of 8 k1 values), leaving the
+//    rows ky = 8h + t + 16 k2 (h < 2, k2 < 8) in registers.
+//  * the x DC and Nyquist columns (x = 0 and x = 64) travel packed in column 0 as
+//    c0 + i c64: both are real-valued along x after the y transform (c2r: the imaginary
+//    parts of those two bins are ignored; r2c: a real row has real DC and Nyquist bins),
+//    so one complex y transform carries both.  c2r packs the Hermitian parts
+//    (c(ky) + conj c(-ky)) / 2 on load; r2c separates F(ky) and conj F(-ky) on store.
+//  * rows, one half plane at a time: row rho of the half (rho = 8w + rl) is carried by
+//    the 8 lanes s of (w, rl) as 64 = 8 x 8: an 8-point DFT over m of the elements
+//    s + 8m, twiddles W_64^{s k1}, an exchange through the row's own LDS slots (XOR
+//    swizzled: conflict-free both ways), and an 8-point DFT; the real r2c / c2r split
+//    of length 128 over the length-64 complex transform as in k_rows_r2c / k_rows_c2r.
+//  * every 64-slot LDS row goes through hpi(), which moves odd rows by half a bank
+//    window: gfx950 serves a 16-byte LDS access in groups of 16 lanes (two teams, or two
+//    rows), and without it those pairs collide on the same banks.
+// Every LDS exchange inside a phase stays within one wavefront (wavefront barriers);
+// the half-plane hand-overs between phases use workgroup barriers.
+// ---------------------------------------------------------------------------
+constexpr int kHPlaneNT = 512;
+
+// LDS slot of element `col` (< 64) of row r of a 64-slot-row region: bit 3 of the column
+// flips with the row's parity, so two adjacent rows accessed by one 16-lane group (8
+// consecutive or XOR-permuted slots each) fall on opposite halves of the 64 banks.
+__device__ __forceinline__ int hpi(int r, int col) { return r * 64 + (col ^ ((r & 1) << 3)); }
+
+// Twiddles W_128^{t k1} (conjugated for INV) on v[8h + j], k1 = 8h + j, then the 8-point
+// DFT over t of k1 = 8h + t after the exchange through the wavefront's region ex (value
+// k1 = 8h + j of team t of column c in row j, slot 8t + c): out r[k2] = y[8h + t + 16 k2].  The twiddles are applied here, on
+// the way to LDS, so that they are not live across the 16-point DFT.
+template <typename R, bool INV>
+__device__ __forceinline__ void hplane_ycombine(const cplx<R> (&v)[16], int h, cplx<R>* ex, int c, int t,
+                                                const cplx<R>* tws, cplx<R> (&r)[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k1 = 8 * h + j;
+    cplx<R> a = v[k1];
+    if (k1 > 0) {
+      const cplx<R> w = tws[t * 24 + k1];
+      a = a * (INV ? conj(w) : w);
+    }
+    ex[hpi(j, 8 * t + c)] = a;
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int tp = 0; tp < 8; ++tp) r[tp] = ex[hpi(t, 8 * tp + c)];
+  bfly<R, INV, 8>(r);
+  wave_lds_sync();
+}
+
+// Forward: rfft over (x, y) of 128 x 128 real planes into half-spectrum planes (the z
+// pass follows).  grid = (planes per field, fields, jobs).
+template <typename R>
+__global__ __launch_bounds__(kHPlaneNT, 4) void k_hplane_r2c(SpecJobs jobs, const cplx<R>* __restrict__ twp) {
+  __shared__ cplx<R> lds[64 * 64];
+  __shared__ cplx<R> tws[kHPlaneTw];
+  __shared__ cplx<R> pk[128];  // the packed DC / Nyquist column after the y transform
+  const SpecJob& jb = jobs.j[blockIdx.z];
+  if ((int)blockIdx.y >= jb.count) return;
+  const long long plane = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+  const R* src = reinterpret_cast<const R*>(jb.src) + plane * (128LL * 128);
+  cplx<R>* dst = reinterpret_cast<cplx<R>*>(jb.dst) + plane * (65LL * 128);
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int c = lane & 7, t = lane >> 3;  // column phase
+  const int s = c, rho = 8 * w + t;       // row phase: lane s of row rho of the half plane
+  if (tid < kHPlaneTw) tws[tid] = twp[tid];
+  cplx<R> v[16];
+  // z[n] = x[2n] + i x[2n+1] of rows y = rho and 64 + rho, n = s + 8m: both halves are
+  // loaded up front, so the second half's loads overlap the first half's transforms
+  cplx<R> zz[2][8];
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    const cplx<R>* zr = reinterpret_cast<const cplx<R>*>(src + (64 * hh + rho) * 128);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) zz[hh][m] = zr[s + 8 * m];
+  }
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+    cplx<R> z[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) z[m] = zz[hh][m];
+    __syncthreads();  // twiddles loaded; the previous half's column reads are done
+    bfly<R, false, 8>(z);
+#pragma unroll
+    for (int k1 = 1; k1 < 8; ++k1) z[k1] = z[k1] * tws[kHPlaneTX + k1 * 8 + s];
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) lds[hpi(rho, k1 * 8 + (s ^ k1))] = z[k1];
+    wave_lds_sync();
+    cplx<R> u[8];
+#pragma unroll
+    for (int sp = 0; sp < 8; ++sp) u[sp] = lds[hpi(rho, s * 8 + (sp ^ s))];
+    bfly<R, false, 8>(u);  // u[k2] = Z[s + 8 k2]
+    wave_lds_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) lds[hpi(rho, s + 8 * k2)] = u[k2];
+    wave_lds_sync();
+    // X[k] = (Z_k + conj Z_{64-k}) / 2 - i W^k (Z_k - conj Z_{64-k}) / 2, and slot 0
+    // holds X[0] + i X[64] = (Re Z0 + Im Z0) + i (Re Z0 - Im Z0).  Z_k is read back from
+    // LDS rather than kept in u: the first half's columns are live here as well.
+    cplx<R> xs[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      const int k = s + 8 * k2;
+      const cplx<R> zk = lds[hpi(rho, k)];
+      const cplx<R> zm = conj(lds[hpi(rho, (64 - k) & 63)]);
+      const cplx<R> ev = R(0.5) * (zk + zm);
+      const cplx<R> od = mul_mi(R(0.5) * (zk - zm));
+      const cplx<R> gen = ev + tws[kHPlaneT2 + k] * od;
+      xs[k2] = k == 0 ? cplx<R>{zk.x + zk.y, zk.x - zk.y} : gen;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) lds[hpi(rho, s + 8 * k2)] = xs[k2];
+    __syncthreads();
+    // column x = 8w + c at rows y = t + 8m, m = 8 hh + mp
+    const int x = 8 * w + c;
+#pragma unroll
+    for (int mp = 0; mp < 8; ++mp) v[8 * hh + mp] = lds[hpi(t + 8 * mp, x)];
+  }
+  bfly<R, false, 16>(v);
+  __syncthreads();  // every column read of the half plane is done: the exchange regions reuse it
+  cplx<R>* ex = lds + w * 512;
+  const int x = 8 * w + c;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cplx<R> r[8];
+    hplane_ycombine<R, false>(v, h, ex, c, t, tws, r);
+    if (x != 0) {
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) dst[x + 65 * (8 * h + t + 16 * k2)] = r[k2];
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) pk[8 * h + t + 16 * k2] = r[k2];
+    }
+  }
+  if (w == 0) {
+    // packed column: F(ky) = A(ky) + i B(ky) with A, B the spectra of the real DC and
+    // Nyquist columns; A = (F(ky) + conj F(-ky)) / 2, B = -i (F(ky) - conj F(-ky)) / 2
+    wave_lds_sync();
+    if (c == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) {
+          const int ky = 8 * h + t + 16 * k2;
+          const cplx<R> fk = pk[ky];
+          const cplx<R> fm = conj(pk[(128 - ky) & 127]);
+          dst[65 * ky] = R(0.5) * (fk + fm);
+          dst[64 + 65 * ky] = mul_mi(R(0.5) * (fk - fm));
+        }
+    }
+  }
+}
+
+// Flat plane index q over the jobs (count_j fields of nz planes each) -> job and plane.
+__device__ __forceinline__ bool hplane_locate(const SpecJobs& jobs, int nz, long long q, int& j, long long& plane) {
+  for (int i = 0; i < jobs.n; ++i) {
+    const long long np = (long long)jobs.j[i].count * nz;
+    if (q < np) {
+      j = i;
+      plane = q;
+      return true;
+    }
+    q -= np;
+  }
+  return false;
+}
+
+// c2r column loads v[M0 .. M0 + 7] of column x (rows y = t + 8m) of the half-spectrum plane
+// at src; column 0 takes c0 + i c64 from the Hermitian parts (c(y) + conj c(-y)) / 2.
+template <typename R, int M0>
+__device__ __forceinline__ void hplane_c2r_load(const cplx<R>* __restrict__ src, int x, int t, cplx<R> (&v)[16]) {
+  if (x != 0) {
+#pragma unroll
+    for (int mm = 0; mm < 8; ++mm) v[M0 + mm] = src[x + 65 * (t + 8 * (M0 + mm))];
+  } else {
+#pragma unroll
+    for (int mm = 0; mm < 8; ++mm) {
+      const int y = t + 8 * (M0 + mm), ym = (128 - y) & 127;
+      const cplx<R> a = src[65 * y], am = src[65 * ym];
+      const cplx<R> b = src[64 + 65 * y], bm = src[64 + 65 * ym];
+      v[M0 + mm] = cplx<R>{R(0.5) * (a.x + am.x) - R(0.5) * (b.y - bm.y), R(0.5) * (a.y - am.y) + R(0.5) * (b.x + bm.x)};
+    }
+  }
+}
+
+// Inverse: irfft over (x, y) of 128 x 128 half-spectrum planes (after the z pass), with
+// FFTW's c2r semantics (the imaginary parts of the x DC and Nyquist bins are ignored
+// after the y transform); output scaled by `scale`.  Workgroup b transforms plane b of the
+// `total` planes of the jobs.  (A persistent form that loads the next plane into registers
+// during the row phase does not fit: the row phase already holds ~118 VGPRs, and two
+// workgroups per CU allow 128.)
+template <typename R>
+__global__ __launch_bounds__(kHPlaneNT, 4) void k_hplane_c2r(SpecJobs jobs, int nz, long long total,
+                                                             const cplx<R>* __restrict__ tmp, double scale,
+                                                             const cplx<R>* __restrict__ twp) {
+  __shared__ cplx<R> lds[64 * 64];
+  __shared__ cplx<R> tws[kHPlaneTw];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int c = lane & 7, t = lane >> 3;
+  const int x = 8 * w + c;
+  const int s = c, rho = 8 * w + t;  // row phase: lane s of row rho of the half plane
+  cplx<R>* ex = lds + w * 512;
+  if (tid < kHPlaneTw) tws[tid] = twp[tid];
+  const R sc = (R)scale;
+  {
+    const long long q = blockIdx.x;
+    int j = 0;
+    long long plane = 0;
+    if (q >= total || !hplane_locate(jobs, nz, q, j, plane)) return;
+    const cplx<R>* src = tmp + jobs.j[j].toff + plane * (65LL * 128);
+    R* dst = reinterpret_cast<R*>(jobs.j[j].dst) + plane * (128LL * 128);
+    cplx<R> v[16];
+    hplane_c2r_load<R, 0>(src, x, t, v);
+    hplane_c2r_load<R, 8>(src, x, t, v);
+    __syncthreads();  // twiddles
+    bfly<R, true, 16>(v);
+    cplx<R> r0[8], r1[8];
+    hplane_ycombine<R, true>(v, 0, ex, c, t, tws, r0);
+    hplane_ycombine<R, true>(v, 1, ex, c, t, tws, r1);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();  // the exchange regions / the previous half are no longer read
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) lds[hpi(k2 * 8 + t, x)] = h == 0 ? r0[k2] : r1[k2];
+      __syncthreads();
+      const int ky = 8 * h + (rho & 7) + 16 * (rho >> 3);
+      cplx<R> z[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int k = s + 8 * m;
+        const cplx<R> pk = lds[hpi(rho, k)];
+        const cplx<R> pm = lds[hpi(rho, (64 - k) & 63)];
+        const cplx<R> xk = k == 0 ? cplx<R>{pk.x, R(0)} : pk;
+        const cplx<R> xm = k == 0 ? cplx<R>{pk.y, R(0)} : pm;
+        const cplx<R> ev = xk + conj(xm);
+        const cplx<R> od = (xk - conj(xm)) * conj(tws[kHPlaneT2 + k]);
+        z[m] = ev + mul_i(od);
+      }
+      bfly<R, true, 8>(z);
+#pragma unroll
+      for (int k1 = 1; k1 < 8; ++k1) z[k1] = z[k1] * conj(tws[kHPlaneTX + k1 * 8 + s]);
+      wave_lds_sync();  // the wavefront's rows are read
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) lds[hpi(rho, k1 * 8 + (s ^ k1))] = z[k1];
+      wave_lds_sync();
+      cplx<R> u[8];
+#pragma unroll
+      for (int sp = 0; sp < 8; ++sp) u[sp] = lds[hpi(rho, s * 8 + (sp ^ s))];
+      bfly<R, true, 8>(u);  // u[k2] = z[s + 8 k2] = x[2k] + i x[2k+1], k = s + 8 k2
+      cplx<R>* o = reinterpret_cast<cplx<R>*>(dst + ky * 128);
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) o[s + 8 * k2] = cplx<R>{sc * u[k2].x, sc * u[k2].y};
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host dispatch
 // ---------------------------------------------------------------------------
 #define SAV_FFT_SIZES(X) \
@@ -564,6 +835,18 @@ template <typename R>
 static void launch_plane(const FftGeom<R>& g, const SpecJobs& jobs, bool inverse, const cplx<R>* tmp, double scale,
                          hipStream_t s) {
   const dim3 grid(g.Nz, max_count(jobs), jobs.n);
+  if (g.Nx == 128 && g.tw_hp && g.plane_kind == 0) {
+    if (inverse) {
+      long long total = 0;
+      for (int i = 0; i < jobs.n; ++i) total += (long long)jobs.j[i].count * g.Nz;
+      if (total == 0) return;
+      const dim3 pgrid((unsigned)total);
+      hipLaunchKernelGGL((k_hplane_c2r<R>), pgrid, dim3(kHPlaneNT), 0, s, jobs, g.Nz, total, tmp, scale, g.tw_hp);
+    } else
+      hipLaunchKernelGGL((k_hplane_r2c<R>), grid, dim3(kHPlaneNT), 0, s, jobs, g.tw_hp);
+    SAV_LAUNCH_CHECK();
+    return;
+  }
   switch (g.Nx) {
 #define X(NN)                                                                                              \
   case NN: {                                                                                               \

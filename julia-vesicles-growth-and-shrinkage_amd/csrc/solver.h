@@ -85,6 +85,10 @@ class ISolver {
   // column-transform tile autotuner (every transform outside the psi matvec)
   virtual int set_transform_tile(int cw) = 0;
   virtual void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) = 0;
+  // (x, y) plane kernels of 128 x 128 x Nz grids: 0 = autotuned, 1 = register columns with a
+  // half plane in LDS, 2 = the whole plane in LDS
+  virtual int set_plane_kernel(int kind) = 0;
+  virtual void plane_kernel(int* kind, double* ms_register, double* ms_lds) = 0;
   virtual int set_graph(int on) = 0;
   // Eager steps run step 1's advection terms and step 4 on a second stream (on by default);
   // ensemble members, which overlap with each other, keep one stream each.

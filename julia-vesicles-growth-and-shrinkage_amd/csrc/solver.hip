@@ -237,6 +237,7 @@ class Solver final : public ISolver {
     g.tw_xh = upload(default_twiddles<R>(Nx / 2));
     g.tw_y = upload(column_twiddles<R>(Ny));
     g.tw_z = nd == 3 ? upload(column_twiddles<R>(Nz)) : g.tw_y;
+    if (nd == 3 && Nx == 128 && Ny == 128) g.tw_hp = upload(hplane_twiddles<R>());
     g.k = KTables{d_kx, d_ky, d_kz, d_kx0};
     // states
     const long long nphys = (4LL * N + nd + 1) * P, nhalf = (4LL * N + nd + 1) * H;
@@ -1067,7 +1068,57 @@ class Solver final : public ISolver {
   // code whatever tile it shares, so every width gives bit-identical results.
   int tcolw = 0;
   double tune_t_ms[3] = {0.0, 0.0, 0.0};
+  int pkind = 0;  // plane kernels: 0 = tune at the next step, 1 = register columns, 2 = LDS plane
+  double tune_p_ms[2] = {0.0, 0.0};
+  // Milliseconds of one irfft and one rfft of the present phi (three repeats after a warm-up).
+  double time_transforms() {
+    auto once = [&]() {
+      irfft({job(fhalf(ip, F_PHI), w_d4, N, MUL_LAP)});
+      rfft({job(fphys(ip, F_PHI), w_phat, N)});
+    };
+    hipEvent_t a, b;
+    SAV_HIP(hipEventCreate(&a));
+    SAV_HIP(hipEventCreate(&b));
+    once();  // warm-up
+    SAV_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < 3; ++r) once();
+    SAV_HIP(hipEventRecord(b, s));
+    SAV_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SAV_HIP(hipEventElapsedTime(&ms, a, b));
+    SAV_HIP(hipEventDestroy(a));
+    SAV_HIP(hipEventDestroy(b));
+    return ms / 3.0;
+  }
   void autotune_transforms() {
+    autotune_columns();
+    if (pkind == 0) {
+      if (!g.tw_hp) {
+        pkind = 2;  // no register-column kernels for this grid
+      } else {
+        for (int k = 0; k < 2; ++k) {
+          g.plane_kind = k;
+          tune_p_ms[k] = time_transforms();
+        }
+        pkind = tune_p_ms[0] <= tune_p_ms[1] ? 1 : 2;
+      }
+    }
+    g.plane_kind = pkind == 1 ? 0 : 1;
+  }
+  int set_plane_kernel(int kind) override {
+    if (kind < 0 || kind > 2) return SAV_E_ARG;
+    if (kind == 1 && !g.tw_hp) return SAV_E_UNSUPPORTED;
+    if (kind != pkind) drop_graphs();  // captured transforms use the previous kernels
+    pkind = kind;
+    if (kind) g.plane_kind = kind == 1 ? 0 : 1;
+    return SAV_OK;
+  }
+  void plane_kernel(int* kind, double* ms_register, double* ms_lds) override {
+    if (kind) *kind = pkind;
+    if (ms_register) *ms_register = tune_p_ms[0];
+    if (ms_lds) *ms_lds = tune_p_ms[1];
+  }
+  void autotune_columns() {
     if (tcolw != 0) return;
     const int widths[3] = {4, 8, 16};
     const int rb = (int)sizeof(R);

@@ -39,6 +39,9 @@ SAV_E_UNSUPPORTED = -5
 SAV_PREC_FP64 = 0
 SAV_PREC_FP32 = 1
 SAV_PREC_MIXED = 2
+SAV_PLANE_AUTO = 0
+SAV_PLANE_REGISTER = 1
+SAV_PLANE_LDS = 2
 SAV_PRECOND_DIAG = 0
 SAV_PRECOND_LEARNED = 1
 SAV_PRECOND_STENCIL = 2
@@ -75,7 +78,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft", "sav_irfft_noalloc",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
-           "sav_set_column_tile", "sav_autotune_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
+           "sav_set_column_tile", "sav_autotune_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
            "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
            "sav_ensemble_member", "sav_ensemble_run")
@@ -129,6 +132,8 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_autotune_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
     lib.sav_set_transform_tile.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_transform_tile_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp, _dp]
+    lib.sav_set_plane_kernel.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_plane_kernel_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
     lib.sav_set_graph.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_graph_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
                                    ct.POINTER(ct.c_longlong)]
@@ -441,6 +446,18 @@ class Operators:
         self._check(self.lib.sav_transform_tile_info(self.h, ct.byref(cw), ct.byref(m4), ct.byref(m8), ct.byref(m16)),
                     "transform_tile_info")
         return dict(columns=cw.value, ms_4=m4.value, ms_8=m8.value, ms_16=m16.value)
+
+    def set_plane_kernel(self, kind: int):
+        """(x, y) plane kernels of 128 x 128 planes in 3D: SAV_PLANE_AUTO (time both at the
+        next step), SAV_PLANE_REGISTER or SAV_PLANE_LDS."""
+        return self._check(self.lib.sav_set_plane_kernel(self.h, int(kind)), "set_plane_kernel")
+
+    def plane_kernel_info(self):
+        k = ct.c_int32(0)
+        a, b = ct.c_double(0), ct.c_double(0)
+        self._check(self.lib.sav_plane_kernel_info(self.h, ct.byref(k), ct.byref(a), ct.byref(b)),
+                    "plane_kernel_info")
+        return dict(kind=k.value, ms_register=a.value, ms_lds=b.value)
 
     def set_graph(self, on: bool):
         """Replay each step as one captured HIP graph (bit-identical results)."""
