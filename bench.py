@@ -262,6 +262,26 @@ def cpu_report(jv, which="all", steps=20, warm=2):
     return rows
 
 
+def run_planes(jv, fields=8, repeats=5):
+    """Both (x, y) plane-kernel families of 3D transforms run alone: `repeats` rfft and irfft
+    calls of `fields` random 128^3 fields per precision and family (host copies included;
+    a kernel trace gives each kernel's duration), with the round-trip error."""
+    out = {}
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal((fields, 128, 128, 128))
+    for prec, pname in ((jv.SAV_PREC_FP64, "fp64"), (jv.SAV_PREC_FP32, "fp32")):
+        conf = jv.workload_c4(n=128, steps=1).conf.with_(N=fields, psi_in_v=[1.2] * fields,
+                                                        psi_out_v=[1.0] * fields)
+        ops = jv.Operators(conf, precision=prec)
+        for kind, kname in ((jv.SAV_PLANE_REGISTER, "register"), (jv.SAV_PLANE_LDS, "lds")):
+            ops.set_plane_kernel(kind)
+            for _ in range(repeats):
+                back = ops.irfft(ops.rfft(u.T))
+            out[f"{pname}_{kname}_roundtrip_max_err"] = float(np.max(np.abs(back - u.T)))
+        ops.close()
+    return out
+
+
 def _run_full(jv, wl, precision):
     """The workload's whole step schedule (uniform dt, BDF1 then BDF2) after the SAV
     initialisation; the steps are timed together."""
@@ -369,7 +389,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble", "fp64-2d", "mixed-2d", "fp64-3d"],
+    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble", "fp64-2d", "mixed-2d", "fp64-3d", "planes"],
                     default="all", help="fp64-2d / mixed-2d / fp64-3d: one run of that form, for profiler runs")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
@@ -409,6 +429,9 @@ def main():
         return
     if a.cpu_report:
         print(json.dumps(cpu_report(jv, a.cpu_report)), flush=True)
+        return
+    if a.only == "planes":
+        print(json.dumps(run_planes(jv)), flush=True)
         return
     if a.only == "ensemble":
         print(json.dumps(run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64)), flush=True)

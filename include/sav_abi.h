@@ -188,7 +188,8 @@ int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, dou
  * columns in registers and stages a 64 KB half plane (FP64) through LDS; SAV_PLANE_LDS holds
  * the whole half-spectrum plane in LDS.  Both compute the same transform (results agree to
  * FP round-off, not bit for bit).  With SAV_PLANE_AUTO (the default) the next step times an
- * irfft and an rfft of the present phi with each and keeps the faster.
+ * irfft and an rfft of the present phi with each and keeps SAV_PLANE_REGISTER unless
+ * SAV_PLANE_LDS is more than 3 % faster.
  * sav_plane_kernel_info returns the kind in use (SAV_PLANE_AUTO before it is chosen) and
  * the measured milliseconds of each (0: not timed). Other grids use SAV_PLANE_LDS where the
  * plane fits in LDS; sav_set_plane_kernel(h, SAV_PLANE_REGISTER) is SAV_E_UNSUPPORTED there. */

@@ -1100,7 +1100,10 @@ class Solver final : public ISolver {
           g.plane_kind = k;
           tune_p_ms[k] = time_transforms();
         }
-        pkind = tune_p_ms[0] <= tune_p_ms[1] ? 1 : 2;
+        // the timed pair (one field set, largely cache-resident) favours neither family
+        // much; the register kernels move the same bytes with less LDS traffic, so the
+        // LDS kernels are kept only when clearly faster
+        pkind = tune_p_ms[1] < 0.97 * tune_p_ms[0] ? 2 : 1;
       }
     }
     g.plane_kind = pkind == 1 ? 0 : 1;
