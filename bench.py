@@ -393,7 +393,7 @@ def main():
                     default="all", help="fp64-2d / mixed-2d / fp64-3d: one run of that form, for profiler runs")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
-    ap.add_argument("--column-tile", type=int, default=0, choices=[0, 8, 16],
+    ap.add_argument("--column-tile", type=int, default=0, choices=[0, 4, 8, 16],
                     help="fix the psi matvec's columns per workgroup (0: autotuned; fixing it makes "
                          "separate profiler runs launch the same kernels)")
     ap.add_argument("--cpu-steps", type=int, default=5, help="steps of the CPU oracle sample")
@@ -522,9 +522,9 @@ def main():
         line["fp64_graph_replay_1024"] = {"steps_per_s": round(o["graph"]["steps_per_s"], 3),
                                           "bit_identical_to_eager": o["graph_equal"], **o["graph"]["graph"]}
         tune = dict(o["fp64"]["autotune"])
-        if tune["ms_8"] == 0.0 and tune["ms_16"] == 0.0:
-            tune["note"] = ("both widths clamp to one kernel shape at this size and precision (160 KB of "
-                            "LDS holds at most 8 columns of 1024 FP64 complex values), so nothing was timed")
+        if 0.0 in (tune["ms_4"], tune["ms_8"], tune["ms_16"]):
+            tune["note"] = ("a width whose kernels resolve to those of a width already timed is not timed "
+                            "(ms 0): at 1024 in FP64, 160 KB of LDS holds at most 8 columns, so 16 is 8")
         line["psi_matvec_column_tile_1024"] = tune
         if "learned" in o:
             L = o["learned"]

@@ -165,13 +165,16 @@ int sav_psi_solve(sav_handle_t* h, const double* phi_star, double a, double dt, 
 int sav_modified_energy(sav_handle_t* h, double* energy, double* area_ratio);
 
 /* Autotuner of the 2D psi matvec (no reference counterpart): at the first psi solve of a
- * handle the column-FFT kernels are timed with 8 and with 16 columns per workgroup (three
- * matvecs each on the solve's right-hand side) and the faster is kept; both compute the
- * same numbers (the dot products are summed per column).  sav_set_column_tile(h, 8 | 16)
- * fixes the width, 0 retunes at the next solve.  sav_autotune_info returns the width in
- * use (0 before the first solve) and the measured milliseconds per matvec of each width. */
+ * handle the column-FFT kernels are timed with 8, 16 and 4 columns per workgroup (three
+ * matvecs each on the solve's right-hand side; a width whose kernels resolve to those of a
+ * width already timed is skipped) and the fastest is kept; all compute the same numbers
+ * (the dot products are summed per column).  sav_set_column_tile(h, 4 | 8 | 16) fixes the
+ * width, 0 retunes at the next solve.  sav_autotune_info returns the width in use (0 before
+ * the first solve) and the measured milliseconds per matvec with 8 and 16 columns;
+ * sav_column_tile_info adds the 4-column time (0: not timed). */
 int sav_set_column_tile(sav_handle_t* h, int columns);
 int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16);
+int sav_column_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16);
 
 /* Autotuner of every other column (strided-axis) transform (no reference counterpart): at a
  * handle's first step an irfft and an rfft of the present phi are timed with 4, 8 and 16

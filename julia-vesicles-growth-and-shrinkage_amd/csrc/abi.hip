@@ -232,7 +232,13 @@ int sav_set_column_tile(sav_handle_t* h, int columns) {
 int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16) {
   if (!h) return SAV_E_ARG;
   int cw = 0;
-  SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16); if (columns) *columns = cw; return SAV_OK);
+  SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16, nullptr); if (columns) *columns = cw; return SAV_OK);
+}
+
+int sav_column_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16) {
+  if (!h) return SAV_E_ARG;
+  int cw = 0;
+  SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16, ms_4); if (columns) *columns = cw; return SAV_OK);
 }
 
 int sav_set_transform_tile(sav_handle_t* h, int columns) {

@@ -48,7 +48,7 @@ struct KrylovCtx {
   int snap_cap = 0, snap_n = 0;
   // columns per workgroup of the matvec's column kernels: 0 = not tuned yet, 8 or 16
   int colw = 0;
-  double tune_ms[2] = {0.0, 0.0};  // measured matvec time with 8 and with 16 columns
+  double tune_ms[3] = {0.0, 0.0, 0.0};  // measured matvec time with 8, 16 and 4 columns (0: not timed)
   // preconditioned directions of the flexible solver
   cplx<R>* Bp = nullptr;
   cplx<R>* Bs = nullptr;

@@ -23,8 +23,8 @@
 
 namespace sav {
 
-// Column kernels of the matvec: CW adjacent columns per workgroup (8 or 16; chosen per
-// handle by timing both at the first solve, see autotune_columns), the column plan of
+// Column kernels of the matvec: CW adjacent columns per workgroup (4, 8 or 16; chosen per
+// handle by timing them at the first solve, see autotune_columns), the column plan of
 // col_plan_maxr / col_plan_ttmax (FWD: the forward kernel's plan) with k-major twiddles
 // (KrylovCtx::tw_ci, tw_cf).
 template <typename R, int L, int CW = 8, bool FWD = false>
@@ -683,6 +683,8 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   tev_pre(k, 1, e0, e1);
   if (k.colw == 16)
     launch_cols_inv<R, 16>(k, y, upd, ks, nsys, e0, e1);
+  else if (k.colw == 4)
+    launch_cols_inv<R, 4>(k, y, upd, ks, nsys, e0, e1);
   else
     launch_cols_inv<R, 8>(k, y, upd, ks, nsys, e0, e1);
   tev_post(k, e0, (upd == 0 ? 6.0 : upd == 1 ? 10.0 : 12.0) * sn);
@@ -705,8 +707,9 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   SAV_LAUNCH_CHECK();
   tev_post(k, e0, 9.0 * sn);
   tev_pre(k, 3, e0, e1);
-  const int nb3 = k.colw == 16 ? launch_cols_fwd<R, 16>(k, y, out, mode, partner, ks, nsys, e0, e1)
-                               : launch_cols_fwd<R, 8>(k, y, out, mode, partner, ks, nsys, e0, e1);
+  const int nb3 = k.colw == 16  ? launch_cols_fwd<R, 16>(k, y, out, mode, partner, ks, nsys, e0, e1)
+                  : k.colw == 4 ? launch_cols_fwd<R, 4>(k, y, out, mode, partner, ks, nsys, e0, e1)
+                                : launch_cols_fwd<R, 8>(k, y, out, mode, partner, ks, nsys, e0, e1);
   tev_post(k, e0, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn);
   return nb3;
 }
@@ -725,24 +728,28 @@ static int effective_columns(int Ny) {
 }
 
 // Autotuner of the matvec's column kernels: at the first solve of a handle, times
-// three matvecs (on the right-hand side in k.b, output into k.t, KScal slot 3) with 8
-// and with 16 columns per workgroup and keeps the faster.  Both widths compute the same
-// numbers: the same transform per column, and the Krylov dot products are summed per
-// column in a fixed order.
+// three matvecs (on the right-hand side in k.b, output into k.t, KScal slot 3) with 8, 16
+// and 4 columns per workgroup (widths that resolve to the kernel shape of a width already
+// timed are skipped) and keeps the fastest.  All widths compute the same numbers: the same
+// transform per column, and the Krylov dot products are summed per column in a fixed order.
 template <typename R>
 static void autotune_columns(KrylovCtx<R>& k) {
   if (k.colw != 0) return;
-  if (effective_columns<R, 8>(k.Ny) == effective_columns<R, 16>(k.Ny)) {
-    k.colw = 8;  // one kernel shape either way: nothing to choose
+  const int e8 = effective_columns<R, 8>(k.Ny), e16 = effective_columns<R, 16>(k.Ny),
+            e4 = effective_columns<R, 4>(k.Ny);
+  if (e8 == e16 && e8 == e4) {
+    k.colw = 8;  // one kernel shape whatever the width: nothing to choose
     return;
   }
   hipEvent_t a, b;
   SAV_HIP(hipEventCreate(&a));
   SAV_HIP(hipEventCreate(&b));
-  const int widths[2] = {8, 16};
+  const int widths[3] = {8, 16, 4};
+  const bool timed_w[3] = {true, e16 != e8, e4 != e8 && e4 != e16};
   const int saved_timed = k.timed;
   k.timed = 0;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < 3; ++w) {
+    if (!timed_w[w]) continue;
     k.colw = widths[w];
     KScal z{};
     SAV_HIP(hipMemcpyAsync(k.ks + 3, &z, sizeof(KScal), hipMemcpyHostToDevice, k.st));
@@ -756,7 +763,10 @@ static void autotune_columns(KrylovCtx<R>& k) {
     k.tune_ms[w] = ms / 3.0;
   }
   k.timed = saved_timed;
-  k.colw = k.tune_ms[1] < k.tune_ms[0] ? 16 : 8;
+  int best = 0;
+  for (int w = 1; w < 3; ++w)
+    if (timed_w[w] && k.tune_ms[w] < k.tune_ms[best]) best = w;
+  k.colw = widths[best];
   SAV_HIP(hipEventDestroy(a));
   SAV_HIP(hipEventDestroy(b));
 }

@@ -81,7 +81,7 @@ class ISolver {
   virtual long long precond_fallbacks() = 0;
   // psi matvec column-tile autotuner (2D)
   virtual int set_column_tile(int cw) = 0;
-  virtual void column_tile(int* cw, double* ms8, double* ms16) = 0;
+  virtual void column_tile(int* cw, double* ms8, double* ms16, double* ms4) = 0;
   // column-transform tile autotuner (every transform outside the psi matvec)
   virtual int set_transform_tile(int cw) = 0;
   virtual void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) = 0;

@@ -1054,7 +1054,7 @@ class Solver final : public ISolver {
 
   int set_column_tile(int cw) override {
     if (nd != 2) return SAV_E_UNSUPPORTED;
-    if (cw != 0 && cw != 8 && cw != 16) return SAV_E_ARG;
+    if (cw != 0 && cw != 4 && cw != 8 && cw != 16) return SAV_E_ARG;
     if (cw != kc.colw) drop_graphs();  // captured matvecs use the previous width
     kc.colw = cw;
     return SAV_OK;
@@ -1179,10 +1179,11 @@ class Solver final : public ISolver {
     if (ms16) *ms16 = tune_t_ms[2];
   }
 
-  void column_tile(int* cw, double* ms8, double* ms16) override {
+  void column_tile(int* cw, double* ms8, double* ms16, double* ms4) override {
     if (cw) *cw = kc.colw;
     if (ms8) *ms8 = kc.tune_ms[0];
     if (ms16) *ms16 = kc.tune_ms[1];
+    if (ms4) *ms4 = kc.tune_ms[2];
   }
 
   int precond_train(int iters, int batch, double lr, uint64_t seed, double* l0, double* l1) override {

@@ -1,15 +1,17 @@
-"""The psi matvec's column-tile autotuner: it picks one of the two widths at the first
-solve, reports both timings, and both widths give bit-identical solutions (the Krylov dot
-products are summed per column, whatever the tile)."""
+"""The psi matvec's column-tile autotuner: it picks one of the widths at the first solve,
+reports the timings of the widths with distinct kernels, and every width gives
+bit-identical solutions (the Krylov dot products are summed per column, whatever the tile)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_column_tile_choice_does_not_change_results(jv, so, precision):
-    wl = jv.workload_c1(n=256, steps=1)   # at 256 the two widths give different kernels
+# at 256 the 8- and 16-column kernels differ (4 resolves to 8); at 1024 in FP64 the 8- and
+# 4-column kernels differ (16 resolves to 8: 160 KB of LDS holds 8 columns of 1024)
+@pytest.mark.parametrize("n,precision,timed", [(256, 0, (8, 16)), (256, 1, (8, 16)), (1024, 0, (8, 4))])
+def test_column_tile_choice_does_not_change_results(jv, so, n, precision, timed):
+    wl = jv.workload_c1(n=n, steps=1)
     conf = wl.conf
     phi, psi = wl.initial_fields()
     orc = so.Oracle(conf)
@@ -18,15 +20,20 @@ def test_column_tile_choice_does_not_change_results(jv, so, precision):
     assert ops.autotune_info()["columns"] == 0
     auto, info = ops.psi_solve(phi.T, 1.5, conf.dt, rhs.T)
     t = ops.autotune_info()
-    print(t)
-    assert t["columns"] in (8, 16) and t["ms_8"] > 0 and t["ms_16"] > 0
+    print(n, precision, t)
+    assert t["columns"] in timed
+    for cw in (4, 8, 16):
+        assert (t[f"ms_{cw}"] > 0) == (cw in timed), cw
+    best = min(timed, key=lambda cw: t[f"ms_{cw}"])
+    assert t["columns"] == best
     outs = {}
-    for cw in (8, 16):
+    for cw in (4, 8, 16):
         ops.set_column_tile(cw)
         outs[cw], inf = ops.psi_solve(phi.T, 1.5, conf.dt, rhs.T)
         assert inf["iters"] == info["iters"]
     ops.close()
     assert np.array_equal(outs[8], outs[16])
+    assert np.array_equal(outs[8], outs[4])
     assert np.array_equal(auto, outs[t["columns"]])
 
 
