@@ -727,11 +727,15 @@ static int effective_columns(int Ny) {
   }
 }
 
-// Autotuner of the matvec's column kernels: at the first solve of a handle, times
-// three matvecs (on the right-hand side in k.b, output into k.t, KScal slot 3) with 8, 16
-// and 4 columns per workgroup (widths that resolve to the kernel shape of a width already
-// timed are skipped) and keeps the fastest.  All widths compute the same numbers: the same
-// transform per column, and the Krylov dot products are summed per column in a fixed order.
+// Autotuner of the matvec's column kernels: at the first solve of a handle, times three
+// pairs of matvecs in the shape of a step's batched solve (three systems, KScal slots 0-2,
+// on the right-hand sides in k.b, output into k.t; in FP64 with the s and p updates formed on
+// load, as an iteration runs them) with 8, 16 and 4 columns per workgroup (widths that
+// resolve to the kernel shape of a width already timed are skipped) and keeps the fastest.
+// One system alone favours narrow tiles (1024^2 FP64: 4 columns 0.085 against 0.089 ms per
+// matvec) that are slower over the three systems of a step.  All widths compute the same
+// numbers: the same transform per column, and the Krylov dot products are summed per column
+// in a fixed order.  The slots are reinitialised by the solve that follows.
 template <typename R>
 static void autotune_columns(KrylovCtx<R>& k) {
   if (k.colw != 0) return;
@@ -748,19 +752,24 @@ static void autotune_columns(KrylovCtx<R>& k) {
   const bool timed_w[3] = {true, e16 != e8, e4 != e8 && e4 != e16};
   const int saved_timed = k.timed;
   k.timed = 0;
+  constexpr int nsys = 3;
+  constexpr int u1 = sizeof(R) == 8 ? 1 : 0, u2 = sizeof(R) == 8 ? 2 : 0;
+  const KScal zs[nsys] = {};
   for (int w = 0; w < 3; ++w) {
     if (!timed_w[w]) continue;
     k.colw = widths[w];
-    KScal z{};
-    SAV_HIP(hipMemcpyAsync(k.ks + 3, &z, sizeof(KScal), hipMemcpyHostToDevice, k.st));
-    matvec_launch(k, k.b, k.t, 0, 3);  // warm-up
+    SAV_HIP(hipMemcpyAsync(k.ks, zs, sizeof(zs), hipMemcpyHostToDevice, k.st));
+    matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);  // warm-up
     SAV_HIP(hipEventRecord(a, k.st));
-    for (int r = 0; r < 3; ++r) matvec_launch(k, k.b, k.t, 0, 3);
+    for (int r = 0; r < 3; ++r) {
+      matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);
+      matvec_launch(k, k.b, k.t, 2, 0, k.b, nsys, u2);
+    }
     SAV_HIP(hipEventRecord(b, k.st));
     SAV_HIP(hipEventSynchronize(b));
     float ms = 0.f;
     SAV_HIP(hipEventElapsedTime(&ms, a, b));
-    k.tune_ms[w] = ms / 3.0;
+    k.tune_ms[w] = ms / 6.0;
   }
   k.timed = saved_timed;
   int best = 0;
