@@ -176,6 +176,17 @@ int sav_set_column_tile(sav_handle_t* h, int columns);
 int sav_autotune_info(sav_handle_t* h, int32_t* columns, double* ms_8, double* ms_16);
 int sav_column_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16);
 
+/* Order of the 2D psi matvec's three kernels over the systems of a batched solve (no
+ * reference counterpart): 1 = every kernel covers all systems; 2 = system by system, each
+ * through all three kernels over one shared pair of intermediate fields, which then stays
+ * in the GPU's infinity cache between the kernels instead of going to HBM.  0 (the default)
+ * times both at the first psi solve, with the chosen column width, and keeps the faster.
+ * Both orders compute the same numbers.  sav_matvec_order_info returns the order in use
+ * (0 before the first solve) and the measured milliseconds per matvec in order 1 and 2
+ * (0: not timed). */
+int sav_set_matvec_order(sav_handle_t* h, int order);
+int sav_matvec_order_info(sav_handle_t* h, int32_t* order, double* ms_batched, double* ms_sequenced);
+
 /* Autotuner of every other column (strided-axis) transform (no reference counterpart): at a
  * handle's first step an irfft and an rfft of the present phi are timed with 4, 8 and 16
  * columns per workgroup (widths that resolve to the same tile for this grid are timed once)

@@ -241,6 +241,17 @@ int sav_column_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double
   SAV_GUARD(h, h->s->column_tile(&cw, ms_8, ms_16, ms_4); if (columns) *columns = cw; return SAV_OK);
 }
 
+int sav_set_matvec_order(sav_handle_t* h, int order) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->set_matvec_order(order));
+}
+
+int sav_matvec_order_info(sav_handle_t* h, int32_t* order, double* ms_batched, double* ms_sequenced) {
+  if (!h) return SAV_E_ARG;
+  int o = 0;
+  SAV_GUARD(h, h->s->matvec_order(&o, ms_batched, ms_sequenced); if (order) *order = o; return SAV_OK);
+}
+
 int sav_set_transform_tile(sav_handle_t* h, int columns) {
   if (!h) return SAV_E_ARG;
   int rc = 0;

@@ -40,6 +40,7 @@ struct KrylovCtx {
   long long ev_seq = 0;
   int cur_iter = 0;
   int ev_iter[512];
+  int ev_sys[512];             // system of a single-system launch, -1: all systems
   double ev_bytes[512];
   long long tk_count = 0;
   double tk_ms = 0.0, tk_bytes = 0.0;
@@ -48,6 +49,12 @@ struct KrylovCtx {
   int snap_cap = 0, snap_n = 0;
   // columns per workgroup of the matvec's column kernels: 0 = not tuned yet, 8 or 16
   int colw = 0;
+  // order of a matvec's kernels over the systems of a batch: 0 = not chosen yet, 1 = all
+  // systems per launch, 2 = system by system over one shared intermediate pair (infinity-
+  // cache resident); chosen after the column width at the first solve.  seq = (order == 2).
+  int order = 0;
+  int seq = 0;
+  double tune_order_ms[2] = {0.0, 0.0};  // measured matvec time in order 1 and 2 (0: not timed)
   double tune_ms[3] = {0.0, 0.0, 0.0};  // measured matvec time with 8, 16 and 4 columns (0: not timed)
   // preconditioned directions of the flexible solver
   cplx<R>* Bp = nullptr;

@@ -49,7 +49,7 @@ struct KRowCfg {
   // The forward transform runs the inverse's plan reversed, so its first pass reads
   // exactly the elements the inverse's last pass left in each lane: the M_psi product is
   // taken in registers, with no LDS exchange between the two transforms.  One wavefront
-  // per workgroup (64 / TT rows), synchronised at wavefront level.
+  // per field (64 / TT rows), synchronised at wavefront level.
   static constexpr int MAXR = row_plan_maxr(L);
   static constexpr int PADS = 4;
   using FI = TeamFFT<R, L, true, 64, PADS, 2, 1, MAXR, false>;
@@ -57,7 +57,7 @@ struct KRowCfg {
   static_assert(FF::R0 == FI::RL && FF::TT == FI::TT && FF::E == FI::E, "reversed plan");
   static constexpr int TT = FI::TT;
   static constexpr int ROWS = 64 / TT;
-  static constexpr int NT = 64;
+  static constexpr int NT = 128;  // two wavefronts: the Bx rows and the By rows
   static constexpr int RS = FI::LDS_LEN | 1;   // odd row stride
 };
 
@@ -153,6 +153,11 @@ __global__ __launch_bounds__((KColCfg<R, L, CW>::NT), (kcol_min_waves<R, L>())) 
 // ---------------------------------------------------------------------------
 // K2: rows: inverse FFT (scaled 1/(Nx Ny)) -> * M_psi -> forward FFT, in place
 // ---------------------------------------------------------------------------
+// A workgroup is two wavefronts: the first transforms rows q of Bx, the second the same
+// rows of By, so the two reads of each M_psi row come from one CU at the same time (with
+// the two rows in different workgroups the second read of an FP64 M row mostly missed the
+// L2: 1.11x the algorithmic bytes).  Each wavefront keeps its own LDS rows and
+// synchronises at wavefront level.
 template <typename R, int L>
 __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __restrict__ Bx, cplx<R>* __restrict__ By,
                                                                 const R* __restrict__ M, long long nrows_half,
@@ -167,14 +172,15 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
   using FI = typename KRowCfg<R, L>::FI;
   using FF = typename KRowCfg<R, L>::FF;
   constexpr int TT = FI::TT, E = FI::E, ROWS = KRowCfg<R, L>::ROWS, RS = KRowCfg<R, L>::RS;
-  __shared__ cplx<R> lds[ROWS * RS];
-  const int rloc = threadIdx.x / TT, t = threadIdx.x % TT;
+  __shared__ cplx<R> lds[2 * ROWS * RS];
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int rloc = lane / TT, t = lane % TT;
   const long long row = (long long)blockIdx.x * ROWS + rloc;
-  const bool act = row < 2 * nrows_half;
-  const long long rr = act ? (row % nrows_half) : 0;
-  cplx<R>* data = (row < nrows_half ? Bx : By) + rr * L;
+  const bool act = row < nrows_half;
+  const long long rr = act ? row : 0;
+  cplx<R>* data = (wv == 0 ? Bx : By) + rr * L;
   const R* m = M + rr * L;
-  cplx<R>* my = lds + rloc * RS;
+  cplx<R>* my = lds + (wv * ROWS + rloc) * RS;
   // FP64: M_psi is loaded with the row rather than after the first transform (measured
   // faster in FP64, slower in FP32)
   constexpr bool MPRE = sizeof(R) == 8;
@@ -194,8 +200,8 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
       mval = mv[e];
     else
       mval = act ? m[FI::out_pos(t, e)] : R(0);
-    const cplx<R> z = cplx<R>{sc * v[e].x, sc * v[e].y};
-    v[e] = cplx<R>{mval * z.x, mval * z.y};
+    const cplx<R> q = cplx<R>{sc * v[e].x, sc * v[e].y};
+    v[e] = cplx<R>{mval * q.x, mval * q.y};
   }
   // (the forward transform's first LDS write is preceded by its own barrier)
   FF::run(v, my, 1, t, twf);
@@ -574,10 +580,11 @@ static void tev_pre(KrylovCtx<R>& k, int id, hipEvent_t& e0, hipEvent_t& e1) {
   }
 }
 template <typename R>
-static void tev_post(KrylovCtx<R>& k, hipEvent_t e0, double bytes) {
+static void tev_post(KrylovCtx<R>& k, hipEvent_t e0, double bytes, int sys = -1) {
   if (e0) {
     k.ev_iter[k.ev_used] = k.cur_iter;
     k.ev_bytes[k.ev_used] = bytes;
+    k.ev_sys[k.ev_used] = sys;
     k.ev_used++;
   }
 }
@@ -588,7 +595,10 @@ template <typename R>
 static void tev_collect(KrylovCtx<R>& k, const int* iters, int nsys) {
   for (int i = 0; i < k.ev_used; ++i) {
     int active = 0;
-    for (int z = 0; z < nsys; ++z) active += k.ev_iter[i] <= iters[z] ? 1 : 0;
+    if (k.ev_sys[i] >= 0)  // a launch of one system (k.seq)
+      active = k.ev_sys[i] < nsys && k.ev_iter[i] <= iters[k.ev_sys[i]] ? 1 : 0;
+    else
+      for (int z = 0; z < nsys; ++z) active += k.ev_iter[i] <= iters[z] ? 1 : 0;
     if (active == 0) continue;
     float ms = 0.f;
     SAV_HIP(hipEventElapsedTime(&ms, k.ev[2 * i], k.ev[2 * i + 1]));
@@ -599,23 +609,37 @@ static void tev_collect(KrylovCtx<R>& k, const int* iters, int nsys) {
   k.ev_used = 0;
 }
 
+// Operands of one matvec launch sequence: the three kernels of a matvec over `nsys` systems
+// whose vectors (and Krylov scalars, partials) start at these pointers.
+template <typename R>
+struct MvArgs {
+  const cplx<R>* xin;      // column-inverse input: y, or r when an update is formed on load
+  const cplx<R>* V;        // v (updates only)
+  cplx<R>* xo;             // s or p (updates only)
+  cplx<R>* Bx;             // the two intermediate fields
+  cplx<R>* By;
+  const cplx<R>* y;        // the vector A is applied to (a/dt y term of the forward kernel)
+  const cplx<R>* partner;  // second operand of the fused dot products
+  cplx<R>* out;
+  const KScal* ks;
+  double* partials;
+  int nsys;
+};
+
 // upd: the vector update the kernel forms on load (k_mv_cols_inv's UPD, FP64 only); 1 and 2
-// take r and v from the context and write s (1) or p (2).
+// take r and v and write s (1) or p (2).
 template <typename R, int CW>
-static void launch_cols_inv(KrylovCtx<R>& k, const cplx<R>* y, int upd, const KScal* ks, int nsys, hipEvent_t e0,
-                            hipEvent_t e1) {
+static void launch_cols_inv(KrylovCtx<R>& k, const MvArgs<R>& a, int upd, hipEvent_t e0, hipEvent_t e1) {
   if (upd != 0 && sizeof(R) != 8) throw HipError("launch_cols_inv: the fused vector update is built for FP64 only", -1);
   const long long vs = k.vs;
-  const cplx<R>* xin = upd == 0 ? y : k.r;
-  cplx<R>* xo = upd == 1 ? k.s : upd == 2 ? k.p : nullptr;
-#define KL(L, U)                                                                                                  \
-  hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW, U>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, xin, k.v, xo, k.Bx, \
-                        k.By, k.Nx, k.S4, k.kx0, k.ky, k.tw_ci, ks, vs)
+#define KL(L, U)                                                                                                   \
+  hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW, U>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.xin, a.V, a.xo, a.Bx, \
+                        a.By, k.Nx, k.S4, k.kx0, k.ky, k.tw_ci, a.ks, vs)
   switch (k.Ny) {
 #define X(L)                                                  \
   case L: {                                                   \
     using CC = KColCfg<R, L, CW>;                             \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys);         \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, a.nsys);       \
     if constexpr (sizeof(R) == 8) {                           \
       if (upd == 1)                                           \
         KL(L, 1);                                             \
@@ -637,28 +661,27 @@ static void launch_cols_inv(KrylovCtx<R>& k, const cplx<R>* y, int upd, const KS
 }
 
 template <typename R, int CW>
-static int launch_cols_fwd(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, const cplx<R>* partner,
-                           const KScal* ks, int nsys, hipEvent_t e0, hipEvent_t e1) {
+static int launch_cols_fwd(KrylovCtx<R>& k, const MvArgs<R>& a, int mode, hipEvent_t e0, hipEvent_t e1) {
   const long long vs = k.vs, ps = k.ps;
   int nb3 = 0;
-#define KL(L, M)                                                                                                   \
-  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, y, partner, \
-                        out, k.Nx, k.S4, k.kx0, k.ky, k.tw_cf, k.ds, ks, k.partials, vs, ps)
+#define KL(L, M)                                                                                                 \
+  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, a.y,      \
+                        a.partner, a.out, k.Nx, k.S4, k.kx0, k.ky, k.tw_cf, k.ds, a.ks, a.partials, vs, ps)
   switch (k.Ny) {
-#define X(L)                                          \
-  case L: {                                           \
-    using CC = KColCfg<R, L, CW, true>;               \
-    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, nsys); \
-    nb3 = k.Nx * k.N;  /* one partial per column */    \
-    switch (mode) {                                   \
-      case 1: KL(L, 1); break;                        \
-      case 2: KL(L, 2); break;                        \
-      case 4: KL(L, 4); break;                        \
-      case 5: KL(L, 5); break;                        \
-      case 6: KL(L, 6); break;                        \
-      default: KL(L, 0); break;                       \
-    }                                                 \
-    break;                                            \
+#define X(L)                                            \
+  case L: {                                             \
+    using CC = KColCfg<R, L, CW, true>;                 \
+    dim3 grid((k.Nx + CC::C - 1) / CC::C, k.N, a.nsys); \
+    nb3 = k.Nx * k.N;  /* one partial per column */      \
+    switch (mode) {                                     \
+      case 1: KL(L, 1); break;                          \
+      case 2: KL(L, 2); break;                          \
+      case 4: KL(L, 4); break;                          \
+      case 5: KL(L, 5); break;                          \
+      case 6: KL(L, 6); break;                          \
+      default: KL(L, 0); break;                         \
+    }                                                   \
+    break;                                              \
   }
     SAV_KRY_SIZES(X)
 #undef X
@@ -669,48 +692,96 @@ static int launch_cols_fwd(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int 
   return nb3;
 }
 
-// One matvec on nsys systems (vectors k.vs apart, KScal slots slot .. slot+nsys-1).
-// mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
-// upd: 0 = apply to y; 1 = y is s = r - alpha v, 2 = y is p = r + beta (p - omega v), formed
-// (and stored) by the first kernel (launch_cols_inv).
 template <typename R>
-static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
-                         const cplx<R>* partner = nullptr, int nsys = 1, int upd = 0) {
-  const KScal* ks = k.ks + slot;
-  const long long vs = k.vs;
-  const double sn = (double)sizeof(R) * (double)k.nfull;
-  hipEvent_t e0, e1;
-  tev_pre(k, 1, e0, e1);
-  if (k.colw == 16)
-    launch_cols_inv<R, 16>(k, y, upd, ks, nsys, e0, e1);
-  else if (k.colw == 4)
-    launch_cols_inv<R, 4>(k, y, upd, ks, nsys, e0, e1);
-  else
-    launch_cols_inv<R, 8>(k, y, upd, ks, nsys, e0, e1);
-  tev_post(k, e0, (upd == 0 ? 6.0 : upd == 1 ? 10.0 : 12.0) * sn);
-  tev_pre(k, 2, e0, e1);
+static void launch_rows(KrylovCtx<R>& k, const MvArgs<R>& a, hipEvent_t e0, hipEvent_t e1) {
   const long long nrh = (long long)k.N * k.Ny;
   const double scale = 1.0 / ((double)k.Nx * k.Ny);
   switch (k.Nx) {
-#define X(L)                                                                                                \
-  case L: {                                                                                                 \
-    using CC = KRowCfg<R, L>;                                                                               \
-    dim3 grid((unsigned)((2 * nrh + CC::ROWS - 1) / CC::ROWS), nsys);                                       \
-    hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, k.Bx, k.By, k.M, nrh,  \
-                          scale, k.tw_ri, k.tw_rf, ks, vs);                                                           \
-    break;                                                                                                  \
+#define X(L)                                                                                                  \
+  case L: {                                                                                                   \
+    using CC = KRowCfg<R, L>;                                                                                 \
+    dim3 grid((unsigned)((nrh + CC::ROWS - 1) / CC::ROWS), a.nsys);                                           \
+    hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, k.M, nrh,    \
+                          scale, k.tw_ri, k.tw_rf, a.ks, k.vs);                                               \
+    break;                                                                                                    \
   }
     SAV_KRY_SIZES(X)
 #undef X
     default: throw HipError("unsupported Nx for the psi solve: " + std::to_string(k.Nx), -1);
   }
   SAV_LAUNCH_CHECK();
-  tev_post(k, e0, 9.0 * sn);
+}
+
+// The three kernels of a matvec on the operands `a` (sys: the system a single-system
+// sequence belongs to, for the kernel timing; -1 when `a` covers all systems).
+template <typename R>
+static int matvec_kernels(KrylovCtx<R>& k, const MvArgs<R>& a, int mode, int upd, int sys) {
+  const double sn = (double)sizeof(R) * (double)k.nfull;
+  hipEvent_t e0, e1;
+  tev_pre(k, 1, e0, e1);
+  if (k.colw == 16)
+    launch_cols_inv<R, 16>(k, a, upd, e0, e1);
+  else if (k.colw == 4)
+    launch_cols_inv<R, 4>(k, a, upd, e0, e1);
+  else
+    launch_cols_inv<R, 8>(k, a, upd, e0, e1);
+  tev_post(k, e0, (upd == 0 ? 6.0 : upd == 1 ? 10.0 : 12.0) * sn, sys);
+  tev_pre(k, 2, e0, e1);
+  launch_rows(k, a, e0, e1);
+  tev_post(k, e0, 9.0 * sn, sys);
   tev_pre(k, 3, e0, e1);
-  const int nb3 = k.colw == 16  ? launch_cols_fwd<R, 16>(k, y, out, mode, partner, ks, nsys, e0, e1)
-                  : k.colw == 4 ? launch_cols_fwd<R, 4>(k, y, out, mode, partner, ks, nsys, e0, e1)
-                                : launch_cols_fwd<R, 8>(k, y, out, mode, partner, ks, nsys, e0, e1);
-  tev_post(k, e0, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn);
+  const int nb3 = k.colw == 16  ? launch_cols_fwd<R, 16>(k, a, mode, e0, e1)
+                  : k.colw == 4 ? launch_cols_fwd<R, 4>(k, a, mode, e0, e1)
+                                : launch_cols_fwd<R, 8>(k, a, mode, e0, e1);
+  tev_post(k, e0, ((mode & 3) == 1 ? 10.0 : (mode & 3) == 2 ? 10.0 : 6.0) * sn, sys);
+  return nb3;
+}
+
+template <typename T>
+static T* shifted(T* p, long long d) {
+  return p ? p + d : p;
+}
+
+// One matvec on nsys systems (vectors k.vs apart, KScal slots slot .. slot+nsys-1).
+// mode bits: &3 = 0 no dots, 1 <partner, out>, 2 <out, partner> and <out, out>; &4 = no preconditioner
+// upd: 0 = apply to y; 1 = y is s = r - alpha v, 2 = y is p = r + beta (p - omega v), formed
+// (and stored) by the first kernel (launch_cols_inv).
+// With k.seq the systems run one after the other, each through all three kernels, over one
+// shared pair of intermediate fields: the intermediate written by one kernel is then read
+// by the next while it is still in the 256 MB infinity cache (at 1024^2, N = 2 in FP64 one
+// system's pair is 64 MB, three systems' 192 MB), and the pair is overwritten in that
+// cache rather than written back.  Each system's arithmetic is unchanged.
+template <typename R>
+static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mode, int slot,
+                         const cplx<R>* partner = nullptr, int nsys = 1, int upd = 0) {
+  MvArgs<R> a;
+  a.xin = upd == 0 ? y : k.r;
+  a.V = upd == 0 ? nullptr : k.v;
+  a.xo = upd == 1 ? k.s : upd == 2 ? k.p : nullptr;
+  a.Bx = k.Bx;
+  a.By = k.By;
+  a.y = y;
+  a.partner = partner;
+  a.out = out;
+  a.ks = k.ks + slot;
+  a.partials = k.partials;
+  a.nsys = nsys;
+  if (!k.seq || nsys == 1) return matvec_kernels(k, a, mode, upd, -1);
+  int nb3 = 0;
+  for (int z = 0; z < nsys; ++z) {
+    const long long d = z * k.vs;
+    MvArgs<R> b = a;
+    b.xin = shifted(a.xin, d);
+    b.V = shifted(a.V, d);
+    b.xo = shifted(a.xo, d);
+    b.y = shifted(a.y, d);
+    b.partner = shifted(a.partner, d);
+    b.out = shifted(a.out, d);
+    b.ks = a.ks + z;
+    b.partials = a.partials + z * k.ps;
+    b.nsys = 1;
+    nb3 = matvec_kernels(k, b, mode, upd, z);
+  }
   return nb3;
 }
 
@@ -727,55 +798,77 @@ static int effective_columns(int Ny) {
   }
 }
 
-// Autotuner of the matvec's column kernels: at the first solve of a handle, times three
-// pairs of matvecs in the shape of a step's batched solve (three systems, KScal slots 0-2,
-// on the right-hand sides in k.b, output into k.t; in FP64 with the s and p updates formed on
-// load, as an iteration runs them) with 8, 16 and 4 columns per workgroup (widths that
-// resolve to the kernel shape of a width already timed are skipped) and keeps the fastest.
-// One system alone favours narrow tiles (1024^2 FP64: 4 columns 0.085 against 0.089 ms per
-// matvec) that are slower over the three systems of a step.  All widths compute the same
-// numbers: the same transform per column, and the Krylov dot products are summed per column
-// in a fixed order.  The slots are reinitialised by the solve that follows.
+// Milliseconds per matvec of three pairs of matvecs in the shape of a step's batched solve
+// (three systems, KScal slots 0-2, on the right-hand sides in k.b, output into k.t; in FP64
+// with the s and p updates formed on load, as an iteration runs them), after one warm-up.
+// The slots are reinitialised by the solve that follows.
+template <typename R>
+static double time_matvec_pairs(KrylovCtx<R>& k, hipEvent_t a, hipEvent_t b) {
+  constexpr int nsys = 3;
+  constexpr int u1 = sizeof(R) == 8 ? 1 : 0, u2 = sizeof(R) == 8 ? 2 : 0;
+  const KScal zs[nsys] = {};
+  SAV_HIP(hipMemcpyAsync(k.ks, zs, sizeof(zs), hipMemcpyHostToDevice, k.st));
+  matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);  // warm-up
+  SAV_HIP(hipEventRecord(a, k.st));
+  for (int r = 0; r < 3; ++r) {
+    matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);
+    matvec_launch(k, k.b, k.t, 2, 0, k.b, nsys, u2);
+  }
+  SAV_HIP(hipEventRecord(b, k.st));
+  SAV_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  SAV_HIP(hipEventElapsedTime(&ms, a, b));
+  return ms / 6.0;
+}
+
+// Autotuner of the matvec at the first solve of a handle.  Column width: 8, 16 and 4 columns
+// per workgroup (widths that resolve to the kernel shape of a width already timed are
+// skipped), all systems per launch; the fastest is kept.  One system alone favours narrow
+// tiles (1024^2 FP64: 4 columns 0.085 against 0.089 ms per matvec) that are slower over the
+// three systems of a step, hence the three-system timing.  Order: with that width, all
+// systems per launch against system by system over one shared intermediate pair (k.seq, see
+// matvec_launch); the faster is kept.  Every choice computes the same numbers: the same
+// transform per column, the Krylov dot products summed per column in a fixed order.
 template <typename R>
 static void autotune_columns(KrylovCtx<R>& k) {
-  if (k.colw != 0) return;
-  const int e8 = effective_columns<R, 8>(k.Ny), e16 = effective_columns<R, 16>(k.Ny),
-            e4 = effective_columns<R, 4>(k.Ny);
-  if (e8 == e16 && e8 == e4) {
-    k.colw = 8;  // one kernel shape whatever the width: nothing to choose
+  if (k.colw != 0 && k.order != 0) {
+    k.seq = k.order == 2;
     return;
   }
   hipEvent_t a, b;
   SAV_HIP(hipEventCreate(&a));
   SAV_HIP(hipEventCreate(&b));
-  const int widths[3] = {8, 16, 4};
-  const bool timed_w[3] = {true, e16 != e8, e4 != e8 && e4 != e16};
   const int saved_timed = k.timed;
   k.timed = 0;
-  constexpr int nsys = 3;
-  constexpr int u1 = sizeof(R) == 8 ? 1 : 0, u2 = sizeof(R) == 8 ? 2 : 0;
-  const KScal zs[nsys] = {};
-  for (int w = 0; w < 3; ++w) {
-    if (!timed_w[w]) continue;
-    k.colw = widths[w];
-    SAV_HIP(hipMemcpyAsync(k.ks, zs, sizeof(zs), hipMemcpyHostToDevice, k.st));
-    matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);  // warm-up
-    SAV_HIP(hipEventRecord(a, k.st));
-    for (int r = 0; r < 3; ++r) {
-      matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);
-      matvec_launch(k, k.b, k.t, 2, 0, k.b, nsys, u2);
+  if (k.colw == 0) {
+    const int e8 = effective_columns<R, 8>(k.Ny), e16 = effective_columns<R, 16>(k.Ny),
+              e4 = effective_columns<R, 4>(k.Ny);
+    const int widths[3] = {8, 16, 4};
+    const bool timed_w[3] = {true, e16 != e8, e4 != e8 && e4 != e16};
+    k.seq = 0;
+    if (e8 == e16 && e8 == e4) {
+      k.colw = 8;  // one kernel shape whatever the width: nothing to choose
+    } else {
+      for (int w = 0; w < 3; ++w) {
+        if (!timed_w[w]) continue;
+        k.colw = widths[w];
+        k.tune_ms[w] = time_matvec_pairs(k, a, b);
+      }
+      int best = 0;
+      for (int w = 1; w < 3; ++w)
+        if (timed_w[w] && k.tune_ms[w] < k.tune_ms[best]) best = w;
+      k.colw = widths[best];
     }
-    SAV_HIP(hipEventRecord(b, k.st));
-    SAV_HIP(hipEventSynchronize(b));
-    float ms = 0.f;
-    SAV_HIP(hipEventElapsedTime(&ms, a, b));
-    k.tune_ms[w] = ms / 6.0;
   }
+  if (k.order == 0) {
+    for (int o = 0; o < 2; ++o) {
+      k.seq = o;
+      k.tune_order_ms[o] = time_matvec_pairs(k, a, b);
+    }
+    k.order = k.tune_order_ms[1] < k.tune_order_ms[0] ? 2 : 1;
+  }
+  k.seq = k.order == 2;
   k.timed = saved_timed;
-  int best = 0;
-  for (int w = 1; w < 3; ++w)
-    if (timed_w[w] && k.tune_ms[w] < k.tune_ms[best]) best = w;
-  k.colw = widths[best];
   SAV_HIP(hipEventDestroy(a));
   SAV_HIP(hipEventDestroy(b));
 }
@@ -849,7 +942,7 @@ static void kry_iteration(KrylovCtx<R>& k, int slot, int nsys, bool first) {
 
 template <typename R>
 void bicgstab_queue(KrylovCtx<R>& k, int slot, int nsys, int iters) {
-  if (k.colw == 0) throw HipError("bicgstab_queue: the column tile of the matvec is not chosen yet", -1);
+  if (k.colw == 0 || k.order == 0) throw HipError("bicgstab_queue: the matvec's tile and order are not chosen yet", -1);
   kry_start(k, slot, nsys);
   for (int it = 0; it < iters && it < k.itmax; ++it) kry_iteration(k, slot, nsys, it == 0);
 }

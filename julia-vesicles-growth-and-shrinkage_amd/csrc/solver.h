@@ -82,6 +82,10 @@ class ISolver {
   // psi matvec column-tile autotuner (2D)
   virtual int set_column_tile(int cw) = 0;
   virtual void column_tile(int* cw, double* ms8, double* ms16, double* ms4) = 0;
+  // order of the psi matvec's kernels over a batch of systems (2D): 0 = autotuned, 1 = all
+  // systems per launch, 2 = system by system over one shared intermediate pair
+  virtual int set_matvec_order(int order) = 0;
+  virtual void matvec_order(int* order, double* ms_batched, double* ms_sequenced) = 0;
   // column-transform tile autotuner (every transform outside the psi matvec)
   virtual int set_transform_tile(int cw) = 0;
   virtual void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) = 0;

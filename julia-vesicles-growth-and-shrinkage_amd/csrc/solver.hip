@@ -1185,6 +1185,19 @@ class Solver final : public ISolver {
     if (ms16) *ms16 = kc.tune_ms[1];
     if (ms4) *ms4 = kc.tune_ms[2];
   }
+  int set_matvec_order(int order) override {
+    if (nd != 2) return SAV_E_UNSUPPORTED;
+    if (order < 0 || order > 2) return SAV_E_ARG;
+    if (order != kc.order) drop_graphs();  // captured matvecs use the previous order
+    kc.order = order;
+    kc32.order = order;
+    return SAV_OK;
+  }
+  void matvec_order(int* order, double* ms_batched, double* ms_sequenced) override {
+    if (order) *order = kc.order;
+    if (ms_batched) *ms_batched = kc.tune_order_ms[0];
+    if (ms_sequenced) *ms_sequenced = kc.tune_order_ms[1];
+  }
 
   int precond_train(int iters, int batch, double lr, uint64_t seed, double* l0, double* l1) override {
     ensure_net();
@@ -2184,7 +2197,8 @@ class Solver final : public ISolver {
 
   bool graph_usable() const {
     // a mixed-precision step decides its rounds on the host: eager only
-    return graph_on && kc.timed == 0 && (nd == 3 || (!mixed && !use_learned && !collecting && kc.colw != 0));
+    return graph_on && kc.timed == 0 &&
+           (nd == 3 || (!mixed && !use_learned && !collecting && kc.colw != 0 && kc.order != 0));
   }
   void drop_graphs() {
     if (!graphs.empty()) SAV_HIP(hipStreamSynchronize(s));

@@ -69,3 +69,43 @@ def test_transform_tile_widths_give_identical_steps(jv, case):
     for w in (4, 8, 16):
         for name in ("phi", "psi", "mu", "nu", "u", "p", "phi_hat", "psi_hat"):
             assert np.array_equal(getattr(states[w], name), getattr(states[0], name)), (w, name)
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_matvec_order_gives_identical_steps(jv, precision):
+    """The psi matvec's kernels over the three systems of a step's batched solve run either
+    all systems per launch (order 1) or system by system over one shared intermediate pair
+    (order 2): three steps of two vesicles give bit-identical states and iteration counts in
+    both orders, and the autotuner (order 0) times both and keeps the faster."""
+    from helpers import device_state
+    wl = jv.workload_c3(n=128, steps=3)
+    conf = wl.conf
+    phi, psi = wl.initial_fields()
+    states, iters = {}, {}
+    for order in (0, 1, 2):
+        ops = jv.Operators(conf, precision=precision)
+        ops.set_state(jv.SAV_PRESENT, device_state(jv, ops, conf, phi, psi))
+        ops.init_sav()
+        if order:
+            ops.set_matvec_order(order)
+        its = []
+        for n in (1, 2, 3):
+            rc, st = ops.step(conf.dt, n)
+            assert rc == 0
+            its.append(tuple(st.krylov_iters))
+        info = ops.matvec_order_info()
+        states[order] = ops.get_state(jv.SAV_PRESENT)
+        iters[order] = its
+        ops.close()
+        if order == 0:
+            print(precision, info)
+            assert info["order"] in (1, 2)
+            assert info["ms_batched"] > 0 and info["ms_sequenced"] > 0
+            best = 1 if info["ms_batched"] <= info["ms_sequenced"] else 2
+            assert info["order"] == best
+        else:
+            assert info["order"] == order
+    assert iters[1] == iters[2] == iters[0]
+    for o in (0, 2):
+        for name in ("phi", "psi", "mu", "nu", "u", "p", "phi_hat", "psi_hat"):
+            assert np.array_equal(getattr(states[o], name), getattr(states[1], name)), (o, name)
