@@ -243,10 +243,11 @@ int sav_kernel_timing_get(sav_handle_t* h, int64_t* launches, double* total_ms, 
  * `prec_func!`/`P_inv_diag` plug-in of src/Solvers.jl:57-65 in the 2D psi solve).
  *   z = P ( r + F( g ) ),   g(x) = kappa sum_tap w_tap(x) Re F^{-1}(P r)(x + d(tap))
  * P the diagonal preconditioner, F the full 2D FFT, kappa = S4 max(M_psi) / (dx dy),
- * and w(x) = (1 - M_psi(x) / max M_psi) y(x), y(x) the nine weights of a 3x3 stencil at
- * pixel x (tap = di*3 + dj, offset (di-1, dj-1), periodic), produced once per mobility
- * field by a periodic 3x3
- * convolutional network 2 -> 16 -> 16 -> 16 -> 9 with GELU on the inputs
+ * and w(x) the weights at pixel x (tap = di*3 + dj, offset (di-1, dj-1), periodic) of the
+ * 5-point stencil of -div(c grad .) with face averages of a coefficient c(x) (the analytic
+ * first-order stencil is c = 1 - M_psi / max M_psi), c being the first output channel of a
+ * periodic 3x3 convolutional network 2 -> 16 -> 16 -> 16 -> 9 with GELU, run once per
+ * mobility field on the inputs
  * (4 (M_psi / max M_psi - 0.75), 1): BF16 weights and activations on the MFMA units,
  * FP32 accumulation.  The preconditioner is linear in r, as BiCGSTAB needs.  The
  * psi solve is then a flexible right-preconditioned BiCGSTAB that stops on the same
@@ -271,8 +272,10 @@ int sav_precond_load(sav_handle_t* h, const float* params, int n);
 int sav_precond_save(sav_handle_t* h, float* params, int n);
 /* on != 0: the following psi solves run with the diagonal preconditioner and log training
  * pairs: for every iterate x_k of a solve, e = x* - x_k, r = A e, and patches_per_snapshot
- * random 64 x 64 windows of the network input, v = Re F^{-1}(P r) / s and the target
- * Re F^{-1}(P^{-1} e - r) / (kappa s), s = rms of F^{-1}(P r).
+ * random 64 x 64 windows of the network input, v = Re F^{-1}(P r) / s and two targets:
+ * the correction Re F^{-1}(P^{-1} e - r) / (kappa s) of the solver's error, and the
+ * analytic first-order stencil's output on v (SAV_PRECOND_STENCIL's weights);
+ * s = rms of F^{-1}(P r).
  * The store is cleared when collection starts and holds up to 24576 windows.
  * on == 0 stops logging (the store is kept for sav_precond_train). */
 int sav_precond_collect(sav_handle_t* h, int on, int patches_per_snapshot, uint64_t seed);
@@ -281,11 +284,16 @@ int sav_precond_collect(sav_handle_t* h, int on, int patches_per_snapshot, uint6
  * repeated from x0 = 0 with the diagonal preconditioner (either pointer may be NULL). */
 int sav_precond_samples(sav_handle_t* h, int64_t* n_windows, int64_t* fallbacks);
 /* Adam (beta 0.9/0.999) on the masked MSE (4-pixel border excluded) of the stencil
- * prediction sum_tap w_tap v(x + d(tap)) against the target, over random batches of the
+ * prediction -div(c grad v) (the form above) against the target, over random batches of the
  * store, learning rate decaying from lr to lr/10 (cosine).  loss_first/loss_last are the
  * mean losses of the first and last 10 iterations. */
 int sav_precond_train(sav_handle_t* h, int iters, int batch, double lr, uint64_t seed, double* loss_first,
                       double* loss_last);
+/* Targets of the following sav_precond_train calls: 0 (the default) the solver's error
+ * corrections, 1 the analytic first-order stencil's output on the same windows
+ * (distillation: the network learns that stencil, which a 3x3 network of the mobility can
+ * represent, and can then be trained further on the error corrections). */
+int sav_precond_train_target(sav_handle_t* h, int target);
 /* Test hooks: the network on host images x[nimg][H][W][2] (channel fastest, rounded to
  * BF16) -> y[nimg][H][W][9]; and the masked MSE of y against t[nimg][H][W][9] with the
  * parameter gradient (flat layout).  H % 16 == 0, W % 32 == 0. */

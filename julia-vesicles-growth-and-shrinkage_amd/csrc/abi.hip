@@ -323,6 +323,11 @@ int sav_precond_samples(sav_handle_t* h, int64_t* n_windows, int64_t* fallbacks)
             if (fallbacks) *fallbacks = h->s->precond_fallbacks(); return SAV_OK);
 }
 
+int sav_precond_train_target(sav_handle_t* h, int target) {
+  if (!h) return SAV_E_ARG;
+  SAV_GUARD(h, return h->s->precond_train_target(target));
+}
+
 int sav_precond_train(sav_handle_t* h, int iters, int batch, double lr, uint64_t seed, double* loss_first,
                       double* loss_last) {
   if (!h) return SAV_E_ARG;

@@ -43,7 +43,8 @@ struct NetLayerDev {
 class Net {
  public:
   static constexpr int NL = 4;
-  // inputs: mobility channel and a constant channel; outputs: the 9 weights of a 3x3 stencil
+  // inputs: mobility channel and a constant channel; outputs: 9 channels, of which the
+  // preconditioner uses the first (the coefficient of its divergence-form stencil)
   static constexpr int CH[NL + 1] = {2, 16, 16, 16, 9};
   static constexpr int COUT = 9;
   static constexpr int NPARAM = 2 * 16 * 9 + 16 + 16 * 16 * 9 + 16 + 16 * 16 * 9 + 16 + 9 * 16 * 9 + 9;
@@ -64,11 +65,11 @@ class Net {
   // Masked MSE of the raw outputs against target[img][H][W][COUT] (test hook); gradients
   // into gW/gb; returns the loss (one device->host copy).
   double loss_backward(const bf16_t* x, const float* target, int nimg, int H, int W, int border);
-  // Masked MSE of the stencil prediction D[p] sum_tap y[p][tap] * v[p + d(tap)] (periodic
-  // in each image, D = 0.25 - x[p][0] / 4) against target[img][H][W]; gradients into
-  // gW/gb; returns the loss.
+  // Masked MSE of the divergence-form stencil prediction -div(c grad v) (5-point, face
+  // averages of the coefficient c = the first output channel; rx = dy/dx, ry = dx/dy;
+  // periodic in each image) against target[img][H][W]; gradients into gW/gb; returns the loss.
   double loss_backward_stencil(const bf16_t* x, const float* v, const float* target, int nimg, int H, int W,
-                               int border);
+                               int border, double rx, double ry);
   void adam_step(float lr, float beta1, float beta2, float eps);
   // gradient vector in the flat parameter layout (test hook)
   void get_grads(float* host);
@@ -88,6 +89,7 @@ class Net {
   bf16_t* g16b = nullptr;
   bf16_t* glast = nullptr; // output-layer gradient (COUT ch)
   float* yb = nullptr;
+  float* dres = nullptr;   // residuals of the divergence-form loss
   float* partials = nullptr;
   double* dloss = nullptr;
   int nb_loss = 0;

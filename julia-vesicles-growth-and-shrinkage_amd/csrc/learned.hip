@@ -245,42 +245,61 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const 
   if (threadIdx.x == 0) lp[blockIdx.x] = (sc[0] + sc[1] + sc[2] + sc[3]) / nint;
 }
 
-// Stencil loss: pred[p] = D[p] sum_tap w[p][tap] v[p + d(tap)] (tap = di*3 + dj, offset
-// (di - 1, dj - 1), periodic in each image), D = 0.25 - x[p][0] / 4 = 1 - M / M_max from
-// the mobility input channel; loss = sum_interior (pred - t)^2 / n_int;
-// g[p][tap] = 2 (pred - t) D[p] v[p + d(tap)] / n_int on the interior.
-__global__ __launch_bounds__(256) void k_stencil_mse(const bf16_t* __restrict__ x, const float* __restrict__ w,
-                                                     const float* __restrict__ v, const float* __restrict__ t,
-                                                     int nimg, int H, int W, int border, bf16_t* __restrict__ g,
-                                                     double* __restrict__ lp) {
+// Stencil loss in divergence form.  The network's first output channel is a coefficient
+// c(p); the prediction is the 5-point form of -div(c grad v) with face averages,
+//   pred[p] = rx/2 [(c_p + c_E)(v_p - v_E) + (c_p + c_W)(v_p - v_W)]
+//           + ry/2 [(c_p + c_N)(v_p - v_N) + (c_p + c_S)(v_p - v_S)]
+// (E/W = x -/+ 1 neighbours, N/S = y +/- 1, periodic in each image; rx = dy/dx, ry = dx/dy),
+// which is the analytic first-order stencil for c = 1 - M / M_max and, for any c, annihilates
+// constants and is symmetric, as the exact correction (a divergence) is.
+// k_div_pred: d[p] = pred[p] - t[p] on the interior (0 outside), loss partials.
+__global__ __launch_bounds__(256) void k_div_pred(const float* __restrict__ w, const float* __restrict__ v,
+                                                  const float* __restrict__ t, int nimg, int H, int W, int border,
+                                                  double rx, double ry, float* __restrict__ dres,
+                                                  double* __restrict__ lp) {
   __shared__ double sc[4];
   const long long n = (long long)nimg * H * W;
   const double nint = (double)nimg * (H - 2 * border) * (W - 2 * border);
   double acc = 0.0;
   for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long long)gridDim.x * 256) {
     const int xc = (int)(p % W), yy = (int)((p / W) % H);
-    const long long img = p / ((long long)H * W);
+    const long long b = p - (long long)yy * W - xc;  // first pixel of the image
     const bool in = xc >= border && xc < W - border && yy >= border && yy < H - border;
-    float vn[9];
-    double pred = 0.0;
-    const double Dp = 0.25 - 0.25 * (double)bf2f(x[2 * p]);
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-      const int sy = (yy + tap / 3 - 1 + H) % H, sx = (xc + tap % 3 - 1 + W) % W;
-      vn[tap] = v[(img * H + sy) * W + sx];
-      pred += (double)w[p * 9 + tap] * (double)vn[tap];
-    }
-    pred *= Dp;
-    const double d = pred - (double)t[p];
-    acc += in ? d * d : 0.0;
-    const double gs = in ? 2.0 * d * Dp / nint : 0.0;
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) g[p * 9 + tap] = f2bf((float)(gs * vn[tap]));
+    const long long pe = b + (long long)yy * W + (xc + 1) % W, pw = b + (long long)yy * W + (xc + W - 1) % W;
+    const long long pn = b + (long long)((yy + 1) % H) * W + xc, ps = b + (long long)((yy + H - 1) % H) * W + xc;
+    const double c = w[p * 9], vp = v[p];
+    const double pred = 0.5 * rx * ((c + w[pe * 9]) * (vp - v[pe]) + (c + w[pw * 9]) * (vp - v[pw])) +
+                        0.5 * ry * ((c + w[pn * 9]) * (vp - v[pn]) + (c + w[ps * 9]) * (vp - v[ps]));
+    const double d = in ? pred - (double)t[p] : 0.0;
+    dres[p] = (float)d;
+    acc += d * d;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if ((threadIdx.x & 63) == 0) sc[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) lp[blockIdx.x] = (sc[0] + sc[1] + sc[2] + sc[3]) / nint;
+}
+
+// k_div_grad: output gradient of the divergence-form loss, d loss / d c(q) collects the
+// five predictions c(q) enters (q and its four neighbours); the other channels get 0.
+__global__ __launch_bounds__(256) void k_div_grad(const float* __restrict__ v, const float* __restrict__ dres, int nimg,
+                                                  int H, int W, int border, double rx, double ry,
+                                                  bf16_t* __restrict__ g) {
+  const long long n = (long long)nimg * H * W;
+  const double nint = (double)nimg * (H - 2 * border) * (W - 2 * border);
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+    const int xc = (int)(q % W), yy = (int)((q / W) % H);
+    const long long b = q - (long long)yy * W - xc;
+    const long long pe = b + (long long)yy * W + (xc + 1) % W, pw = b + (long long)yy * W + (xc + W - 1) % W;
+    const long long pn = b + (long long)((yy + 1) % H) * W + xc, ps = b + (long long)((yy + H - 1) % H) * W + xc;
+    const double vq = v[q], ve = v[pe], vw = v[pw], vn = v[pn], vs = v[ps];
+    const double gq = (double)dres[q] * 0.5 * (rx * (2.0 * vq - ve - vw) + ry * (2.0 * vq - vn - vs)) +
+                      0.5 * rx * ((double)dres[pw] * (vw - vq) + (double)dres[pe] * (ve - vq)) +
+                      0.5 * ry * ((double)dres[ps] * (vs - vq) + (double)dres[pn] * (vn - vq));
+    g[q * 9] = f2bf((float)(2.0 * gq / nint));
+#pragma unroll
+    for (int ch = 1; ch < 9; ++ch) g[q * 9 + ch] = f2bf(0.f);
+  }
 }
 
 __global__ void k_sum_d(const double* __restrict__ p, int n, double* __restrict__ out) {
@@ -373,6 +392,7 @@ void Net::ensure(long long npix) {
   g16b = alloc<bf16_t>(npix * 16);
   glast = alloc<bf16_t>(npix * COUT);
   yb = alloc<float>(npix * COUT);
+  dres = alloc<float>(npix);
   const long long nblk = (npix + CTX * CTY - 1) / (CTX * CTY);
   partials = alloc<float>(nblk * (16 * 144 + 16));
 }
@@ -497,13 +517,14 @@ double Net::loss_backward(const bf16_t* x, const float* target, int nimg, int H,
 }
 
 double Net::loss_backward_stencil(const bf16_t* x, const float* v, const float* target, int nimg, int H, int W,
-                                  int border) {
+                                  int border, double rx, double ry) {
   forward(x, nimg, H, W, nullptr, true);
   const long long n = (long long)nimg * H * W;
   int nb = (int)((n + 1023) / 1024);
   if (nb > 4096) nb = 4096;
-  hipLaunchKernelGGL(k_stencil_mse, dim3(nb), dim3(256), 0, st, x, yb, v, target, nimg, H, W, border, glast,
-                     dloss);
+  hipLaunchKernelGGL(k_div_pred, dim3(nb), dim3(256), 0, st, yb, v, target, nimg, H, W, border, rx, ry, dres, dloss);
+  hipLaunchKernelGGL(k_div_grad, dim3(nb), dim3(256), 0, st, v, dres, nimg, H, W, border, rx, ry, glast);
+  SAV_LAUNCH_CHECK();
   nb_loss = nb;
   return backward(x, nimg, H, W);
 }

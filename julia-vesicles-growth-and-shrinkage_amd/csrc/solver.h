@@ -98,6 +98,9 @@ class ISolver {
   // ensemble members, which overlap with each other, keep one stream each.
   virtual void set_step_fork(bool on) = 0;
   virtual void graph_info(long long* replays, long long* repeats, long long* graphs) = 0;
+  // targets of the following precond_train calls: 0 the solver's error corrections, 1 the
+  // analytic first-order stencil's output on the same windows
+  virtual int precond_train_target(int kind) = 0;
   virtual int precond_train(int iters, int batch, double lr, uint64_t seed, double* loss_first,
                             double* loss_last) = 0;
   virtual int net_forward(const float* x, int nimg, int Hh, int Ww, float* y) = 0;

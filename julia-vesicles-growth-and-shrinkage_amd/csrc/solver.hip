@@ -754,16 +754,22 @@ class Solver final : public ISolver {
   // Learned psi preconditioner (no reference counterpart).
   //   z = P ( r + F( g ) ),  g(x) = kappa * sum_tap w_tap(x) * Re F^{-1}(P r)(x + d(tap))
   // P = the diagonal preconditioner of Solvers.jl:57, F = the full 2D FFT,
-  // kappa = S4 * M_max / (dx dy), and w(x) = (1 - M_psi / M_max) net(4 (M_psi / M_max -
-  // 0.75), 1) the nine weights of a 3x3 stencil per pixel (zero where M_psi = M_max, as
-  // the exact correction is), computed once per M_psi (once per step).  The
+  // kappa = S4 * M_max / (dx dy), and w(x) the 5-point stencil of -div(c grad .) with face
+  // averages of a coefficient c(x) = net(4 (M_psi / M_max - 0.75), 1) (the first output
+  // channel of the network), computed once per M_psi (once per step).  The form is that of
+  // the analytic first-order stencil (c = 1 - M_psi / M_max): for any c it annihilates
+  // constants and is symmetric, as the exact correction (a divergence) is, so an error of
+  // the network perturbs the differences of v, not v itself.  The
   // preconditioner is therefore a fixed LINEAR operator during a solve, which BiCGSTAB
   // needs.  Its target: A^{-1} = P (I + (P^{-1} - A) A^{-1}) and
-  // (P^{-1} - A) e = S4 div((M_max - M_psi) grad e), a local operator of e ~ P r; the
-  // network is trained on pairs from diagonal solves, e = x* - x_k (error of iterate k),
-  // r = A e, target Re F^{-1}(P^{-1} e - r) / kappa.  The outer P damps the BF16 noise
-  // of the stencil at high k.  The flexible BiCGSTAB (fbicgstab_2d) that uses it stops on
-  // the same diagonally preconditioned residual as the reference's solve.
+  // (P^{-1} - A) e = S4 div((M_max - M_psi) grad e), a local operator of e ~ P r.  The
+  // network is trained on windows logged from diagonal solves, e = x* - x_k (error of
+  // iterate k), r = A e, v = Re F^{-1}(P r), against one of two targets: the correction
+  // Re F^{-1}(P^{-1} e - r) / kappa of the solver's own error, or the analytic
+  // first-order stencil's output on the same v (distillation: that stencil, a function of
+  // M_psi on a 3x3 neighbourhood, is within the network's reach).  The outer P damps the
+  // BF16 noise of the stencil at high k.  The flexible BiCGSTAB (fbicgstab_2d) that uses
+  // it stops on the same diagonally preconditioned residual as the reference's solve.
   // ==========================================================================
   std::unique_ptr<Net> net;
   bool use_learned = false, collecting = false, stencil_stale = true;
@@ -779,7 +785,9 @@ class Solver final : public ISolver {
   long long store_cap = 0, store_n = 0;
   bf16_t* st_in = nullptr;   // [store_cap][PATCH][PATCH][2]
   float* st_v = nullptr;     // [store_cap][PATCH][PATCH]
-  float* st_tg = nullptr;    // [store_cap][PATCH][PATCH]
+  float* st_tg = nullptr;    // [store_cap][PATCH][PATCH]: correction of the solver's error
+  float* st_ta = nullptr;    // [store_cap][PATCH][PATCH]: the analytic first-order stencil's
+  int train_target = 0;      // targets of precond_train: 0 st_tg, 1 st_ta
   int* d_pos = nullptr;      // window origins (image, y0, x0)
   std::vector<int> h_pos;
   bf16_t* bt_in = nullptr;   // training batch
@@ -877,15 +885,34 @@ class Solver final : public ISolver {
         wi[8] = 0.f;
       });
     } else {
+      // the network's coefficient c (first output channel) in the same divergence form:
+      // g = -S4 div(c grad v), the analytic stencil being c = 1 - M / M_max
       mobility_input(lp_in);
       net->forward(lp_in, N, Ny, Nx, lp_W, false);
-      const R* M = w_M;
-      const DScal* dsl = ds;
       float* w = lp_W;
+      float* cf = reinterpret_cast<float*>(lp_tmp);  // c of every pixel (lp_tmp is free here)
+      foreach_n(s, kc.nfull, [=] __device__(long long i) { cf[i] = w[i * 9]; });
+      const int Nxl = Nx, Nyl = Ny;
+      const double rx = (c.Ly / Ny) / (c.Lx / Nx), ry = 1.0 / rx;  // dy/dx, dx/dy
       foreach_n(s, kc.nfull, [=] __device__(long long i) {
-        const float D = (float)(1.0 - (double)M[i] / dsl->Mmax);
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) w[i * 9 + tap] *= D;
+        const int ix = (int)(i % Nxl);
+        const long long row = i / Nxl;
+        const int iy = (int)(row % Nyl);
+        const long long base = (row - iy) * Nxl;
+        auto C_ = [&](int y, int x) { return (double)cf[base + (long long)((y + Nyl) % Nyl) * Nxl + (x + Nxl) % Nxl]; };
+        const double cc = C_(iy, ix);
+        const double de = 0.5 * (cc + C_(iy, ix + 1)) * rx, dw = 0.5 * (cc + C_(iy, ix - 1)) * rx;
+        const double dn = 0.5 * (cc + C_(iy + 1, ix)) * ry, dsv = 0.5 * (cc + C_(iy - 1, ix)) * ry;
+        float* wi = w + i * 9;
+        wi[0] = 0.f;
+        wi[1] = (float)(-dsv);
+        wi[2] = 0.f;
+        wi[3] = (float)(-dw);
+        wi[4] = (float)(de + dw + dn + dsv);
+        wi[5] = (float)(-de);
+        wi[6] = 0.f;
+        wi[7] = (float)(-dn);
+        wi[8] = 0.f;
       });
     }
     stencil_stale = false;
@@ -979,7 +1006,10 @@ class Solver final : public ISolver {
       bf16_t* tin = st_in + store_n * (long long)(2 * PATCH * PATCH);
       float* tv = st_v + store_n * (long long)(PATCH * PATCH);
       float* ttg = st_tg + store_n * (long long)(PATCH * PATCH);
+      float* tta = st_ta + store_n * (long long)(PATCH * PATCH);
+      const R* M = w_M;
       const int Nxl = Nx, Nyl = Ny;
+      const double rx = (c.Ly / Ny) / (c.Lx / Nx), ry = 1.0 / rx;  // dy/dx, dx/dy
       foreach_n(s, np * PATCH * PATCH, [=] __device__(long long i) {
         const long long p = i / (PATCH * PATCH);
         const int py = (int)((i / PATCH) % PATCH), px = (int)(i % PATCH);
@@ -991,6 +1021,18 @@ class Solver final : public ISolver {
         tin[2 * i + 1] = xin[2 * src + 1];
         tv[i] = (float)((double)yc[src].x * iy);
         ttg[i] = (float)((double)d[src].x * iy / (kom * dsl->Mmax));
+        // the analytic first-order stencil's prediction at this pixel (refresh_stencil's
+        // weights on the same v): the distillation target
+        const long long fb = (long long)img * Nyl * Nxl;
+        auto at = [&](int y2, int x2) { return fb + (long long)((y2 + Nyl) % Nyl) * Nxl + (x2 + Nxl) % Nxl; };
+        auto Dm = [&](int y2, int x2) { return 1.0 - (double)M[at(y2, x2)] / dsl->Mmax; };
+        auto V = [&](int y2, int x2) { return (double)yc[at(y2, x2)].x; };
+        const double dc = Dm(yy, xx);
+        const double de = 0.5 * (dc + Dm(yy, xx + 1)) * rx, dw = 0.5 * (dc + Dm(yy, xx - 1)) * rx;
+        const double dn = 0.5 * (dc + Dm(yy + 1, xx)) * ry, dsv = 0.5 * (dc + Dm(yy - 1, xx)) * ry;
+        const double an = (de + dw + dn + dsv) * V(yy, xx) - de * V(yy, xx + 1) - dw * V(yy, xx - 1) -
+                          dn * V(yy + 1, xx) - dsv * V(yy - 1, xx);
+        tta[i] = (float)(an * iy);
       });
       store_n += np;
     }
@@ -1040,6 +1082,7 @@ class Solver final : public ISolver {
       st_in = dalloc<bf16_t>(store_cap * 2 * PATCH * PATCH);
       st_v = dalloc<float>(store_cap * PATCH * PATCH);
       st_tg = dalloc<float>(store_cap * PATCH * PATCH);
+      st_ta = dalloc<float>(store_cap * PATCH * PATCH);
     }
     if (!d_pos || per_snapshot > ppsnap) d_pos = dalloc<int>(3LL * per_snapshot);
     ppsnap = per_snapshot;
@@ -1219,7 +1262,7 @@ class Solver final : public ISolver {
       SAV_HIP(hipMemcpyAsync(d_idx, h_idx.data(), batch * sizeof(int), hipMemcpyHostToDevice, s));
       const int* idx = d_idx;
       const bf16_t* sin = st_in;
-      const float *sv = st_v, *stg = st_tg;
+      const float *sv = st_v, *stg = train_target == 1 ? st_ta : st_tg;
       bf16_t* bin = bt_in;
       float *bv = bt_v, *btg = bt_tg;
       foreach_n(s, (long long)batch * PATCH * PATCH, [=] __device__(long long i) {
@@ -1230,7 +1273,9 @@ class Solver final : public ISolver {
         bv[i] = sv[src];
         btg[i] = stg[src];
       });
-      const double loss = net->loss_backward_stencil(bt_in, bt_v, bt_tg, batch, PATCH, PATCH, PATCH_BORDER);
+      const double rx = (c.Ly / Ny) / (c.Lx / Nx);
+      const double loss =
+          net->loss_backward_stencil(bt_in, bt_v, bt_tg, batch, PATCH, PATCH, PATCH_BORDER, rx, 1.0 / rx);
       if (it < 10) {
         s0 += loss;
         ++n0;
@@ -1248,6 +1293,12 @@ class Solver final : public ISolver {
     stencil_stale = true;
     if (l0) *l0 = s0 / (n0 > 0 ? n0 : 1);
     if (l1) *l1 = s1 / (n1 > 0 ? n1 : 1);
+    return SAV_OK;
+  }
+
+  int precond_train_target(int kind) override {
+    if (kind != 0 && kind != 1) return SAV_E_ARG;
+    train_target = kind;
     return SAV_OK;
   }
 

@@ -79,7 +79,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
            "sav_set_column_tile", "sav_autotune_info", "sav_column_tile_info", "sav_set_matvec_order", "sav_matvec_order_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
-           "sav_precond_samples", "sav_precond_train", "sav_net_forward", "sav_net_grad",
+           "sav_precond_samples", "sav_precond_train", "sav_precond_train_target", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
            "sav_ensemble_member", "sav_ensemble_run")
 
@@ -144,6 +144,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_precond_load.argtypes = [ct.c_void_p, _fp, ct.c_int]
     lib.sav_precond_save.argtypes = [ct.c_void_p, _fp, ct.c_int]
     lib.sav_precond_collect.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_uint64]
+    lib.sav_precond_train_target.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_precond_samples.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]
     lib.sav_precond_train.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double, ct.c_uint64, _dp, _dp]
     lib.sav_net_forward.argtypes = [ct.c_void_p, _fp, ct.c_int, ct.c_int, ct.c_int, _fp]
@@ -517,6 +518,11 @@ class Operators:
         n = ct.c_int64(0)
         self._check(self.lib.sav_precond_samples(self.h, None, ct.byref(n)), "precond_samples")
         return n.value
+
+    def precond_train_target(self, target: int):
+        """Targets of the following training: 0 the solver's error corrections, 1 the analytic
+        first-order stencil's output on the same windows (distillation)."""
+        self._check(self.lib.sav_precond_train_target(self.h, int(target)), "precond_train_target")
 
     def precond_train(self, iters: int = 1000, batch: int = 32, lr: float = 2e-3, seed: int = 7):
         l0, l1 = ct.c_double(0), ct.c_double(0)
