@@ -93,7 +93,10 @@ typedef struct sav_step_stats {
 int sav_abi_version(void);
 
 /* Builds the handle: the `Operators` bundle of build_operators (src/SpectralUtils.jl:36-118),
- * device state, twiddles and work buffers.  Replaces Config(...)+build_operators(conf). */
+ * device state, twiddles and work buffers.  Replaces Config(...)+build_operators(conf).
+ * Grid lengths (FFTW's plans accept any, SpectralUtils.jl:61-71): Nx even, and Nx, Nx/2, Ny
+ * (and Nz) each 2^a, 3 2^a or 5 2^a up to 1280, or 2048; in 2D Nx, Ny >= 4.  Other lengths
+ * return SAV_E_ARG with the rule in sav_last_error. */
 int sav_create(const sav_config_t* cfg, sav_handle_t** out);
 
 /* Frees the handle and its device memory. */

@@ -2,6 +2,7 @@
 #include <cstring>
 #include <string>
 
+#include "krylov.h"
 #include "solver.h"
 
 struct sav_handle {
@@ -42,7 +43,10 @@ int sav_create(const sav_config_t* cfg, sav_handle_t** out) {
     if (cfg->ndim != 2 && cfg->ndim != 3) throw std::invalid_argument("ndim must be 2 or 3");
     if (cfg->N < 1 || cfg->N > sav::MAXN) throw std::invalid_argument("N out of range");
     if (!size_ok(cfg->Nx) || !sav::fft_size_supported(cfg->Ny) || (cfg->ndim == 3 && !sav::fft_size_supported(cfg->Nz)))
-      throw std::invalid_argument("grid sizes must be 2^a 3^b, Nx even, Nx/2, Ny, Nz <= 1024");
+      throw std::invalid_argument(
+          "grid sizes: Nx even with Nx, Nx/2, Ny (and Nz) in {2^a, 3 2^a, 5 2^a up to 1280, 2048}");
+    if (cfg->ndim == 2 && (!sav::kry_size_supported(cfg->Nx) || !sav::kry_size_supported(cfg->Ny)))
+      throw std::invalid_argument("2D grids need Nx, Ny in {4 .. 2048} of the form 2^a, 3 2^a or 5 2^a (psi solve)");
     if (cfg->ndim == 2 && cfg->Ny < 4) throw std::invalid_argument("Ny must be >= 4");
     if (!cfg->psi_in_v || !cfg->psi_out_v) throw std::invalid_argument("psi_in_v/psi_out_v required");
     if (cfg->precision != SAV_PREC_FP64 && cfg->precision != SAV_PREC_FP32 && cfg->precision != SAV_PREC_MIXED)

@@ -1,4 +1,4 @@
-// Batched mixed-radix (2/3/4/8, optionally 16/32) Stockham FFTs for gfx950.
+// Batched mixed-radix (2/3/4/5/8, optionally 16/32) Stockham FFTs for gfx950.
 //
 // One transform of length L is carried by a "team" of TT = L/E threads, each
 // holding E elements in registers.  A radix-R pass makes every thread do E/R
@@ -33,17 +33,20 @@ struct FftPlanData {
 // first pass of a reversed plan has the radix of the last pass of the plain one).
 constexpr FftPlanData make_fft_plan(int L, int ttmax = 64, int maxr = 8, bool rev = false) {
   FftPlanData p{};
-  int n = L, np = 0, has3 = 0;
+  int n = L, np = 0, has3 = 0, has5 = 0;
   while (n % 3 == 0) { p.rad[np++] = 3; n /= 3; has3 = 1; }
+  while (n % 5 == 0) { p.rad[np++] = 5; n /= 5; has5 = 1; }
   int a = 0;
   while (n % 2 == 0) { n /= 2; ++a; }
-  const int lg = maxr >= 32 ? 5 : maxr >= 16 ? 4 : 3;
+  // with a radix-5 pass the power-of-two passes are at most radix 4, so that a lane's
+  // elements (a multiple of every radix) stay at 20
+  const int lg = has5 ? 2 : maxr >= 32 ? 5 : maxr >= 16 ? 4 : 3;
   int used = 1;
   const int nbig = a / lg, rem = a % lg;
   for (int i = 0; i < nbig; ++i) { p.rad[np++] = 1 << lg; used = 1 << lg; }
   if (rem > 0) { p.rad[np++] = 1 << rem; if (used < (1 << rem)) used = 1 << rem; }
   p.np = np;
-  int lcm = has3 ? 3 * used : used;
+  int lcm = (has3 ? 3 : 1) * (has5 ? 5 : 1) * used;
   if (lcm < 1) lcm = 1;
   int E = lcm;
   while (L / E > ttmax && (L % (2 * E)) == 0 && 2 * E <= 32) E *= 2;
@@ -70,9 +73,13 @@ template <typename R>
 constexpr int col_plan_maxr(int L, bool fwd) {
   return ((L & (L - 1)) == 0 && L >= 512) ? ((fwd || sizeof(R) == 8) ? 16 : 32) : 8;
 }
+// (FP64 columns of 2048 points run 128-lane teams, E = 16: with 64-lane teams a lane would
+// hold 32 complex doubles per field.)
 template <typename R>
 constexpr int col_plan_ttmax(int L, bool fwd) {
-  return col_plan_maxr<R>(L, fwd) == 32 ? 32 : col_plan_maxr<R>(L, fwd) == 16 ? 64 : (sizeof(R) == 8 ? 128 : 64);
+  return col_plan_maxr<R>(L, fwd) == 32                                ? 32
+         : col_plan_maxr<R>(L, fwd) == 16 ? ((sizeof(R) == 8 && L >= 2048) ? 128 : 64)
+                                           : (sizeof(R) == 8 ? 128 : 64);
 }
 
 // Largest radix of the general column transforms (k_cols: y and z passes, the full
@@ -81,11 +88,13 @@ constexpr int col_plan_ttmax(int L, bool fwd) {
 constexpr int cols_maxr(int L, bool fp32) { return (fp32 && (L & (L - 1)) == 0 && L >= 512) ? 16 : 8; }
 
 
+// Lengths with a plan: 2^a 3^b or 2^a 5^c (3 and 5 are not mixed) up to 2048.
 constexpr bool fft_len_supported(int L) {
-  int n = L;
-  while (n % 3 == 0) n /= 3;
+  int n = L, f3 = 0, f5 = 0;
+  while (n % 3 == 0) { n /= 3; f3 = 1; }
+  while (n % 5 == 0) { n /= 5; f5 = 1; }
   while (n % 2 == 0) n /= 2;
-  return n == 1 && L >= 2 && L <= 1024;
+  return n == 1 && !(f3 && f5) && L >= 2 && L <= 2048;
 }
 
 // ---------------------------------------------------------------------------
@@ -108,6 +117,30 @@ __device__ __forceinline__ void bfly3(cplx<R>* a) {
   cplx<R> t2 = INV ? mul_i(d) : mul_mi(d);
   a[1] = t1 + t2;
   a[2] = t1 - t2;
+}
+
+// 5-point DFT from its definition: with b1 = a1 + a4, b2 = a2 + a3, d1 = a1 - a4,
+// d2 = a2 - a3, c_m = cos(2 pi m / 5), s_m = sin(2 pi m / 5):
+//   X0 = a0 + b1 + b2,  X1,4 = (a0 + c1 b1 + c2 b2) -/+ i (s1 d1 + s2 d2),
+//   X2,3 = (a0 + c2 b1 + c1 b2) -/+ i (s2 d1 - s1 d2)   (signs of i flipped for INV).
+template <typename R, bool INV>
+__device__ __forceinline__ void bfly5(cplx<R>* a) {
+  const R c1 = R(0.309016994374947424102293417182819058860);
+  const R c2 = R(-0.809016994374947424102293417182819058860);
+  const R s1 = R(0.951056516295153572116439333379382143406);
+  const R s2 = R(0.587785252292473129168705954639072768598);
+  const cplx<R> b1 = a[1] + a[4], b2 = a[2] + a[3], d1 = a[1] - a[4], d2 = a[2] - a[3];
+  const cplx<R> t1 = a[0] + c1 * b1 + c2 * b2;
+  const cplx<R> t2 = a[0] + c2 * b1 + c1 * b2;
+  const cplx<R> u1 = s1 * d1 + s2 * d2;
+  const cplx<R> u2 = s2 * d1 - s1 * d2;
+  const cplx<R> iu1 = INV ? mul_i(u1) : mul_mi(u1);
+  const cplx<R> iu2 = INV ? mul_i(u2) : mul_mi(u2);
+  a[0] = a[0] + b1 + b2;
+  a[1] = t1 + iu1;
+  a[4] = t1 - iu1;
+  a[2] = t2 + iu2;
+  a[3] = t2 - iu2;
 }
 
 template <typename R, bool INV>
@@ -211,6 +244,7 @@ __device__ __forceinline__ void bfly(cplx<R>* a) {
   if constexpr (RAD == 2) bfly2<R, INV>(a);
   else if constexpr (RAD == 3) bfly3<R, INV>(a);
   else if constexpr (RAD == 4) bfly4<R, INV>(a[0], a[1], a[2], a[3]);
+  else if constexpr (RAD == 5) bfly5<R, INV>(a);
   else if constexpr (RAD == 8) bfly8<R, INV>(a);
   else if constexpr (RAD == 16) bfly_comp<R, INV, 4, 4>(a);
   else if constexpr (RAD == 32) bfly_comp<R, INV, 8, 4>(a);

@@ -11,6 +11,8 @@ namespace sav {
 // FP32 keeps 8 (4 columns: 35.7 -> 39.0 us inverse).
 // The width is also tunable at run time (FftGeom::colw, 4 / 8 / 16; ColCfgW), since each
 // column's transform is the same whichever tile it shares: the results are bit-identical.
+// k_cols is instantiated with CW = the resolved column count (ColCfgW<R, L, CW>::C is a
+// fixed point), so widths that resolve to the same tile share one kernel.
 template <typename R, int L, int CW = 0>
 using ColCfgW = ColTile<R, L, CW != 0 ? CW : ((sizeof(R) == 8 && L >= 512) ? 4 : 8), cols_maxr(L, sizeof(R) == 4)>;
 template <typename R, int L>
@@ -23,7 +25,7 @@ int col_tile_columns(int L, int real_bytes, int cw) {
     if (real_bytes == 8)                                                                                   \
       return cw == 4 ? ColCfgW<double, LL, 4>::C : cw == 16 ? ColCfgW<double, LL, 16>::C : cw == 8 ? ColCfgW<double, LL, 8>::C : ColCfgW<double, LL, 0>::C; \
     return cw == 4 ? ColCfgW<float, LL, 4>::C : cw == 16 ? ColCfgW<float, LL, 16>::C : cw == 8 ? ColCfgW<float, LL, 8>::C : ColCfgW<float, LL, 0>::C;
-    X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+    X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(2048)
 #undef X
     default: return 0;
   }
@@ -682,7 +684,7 @@ __global__ __launch_bounds__(kHPlaneNT, 4) void k_hplane_c2r(SpecJobs jobs, int 
 // Host dispatch
 // ---------------------------------------------------------------------------
 #define SAV_FFT_SIZES(X) \
-  X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+  X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(1280) X(2048)
 
 bool fft_size_supported(int n) {
   switch (n) {
@@ -709,7 +711,7 @@ static void launch_cols_w(int L, const SpecJobs& jobs, const ColGeom& cg, const 
   case LL: {                                                                                  \
     using CC = ColCfgW<R, LL, CW>;                                                            \
     dim3 grid((cg.inner + CC::C - 1) / CC::C, mc * cg.nsub, jobs.n);                          \
-    hipLaunchKernelGGL((k_cols<R, LL, INV, CW>), grid, dim3(CC::NT), 0, s, jobs, cg, tw, kt); \
+    hipLaunchKernelGGL((k_cols<R, LL, INV, CC::C>), grid, dim3(CC::NT), 0, s, jobs, cg, tw, kt); \
     break;                                                                                    \
   }
     SAV_FFT_SIZES(X)

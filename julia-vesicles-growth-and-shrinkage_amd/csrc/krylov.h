@@ -64,6 +64,9 @@ struct KrylovCtx {
   const KScal* outer = nullptr;
 };
 
+// Grid lengths (Nx, Ny) the 2D psi solve is built for.
+bool kry_size_supported(int n);
+
 // Solves A x = b with b already in ctx.b; x in ctx.x on return.  Returns the
 // number of iterations; ctx.hks[slot] holds the final Krylov scalars.
 template <typename R>

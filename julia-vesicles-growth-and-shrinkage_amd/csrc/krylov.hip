@@ -27,6 +27,8 @@ namespace sav {
 // handle by timing them at the first solve, see autotune_columns), the column plan of
 // col_plan_maxr / col_plan_ttmax (FWD: the forward kernel's plan) with k-major twiddles
 // (KrylovCtx::tw_ci, tw_cf).
+// The kernels are instantiated with CW = the tile's resolved column count (KColCfg<R, L, CW>::C
+// is a fixed point), so widths that resolve to the same tile share one kernel.
 template <typename R, int L, int CW = 8, bool FWD = false>
 using KColCfg = ColTile<R, L, CW, col_plan_maxr<R>(L, FWD), col_plan_ttmax<R>(L, FWD)>;
 template <typename R, int L, bool INV>
@@ -564,7 +566,20 @@ static int vec_blocks(long long n) {
   return (int)nb;
 }
 
-#define SAV_KRY_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+#define SAV_KRY_SIZES(X)                                                                                         \
+  X(4) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) \
+  X(384) X(512) X(640) X(768) X(1024) X(1280) X(2048)
+
+// Grid lengths the psi solve's matvec kernels are built for (Nx and Ny of a 2D grid).
+bool kry_size_supported(int n) {
+  switch (n) {
+#define X(L) case L:
+    SAV_KRY_SIZES(X)
+#undef X
+    return true;
+    default: return false;
+  }
+}
 
 // Start/stop events of the next launch of kernel `id` when it is the timed one and the
 // launch is sampled (null otherwise). They are passed to hipExtLaunchKernel, which stamps
@@ -633,7 +648,7 @@ static void launch_cols_inv(KrylovCtx<R>& k, const MvArgs<R>& a, int upd, hipEve
   if (upd != 0 && sizeof(R) != 8) throw HipError("launch_cols_inv: the fused vector update is built for FP64 only", -1);
   const long long vs = k.vs;
 #define KL(L, U)                                                                                                   \
-  hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CW, U>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.xin, a.V, a.xo, a.Bx, \
+  hipExtLaunchKernelGGL((k_mv_cols_inv<R, L, CC::C, U>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.xin, a.V, a.xo, a.Bx, \
                         a.By, k.Nx, k.S4, k.kx0, k.ky, k.tw_ci, a.ks, vs)
   switch (k.Ny) {
 #define X(L)                                                  \
@@ -665,7 +680,7 @@ static int launch_cols_fwd(KrylovCtx<R>& k, const MvArgs<R>& a, int mode, hipEve
   const long long vs = k.vs, ps = k.ps;
   int nb3 = 0;
 #define KL(L, M)                                                                                                 \
-  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CW>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, a.y,      \
+  hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CC::C>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, a.y,   \
                         a.partner, a.out, k.Nx, k.S4, k.kx0, k.ky, k.tw_cf, k.ds, a.ks, a.partials, vs, ps)
   switch (k.Ny) {
 #define X(L)                                            \

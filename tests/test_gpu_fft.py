@@ -22,7 +22,9 @@ def _conf(jv, Nx, Ny, N=1, Nz=None):
                           Nx=Nx, Ny=Ny, Lx=2 * np.pi, Ly=2 * np.pi, tol=1e-10)
 
 
-SIZES_2D = [(8, 6), (16, 12), (128, 64), (64, 128), (96, 48), (256, 256), (384, 384), (512, 512), (1024, 1024)]
+SIZES_2D = [(8, 6), (16, 12), (128, 64), (64, 128), (96, 48), (256, 256), (384, 384), (512, 512), (1024, 1024),
+            # radix 5 (lengths 5 2^a) and 2048
+            (10, 20), (40, 80), (160, 320), (640, 40), (1280, 160), (2048, 64), (64, 2048)]
 
 
 @pytest.mark.parametrize("Nx,Ny", SIZES_2D)
@@ -43,7 +45,7 @@ def test_rfft_irfft_vs_numpy(jv, Nx, Ny, prec):
     ops.close()
 
 
-@pytest.mark.parametrize("Nx,Ny", [(8, 6), (128, 64), (1024, 1024)])
+@pytest.mark.parametrize("Nx,Ny", [(8, 6), (128, 64), (1024, 1024), (80, 40), (2048, 32), (40, 2048), (1280, 20)])
 def test_fft_full_vs_numpy(jv, Nx, Ny):
     conf = _conf(jv, Nx, Ny, N=2)
     ops = jv.Operators(conf)
@@ -57,7 +59,7 @@ def test_fft_full_vs_numpy(jv, Nx, Ny):
 # Square (Nx == Ny) planes of up to 128 run the fused (x, y) plane kernels; the others
 # (8, and Nx != Ny) the separate row and column passes.
 SIZES_3D = [(8, 8, 8), (16, 16, 16), (32, 32, 32), (64, 64, 16), (96, 96, 24), (128, 128, 128),
-            (24, 16, 12), (32, 64, 16)]
+            (24, 16, 12), (32, 64, 16), (20, 40, 10), (80, 80, 20)]
 
 
 @pytest.mark.parametrize("Nx,Ny,Nz", SIZES_3D)

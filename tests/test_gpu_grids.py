@@ -55,6 +55,26 @@ def test_rectangular_and_mixed_radix_grids(jv, so, Nx, Ny):
     _check(jv, so, conf, _vesicles(jv, 1, 2))
 
 
+# radix-5 lengths (5 2^a: rows of 40 -> 20-point transforms, columns of 80, 160, 320, 1280)
+# and 2048-point rows and columns; a few steps each against the oracle
+@pytest.mark.parametrize("Nx,Ny", [(80, 40), (40, 160), (320, 320), (1280, 40), (40, 1280), (2048, 64), (64, 2048)])
+def test_radix5_and_2048_grids(jv, so, Nx, Ny):
+    conf = _conf(jv, Nx, Ny, 1, eps=0.3)
+    _check(jv, so, conf, _vesicles(jv, 1, 2), nsteps=2)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(80, 40), (2048, 64)])
+def test_radix5_and_2048_grids_mixed(jv, so, Nx, Ny):
+    """The mixed form's FP32 inner psi iterations on the new lengths: two steps within the
+    solve tolerance of the oracle's FP64 steps."""
+    conf = _conf(jv, Nx, Ny, 1, eps=0.3)
+    phi, psi = jv.initial_fields(conf, _vesicles(jv, 1, 2), 1.0)
+    for k, pres, old, out, g, st, rc in paired_runs(jv, so, conf, phi, psi, 2, precision=2):
+        assert rc >= 0
+        for name in ("phi", "psi"):
+            assert relerr(getattr(g, name).T, getattr(pres, name)) < 1e-8, (k, name)
+
+
 def test_three_vesicles(jv, so):
     conf = _conf(jv, 96, 48, 3, eps=0.2)
     _check(jv, so, conf, _vesicles(jv, 3, 2))
