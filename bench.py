@@ -47,6 +47,27 @@ def _sync():
     torch.cuda.synchronize()
 
 
+def copy_ceiling_gbs(nbytes=1 << 30, reps=5):
+    """Rate of a plain device copy of `nbytes` (read + write, buffers four times the 256 MB
+    infinity cache), best of `reps`: what HBM sustains for a streaming kernel, beside the
+    8 TB/s peak the roofline fraction is taken against."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
+
+
 def _setup(jv, wl, precision):
     conf = wl.conf
     phi, psi = wl.initial_fields()
@@ -472,7 +493,8 @@ def main():
         rg, sg = run_case(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup, graph=True)
         out["2d"] = dict(fp64=r64, timed=timed, mixed=rmx, fp32=r32, graph=rg,
                          mixed_err=_errors(smx, s64), fp32_err=_errors(s32, s64),
-                         graph_equal=all(np.array_equal(getattr(sg, k), getattr(s64, k)) for k in FIELD_KEYS))
+                         graph_equal=all(np.array_equal(getattr(sg, k), getattr(s64, k)) for k in FIELD_KEYS),
+                         copy_gb_s=copy_ceiling_gbs())
         if not a.no_learned:
             out["2d"]["learned"] = run_case_learned(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup)
     if a.only in ("all", "3d"):
@@ -518,6 +540,9 @@ def main():
                             "traffic_over_algorithmic": ratio,
                             "avg_launch_ms": round(k["avg_ms"], 5), "launches": k["launches"],
                             "bytes_per_launch": round(k["bytes"])}
+        if "copy_gb_s" in o:
+            line["roofline"]["copy_gb_s_1GiB"] = round(o["copy_gb_s"], 1)
+            line["roofline"]["frac_of_copy"] = round(achieved / o["copy_gb_s"], 4)
         line["psi_matvec_kernels_fp64_1024"] = {
             v["name"]: {"avg_launch_ms": round(v["avg_ms"], 5), "bytes_per_launch": round(v["bytes"]),
                         "achieved_gb_s": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)}
