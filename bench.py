@@ -112,6 +112,7 @@ def run_case(jv, wl, precision, steps, warmup, timed_kernel=0, graph=False):
         res["kernel"] = dict(name=ops.KERNELS[timed_kernel], launches=n, avg_ms=ms / max(n, 1), bytes=b)
     res["autotune"] = ops.autotune_info() if conf.ndim == 2 else None
     res["matvec_order"] = ops.matvec_order_info() if conf.ndim == 2 else None
+    res["transform_tiles"] = dict(inverse=ops.transform_tile_info(), forward=ops.transform_tile_fwd_info())
     res["plane_kernel"] = ops.plane_kernel_info() if conf.ndim == 3 else None
     if graph:
         res["graph"] = ops.graph_info()
@@ -565,6 +566,8 @@ def main():
             tune["note"] = ("a width whose kernels resolve to those of a width already timed is not timed "
                             "(ms 0): at 1024 in FP64, 160 KB of LDS holds at most 8 columns, so 16 is 8")
         line["psi_matvec_column_tile_1024"] = tune
+        tt = o["fp64"]["transform_tiles"]
+        line["transform_tiles_fp64_1024"] = {d: {"columns": v["columns"], **{f"ms_{c}": round(v[f"ms_{c}"], 4) for c in (4, 8, 16)}} for d, v in tt.items()}
         mo = o["fp64"]["matvec_order"]
         line["psi_matvec_order_1024"] = {
             "order": {1: "all systems per launch", 2: "system by system (shared intermediate pair)"}.get(mo["order"], mo["order"]),

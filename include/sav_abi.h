@@ -191,14 +191,17 @@ int sav_set_matvec_order(sav_handle_t* h, int order);
 int sav_matvec_order_info(sav_handle_t* h, int32_t* order, double* ms_batched, double* ms_sequenced);
 
 /* Autotuner of every other column (strided-axis) transform (no reference counterpart): at a
- * handle's first step an irfft and an rfft of the present phi are timed with 4, 8 and 16
- * columns per workgroup (widths that resolve to the same tile for this grid are timed once)
- * and the fastest is kept; all widths compute bit-identical results.
- * sav_set_transform_tile(h, 4 | 8 | 16) fixes the width, 0 retunes at the next step.
- * sav_transform_tile_info returns the width in use (0 before the first step) and the measured
- * milliseconds of each width (-1: same tile as a smaller width, 0: not timed yet). */
+ * handle's first step inverse transforms (irfft of N and of 4N fields of the present state)
+ * and forward transforms (rfft of the same) are timed with 4, 8 and 16 columns per workgroup
+ * (widths that resolve to the same tile for this grid are timed once) and the fastest width
+ * of each direction is kept; all widths compute bit-identical results.
+ * sav_set_transform_tile(h, 4 | 8 | 16) fixes both widths, 0 retunes at the next step.
+ * sav_transform_tile_info (inverse) and sav_transform_tile_fwd_info (forward) return the
+ * width in use (0 before the first step) and the measured milliseconds of each width (-1:
+ * same tile as a smaller width, 0: not timed yet). */
 int sav_set_transform_tile(sav_handle_t* h, int columns);
 int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16);
+int sav_transform_tile_fwd_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16);
 
 /* (x, y) plane kernels of 3D grids with 128 x 128 planes (no reference counterpart; they
  * carry SpectralUtils_3d.jl's rfft/irfft over x and y).  SAV_PLANE_REGISTER keeps the plane's

@@ -271,6 +271,14 @@ int sav_transform_tile_info(sav_handle_t* h, int32_t* columns, double* ms_4, dou
   return SAV_OK;
 }
 
+int sav_transform_tile_fwd_info(sav_handle_t* h, int32_t* columns, double* ms_4, double* ms_8, double* ms_16) {
+  if (!h) return SAV_E_ARG;
+  int cw = 0;
+  SAV_GUARD(h, h->s->transform_tile_fwd(&cw, ms_4, ms_8, ms_16));
+  if (columns) *columns = cw;
+  return SAV_OK;
+}
+
 int sav_set_plane_kernel(sav_handle_t* h, int kind) {
   if (!h) return SAV_E_ARG;
   int rc = 0;

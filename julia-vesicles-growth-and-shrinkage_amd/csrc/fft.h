@@ -545,7 +545,8 @@ struct FftGeom {
   const cplx<R>* tw_y;    // table for L = Ny
   const cplx<R>* tw_z;    // table for L = Nz
   KTables k;
-  int colw = 0;           // columns per workgroup of the column transforms: 0 = default rule, 4 / 8 / 16
+  int colw = 0;           // columns per workgroup of the inverse column transforms: 0 = default rule, 4 / 8 / 16
+  int colw_fwd = 0;       // the same for the forward column transforms
   const cplx<R>* tw_hp = nullptr;  // hplane_twiddles() for 128 x 128 planes of 3D fields
   int plane_kind = 0;     // 128 x 128 planes: 0 = register-column kernels, 1 = whole plane in LDS
 };

@@ -876,18 +876,18 @@ void rfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs, hipStream_t s) {
     launch_plane<R>(g, jobs, false, nullptr, 1.0, s);
     ColGeom cz = geom_z(g);
     cz.use_src = 0;
-    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s, g.colw);
+    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s, g.colw_fwd);
     return;
   }
   const int rows = g.Ny * (g.ndim == 3 ? g.Nz : 1);
   launch_r2c<R>(g.Nx / 2, jobs, rows, g.tw_xh, g.tw_x2, s);
   ColGeom cy = geom_y(g);
   cy.use_src = 0;
-  launch_cols<R, false>(g.Ny, jobs, cy, g.tw_y, g.k, s, g.colw);
+  launch_cols<R, false>(g.Ny, jobs, cy, g.tw_y, g.k, s, g.colw_fwd);
   if (g.ndim == 3) {
     ColGeom cz = geom_z(g);
     cz.use_src = 0;
-    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s, g.colw);
+    launch_cols<R, false>(g.Nz, jobs, cz, g.tw_z, g.k, s, g.colw_fwd);
   }
 }
 
@@ -942,7 +942,7 @@ void fft_full_2d(const FftGeom<R>& g, cplx<R>* data, int count, bool inverse, hi
   c.use_src = 0;
   if (!inverse) {
     launch_rows_c2c<R, false>(g.Nx, data, nrows, 1.0, g.tw_x, s);
-    launch_cols<R, false>(g.Ny, jobs, c, g.tw_y, g.k, s, g.colw);
+    launch_cols<R, false>(g.Ny, jobs, c, g.tw_y, g.k, s, g.colw_fwd);
   } else {
     launch_cols<R, true>(g.Ny, jobs, c, g.tw_y, g.k, s, g.colw);
     launch_rows_c2c<R, true>(g.Nx, data, nrows, 1.0 / ((double)g.Nx * g.Ny), g.tw_x, s);

@@ -88,7 +88,8 @@ class ISolver {
   virtual void matvec_order(int* order, double* ms_batched, double* ms_sequenced) = 0;
   // column-transform tile autotuner (every transform outside the psi matvec)
   virtual int set_transform_tile(int cw) = 0;
-  virtual void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) = 0;
+  virtual void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) = 0;      // inverse
+  virtual void transform_tile_fwd(int* cw, double* ms4, double* ms8, double* ms16) = 0;  // forward
   // (x, y) plane kernels of 128 x 128 x Nz grids: 0 = autotuned, 1 = register columns with a
   // half plane in LDS, 2 = the whole plane in LDS
   virtual int set_plane_kernel(int kind) = 0;

@@ -1104,13 +1104,15 @@ class Solver final : public ISolver {
   }
 
   // Autotuner of the column transforms outside the psi matvec (steps 1-7, back-transforms,
-  // monitor): at a handle's first step an irfft (with the Laplacian multiplier) and an rfft of
-  // the present phi are timed with 4, 8 and 16 columns per workgroup, and the fastest width is
-  // kept for every later transform of the handle.  Widths that resolve to the same tile for
-  // this grid's column lengths are timed once.  Each column is transformed by the same team
-  // code whatever tile it shares, so every width gives bit-identical results.
-  int tcolw = 0;
-  double tune_t_ms[3] = {0.0, 0.0, 0.0};
+  // monitor): at a handle's first step inverse transforms (with the Laplacian multiplier) and
+  // forward transforms of the present state are timed with 4, 8 and 16 columns per workgroup,
+  // and the fastest width of each direction is kept for every later transform of the handle.
+  // Widths that resolve to the same tile for this grid's column lengths are timed once.  Each
+  // column is transformed by the same team code whatever tile it shares, so every width gives
+  // bit-identical results.
+  int tcolw = 0, tcolw_f = 0;                 // inverse and forward widths
+  double tune_t_ms[3] = {0.0, 0.0, 0.0};      // inverse transforms with 4, 8, 16 columns
+  double tune_f_ms[3] = {0.0, 0.0, 0.0};      // forward transforms
   int pkind = 0;  // plane kernels: 0 = tune at the next step, 1 = register columns, 2 = LDS plane
   double tune_p_ms[2] = {0.0, 0.0};
   // Milliseconds of one irfft and one rfft of the present phi (three repeats after a warm-up).
@@ -1174,45 +1176,62 @@ class Solver final : public ISolver {
     hipEvent_t a, b;
     SAV_HIP(hipEventCreate(&a));
     SAV_HIP(hipEventCreate(&b));
-    double best = 1e300;
-    int best_w = 0;
+    // the inverse and the forward transforms are timed apart (their best widths differ: at
+    // 1024^2 in FP64 the inverse is faster with 4 columns at every field count, the forward
+    // one with 8), each over a launch of N fields and one of 4N (the step's launches carry
+    // 2 to 11 N-field jobs); w_Hh is recomputed by step 1
+    double best_i = 1e300, best_f = 1e300;
+    int wi = 0, wf = 0;
     std::vector<int> seen;
     for (int i = 0; i < 3; ++i) {
       const int w = widths[i];
       const int kk = key(w);
       if (std::find(seen.begin(), seen.end(), kk) != seen.end()) {
-        tune_t_ms[i] = -1.0;  // the same tile as a width already timed
+        tune_t_ms[i] = tune_f_ms[i] = -1.0;  // the same tile as a width already timed
         continue;
       }
       seen.push_back(kk);
-      g.colw = w;
-      auto once = [&]() {
+      g.colw = g.colw_fwd = w;
+      auto inv = [&]() {
         irfft({job(fhalf(ip, F_PHI), w_d4, N, MUL_LAP)});
-        rfft({job(fphys(ip, F_PHI), w_phat, N)});
+        irfft({job(fhalf(ip, F_PHI), w_d4, 4 * N, MUL_LAP)});
       };
-      once();  // warm-up
-      SAV_HIP(hipEventRecord(a, s));
-      for (int r = 0; r < 3; ++r) once();
-      SAV_HIP(hipEventRecord(b, s));
-      SAV_HIP(hipEventSynchronize(b));
-      float ms = 0.f;
-      SAV_HIP(hipEventElapsedTime(&ms, a, b));
-      tune_t_ms[i] = ms / 3.0;
-      if (tune_t_ms[i] < best) {
-        best = tune_t_ms[i];
-        best_w = w;
+      auto fwd = [&]() {
+        rfft({job(fphys(ip, F_PHI), w_phat, N)});
+        rfft({job(fphys(ip, F_PHI), w_Hh, 4 * N)});
+      };
+      for (int dir = 0; dir < 2; ++dir) {
+        if (dir == 0) inv(); else fwd();  // warm-up
+        SAV_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < 3; ++r) {
+          if (dir == 0) inv(); else fwd();
+        }
+        SAV_HIP(hipEventRecord(b, s));
+        SAV_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        SAV_HIP(hipEventElapsedTime(&ms, a, b));
+        const double t = ms / 3.0;
+        if (dir == 0) {
+          tune_t_ms[i] = t;
+          if (t < best_i) { best_i = t; wi = w; }
+        } else {
+          tune_f_ms[i] = t;
+          if (t < best_f) { best_f = t; wf = w; }
+        }
       }
     }
     SAV_HIP(hipEventDestroy(a));
     SAV_HIP(hipEventDestroy(b));
-    tcolw = best_w;
+    tcolw = wi;
+    tcolw_f = wf;
     g.colw = tcolw;
+    g.colw_fwd = tcolw_f;
   }
   int set_transform_tile(int cw) override {
     if (cw != 0 && cw != 4 && cw != 8 && cw != 16) return SAV_E_ARG;
-    if (cw != tcolw) drop_graphs();  // captured transforms use the previous width
-    tcolw = cw;
-    g.colw = cw;
+    if (cw != tcolw || cw != tcolw_f) drop_graphs();  // captured transforms use the previous widths
+    tcolw = tcolw_f = cw;
+    g.colw = g.colw_fwd = cw;
     return SAV_OK;
   }
   void transform_tile(int* cw, double* ms4, double* ms8, double* ms16) override {
@@ -1220,6 +1239,12 @@ class Solver final : public ISolver {
     if (ms4) *ms4 = tune_t_ms[0];
     if (ms8) *ms8 = tune_t_ms[1];
     if (ms16) *ms16 = tune_t_ms[2];
+  }
+  void transform_tile_fwd(int* cw, double* ms4, double* ms8, double* ms16) override {
+    if (cw) *cw = tcolw_f;
+    if (ms4) *ms4 = tune_f_ms[0];
+    if (ms8) *ms8 = tune_f_ms[1];
+    if (ms16) *ms16 = tune_f_ms[2];
   }
 
   void column_tile(int* cw, double* ms8, double* ms16, double* ms4) override {

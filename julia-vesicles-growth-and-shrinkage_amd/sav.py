@@ -78,7 +78,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft", "sav_irfft_noalloc",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
-           "sav_set_column_tile", "sav_autotune_info", "sav_column_tile_info", "sav_set_matvec_order", "sav_matvec_order_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
+           "sav_set_column_tile", "sav_autotune_info", "sav_column_tile_info", "sav_set_matvec_order", "sav_matvec_order_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_transform_tile_fwd_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
            "sav_precond_samples", "sav_precond_train", "sav_precond_train_target", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
            "sav_ensemble_member", "sav_ensemble_run")
@@ -135,6 +135,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_matvec_order_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
     lib.sav_set_transform_tile.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_transform_tile_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp, _dp]
+    lib.sav_transform_tile_fwd_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp, _dp]
     lib.sav_set_plane_kernel.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_plane_kernel_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
     lib.sav_set_graph.argtypes = [ct.c_void_p, ct.c_int]
@@ -461,6 +462,13 @@ class Operators:
         m4, m8, m16 = ct.c_double(0), ct.c_double(0), ct.c_double(0)
         self._check(self.lib.sav_transform_tile_info(self.h, ct.byref(cw), ct.byref(m4), ct.byref(m8), ct.byref(m16)),
                     "transform_tile_info")
+        return dict(columns=cw.value, ms_4=m4.value, ms_8=m8.value, ms_16=m16.value)
+
+    def transform_tile_fwd_info(self):
+        cw = ct.c_int32(0)
+        m4, m8, m16 = ct.c_double(0), ct.c_double(0), ct.c_double(0)
+        self._check(self.lib.sav_transform_tile_fwd_info(self.h, ct.byref(cw), ct.byref(m4), ct.byref(m8), ct.byref(m16)),
+                    "transform_tile_fwd_info")
         return dict(columns=cw.value, ms_4=m4.value, ms_8=m8.value, ms_16=m16.value)
 
     def set_plane_kernel(self, kind: int):

@@ -56,16 +56,18 @@ def test_transform_tile_widths_give_identical_steps(jv, case):
         for n in (1, 2):
             rc, st = ops.step(conf.dt, n)
             assert rc == 0
-        info = ops.transform_tile_info()
+        infos = (ops.transform_tile_info(), ops.transform_tile_fwd_info())
         states[w] = ops.get_state(jv.SAV_PRESENT)
         ops.close()
-        if w == 0:
-            print(case, info)
-            assert info["columns"] in (4, 8, 16)
-            timed = [info[f"ms_{c}"] for c in (4, 8, 16)]
-            assert max(timed) > 0 and info[f"ms_{info['columns']}"] > 0
-        else:
-            assert info["columns"] == w
+        for info in infos:  # inverse and forward widths
+            if w == 0:
+                print(case, info)
+                assert info["columns"] in (4, 8, 16)
+                timed = [info[f"ms_{c}"] for c in (4, 8, 16)]
+                assert max(timed) > 0 and info[f"ms_{info['columns']}"] > 0
+                assert info[f"ms_{info['columns']}"] == min(t for t in timed if t > 0)
+            else:
+                assert info["columns"] == w
     for w in (4, 8, 16):
         for name in ("phi", "psi", "mu", "nu", "u", "p", "phi_hat", "psi_hat"):
             assert np.array_equal(getattr(states[w], name), getattr(states[0], name)), (w, name)
