@@ -176,7 +176,7 @@ def run_case_learned(jv, wl, precision, steps, warmup, collect_steps=4, train_it
     return res
 
 
-def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False):
+def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False, batched=True):
     """The 64-instance ensemble (16 each of 128^2, 256^2, 384^2, 512^2, 1-3 vesicles):
     aggregate steps/s for each host-thread count (1 = the sequential per-instance loop),
     after one warm-up step of every member."""
@@ -212,6 +212,25 @@ def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False):
         B = sum(algorithmic_bytes(w.conf, sb, float(k) / steps) * steps for w, k in zip(wls, kit))
         res[th] = dict(seconds=el, steps_per_s=len(wls) * steps / el, rc=int(rc), psi_iters=int(kit.sum()),
                        algorithmic_tb_per_s=B / el / 1e12)
+    # size classes: each step's psi solves batched over the members of equal grid and N
+    # (one host thread per member); one warm-up step tunes the groups' matvecs
+    if not batched:
+        ens.close()
+        return dict(instances=len(wls), steps=steps, by_threads=res)
+    groups = ens.set_batched(True)
+    ens.run(np.arange(n0 - 1, n0 + 1) * dt, n0)
+    n0 += 1
+    t = np.arange(n0 - 1, n0 + steps) * dt
+    _sync()
+    t0 = time.perf_counter()
+    rc, E, A, kit, st = ens.run(t, n0)
+    _sync()
+    el = time.perf_counter() - t0
+    gof, _ = ens.batched_info()
+    sb = 4 if precision == jv.SAV_PREC_FP32 else 8
+    B = sum(algorithmic_bytes(w.conf, sb, float(k) / steps) * steps for w, k in zip(wls, kit))
+    batched = dict(seconds=el, steps_per_s=len(wls) * steps / el, rc=int(rc), psi_iters=int(kit.sum()),
+                   algorithmic_tb_per_s=B / el / 1e12, groups=int(groups), grouped_members=int((gof >= 0).sum()))
     ens.close()
     best = max(res, key=lambda k: res[k]["steps_per_s"])
     sizes = sorted({w.conf.Nx for w in wls})
@@ -219,7 +238,7 @@ def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False):
     per = {n: sum(algorithmic_bytes(w.conf, sb, 20.0) for w in wls if w.conf.Nx == n) for n in sizes}
     tot = sum(per.values())
     return dict(instances=len(wls), steps=steps, by_threads=res, best_threads=best,
-                speedup_vs_sequential=res[best]["steps_per_s"] / res[1]["steps_per_s"],
+                speedup_vs_sequential=res[best]["steps_per_s"] / res[1]["steps_per_s"], batched=batched,
                 byte_share_by_grid={f"{n}^2": round(v / tot, 4) for n, v in per.items()})
 
 
@@ -392,7 +411,9 @@ def run_configs(jv, cpu=True):
                                sequential_steps_per_s=round(ens["by_threads"][1]["steps_per_s"], 2),
                                threads16_steps_per_s=round(ens["by_threads"][16]["steps_per_s"], 2),
                                speedup_vs_sequential=round(ens["speedup_vs_sequential"], 3),
-                               status=[v["rc"] for v in ens["by_threads"].values()])
+                               batched_size_classes_steps_per_s=round(ens["batched"]["steps_per_s"], 2),
+                               batched_groups=ens["batched"]["groups"],
+                               status=[v["rc"] for v in ens["by_threads"].values()] + [ens["batched"]["rc"]])
     return rows
 
 
@@ -604,7 +625,7 @@ def main():
                                             "fp32": o["fp32"]["energy_monotone"]}
     if a.only == "all":
         ens = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64)
-        ensg = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64, threads_list=(1, 16), graph=True)
+        ensg = run_ensemble(jv, a.ensemble_steps, jv.SAV_PREC_FP64, threads_list=(1, 16), graph=True, batched=False)
         line["ensemble_64_fp64"] = {"instances": ens["instances"], "steps": ens["steps"],
                                     "sequential_steps_per_s": round(ens["by_threads"][1]["steps_per_s"], 2),
                                     "best_steps_per_s": round(ens["by_threads"][ens["best_threads"]]["steps_per_s"], 2),
@@ -616,6 +637,11 @@ def main():
                                     "byte_share_by_grid": ens["byte_share_by_grid"],
                                     "graph_replay_by_threads": {k: round(v["steps_per_s"], 2)
                                                                 for k, v in ensg["by_threads"].items()}}
+        bt = ens["batched"]
+        line["ensemble_64_fp64"]["batched_size_classes"] = {
+            "steps_per_s": round(bt["steps_per_s"], 2), "groups": bt["groups"],
+            "grouped_members": bt["grouped_members"], "algorithmic_tb_per_s": round(bt["algorithmic_tb_per_s"], 3),
+            "speedup_vs_best_threads": round(bt["steps_per_s"] / ens["by_threads"][ens["best_threads"]]["steps_per_s"], 3)}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(jv, wl_head if "2d" in out else jv.workload_c4(n=128, steps=1),
                                             nsteps=a.cpu_steps if "2d" in out else 2)

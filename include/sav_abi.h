@@ -326,6 +326,20 @@ sav_handle_t* sav_ensemble_member(sav_ensemble_t* e, int i);
 int sav_ensemble_run(sav_ensemble_t* e, const double* t_nodes, int n_nodes, int n0, int n_threads,
                      double* energy_hist, double* area_hist, int32_t* krylov_iters, int32_t* status);
 
+/* Size classes (no reference counterpart): with on = 1, members with the same 2D grid, vesicle
+ * count, storage precision (FP64 or FP32 fields; not mixed), S4 and psi-solve settings
+ * (diagonal preconditioner, tol, atol, itmax) form groups of 2 to 64 whose three psi solves per
+ * step run as ONE batched BiCGSTAB over all their systems (every Krylov and matvec launch
+ * covers the size class, the system on a grid dimension).  Each member's iterates, iteration
+ * counts and states stay bit-identical to its own solves.  sav_ensemble_run then steps every
+ * member in a host thread of its own (n_threads is not used); members meet at each step's
+ * solve, which the last to arrive runs.  The groups' pools replace the members' own psi-solve
+ * buffers and stay in use with on = 0 (members then solve on their own again).  n_groups
+ * returns the number of groups.  sav_ensemble_batched_info: each member's group (-1: none)
+ * and the number of batched solves its group ran (either array may be NULL). */
+int sav_ensemble_set_batched(sav_ensemble_t* e, int on, int32_t* n_groups);
+int sav_ensemble_batched_info(sav_ensemble_t* e, int32_t* group_of, int64_t* solves);
+
 #ifdef __cplusplus
 }
 #endif

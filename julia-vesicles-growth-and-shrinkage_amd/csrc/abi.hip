@@ -17,6 +17,8 @@ namespace sav {
 void set_step_fork(sav_handle_t* h, bool on) {
   if (h && h->s) h->s->set_step_fork(on);
 }
+
+ISolver* handle_solver(sav_handle_t* h) { return h ? h->s.get() : nullptr; }
 }  // namespace sav
 
 #define SAV_GUARD(h, body)                                  \

@@ -7,19 +7,30 @@
 // BiCGSTAB iterations), the kernels of the other members keep the GPU busy.  Small
 // grids are latency-bound on their own, so overlapping instances is what fills the
 // 256 CUs.  Members are handed out largest grid first so the long runs start early.
+// With sav_ensemble_set_batched the members of each size class (same grid, vesicle
+// count and solve settings) also share their psi solves: one batched BiCGSTAB per step
+// over all their systems (sav::ISolveGroup), each member stepping in a thread of its own.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "krylov.h"
 #include "solver.h"
 
 struct sav_ensemble {
   std::vector<sav_handle_t*> members;
+  std::vector<sav_config_t> cfgs;
   std::vector<long long> work;  // grid points x vesicles, for the hand-out order
   std::string err;
+  // size-class groups (sav_ensemble_set_batched): group_of[i] = group index or -1, slot_of[i]
+  // = the member's index in its group
+  std::vector<std::unique_ptr<sav::ISolveGroup>> groups;
+  std::vector<int> group_of, slot_of;
+  bool batched = false;
 };
 
 static thread_local std::string g_ens_error;
@@ -41,6 +52,7 @@ int sav_ensemble_create(const sav_config_t* cfgs, int n, sav_ensemble_t** out) {
     }
     sav::set_step_fork(h, false);  // members overlap with each other: one stream each
     e->members.push_back(h);
+    e->cfgs.push_back(cfgs[i]);
     const long long pts = (long long)cfgs[i].Nx * cfgs[i].Ny * (cfgs[i].ndim == 3 ? cfgs[i].Nz : 1);
     e->work.push_back(pts * cfgs[i].N);
   }
@@ -50,8 +62,72 @@ int sav_ensemble_create(const sav_config_t* cfgs, int n, sav_ensemble_t** out) {
 
 int sav_ensemble_destroy(sav_ensemble_t* e) {
   if (!e) return SAV_OK;
-  for (auto* m : e->members) sav_destroy(m);
+  for (auto* m : e->members) sav_destroy(m);  // the members first: they use the groups' pools
+  e->groups.clear();
   delete e;
+  return SAV_OK;
+}
+
+int sav_ensemble_set_batched(sav_ensemble_t* e, int on, int32_t* n_groups) {
+  if (!e || (on != 0 && on != 1)) return SAV_E_ARG;
+  try {
+    const int n = (int)e->members.size();
+    if (on && e->groups.empty()) {
+      // size classes: members whose psi solves can share launches (same 2D grid, vesicle
+      // count, storage precision, S4 and solve settings); at most 64 members per group
+      e->group_of.assign(n, -1);
+      e->slot_of.assign(n, -1);
+      std::vector<bool> taken(n, false);
+      for (int i = 0; i < n; ++i) {
+        if (taken[i]) continue;
+        const sav_config_t& a = e->cfgs[i];
+        if (a.ndim != 2 || a.precision == SAV_PREC_MIXED || a.precond != SAV_PRECOND_DIAG) continue;
+        std::vector<int> cls;
+        for (int j = i; j < n; ++j) {
+          const sav_config_t& b = e->cfgs[j];
+          if (taken[j] || b.ndim != 2 || b.precision != a.precision || b.precond != SAV_PRECOND_DIAG) continue;
+          if (b.Nx != a.Nx || b.Ny != a.Ny || b.N != a.N || b.S4 != a.S4 || b.tol != a.tol || b.atol != a.atol ||
+              b.itmax != a.itmax || b.device != a.device)
+            continue;
+          cls.push_back(j);
+        }
+        for (size_t c0 = 0; c0 < cls.size(); c0 += sav::KRY_MAX_SYS / 3) {
+          const size_t c1 = std::min(cls.size(), c0 + (size_t)sav::KRY_MAX_SYS / 3);
+          if (c1 - c0 < 2) continue;
+          std::vector<sav::ISolver*> ss;
+          for (size_t k = c0; k < c1; ++k) ss.push_back(sav::handle_solver(e->members[cls[k]]));
+          auto grp = sav::make_solve_group(ss);
+          for (size_t k = c0; k < c1; ++k) {
+            e->group_of[cls[k]] = (int)e->groups.size();
+            e->slot_of[cls[k]] = (int)(k - c0);
+            taken[cls[k]] = true;
+          }
+          e->groups.push_back(std::move(grp));
+        }
+      }
+    }
+    for (auto& g : e->groups) g->set_coupled(on != 0);
+    e->batched = on != 0;
+    if (n_groups) *n_groups = (int32_t)e->groups.size();
+    return SAV_OK;
+  } catch (const std::exception& ex) {
+    e->err = ex.what();
+    return SAV_E_ARG;
+  }
+}
+
+int sav_ensemble_batched_info(sav_ensemble_t* e, int32_t* group_of, int64_t* solves) {
+  if (!e) return SAV_E_ARG;
+  const int n = (int)e->members.size();
+  for (int i = 0; i < n; ++i) {
+    const int gi = e->group_of.empty() ? -1 : e->group_of[i];
+    if (group_of) group_of[i] = gi;
+    if (solves) {
+      long long ns = 0;
+      if (gi >= 0) e->groups[gi]->info(&ns, nullptr, nullptr);
+      solves[i] = ns;
+    }
+  }
   return SAV_OK;
 }
 
@@ -78,20 +154,39 @@ int sav_ensemble_run(sav_ensemble_t* e, const double* t_nodes, int n_nodes, int 
   std::atomic<int> next{0};
   int device = 0;
   (void)hipGetDevice(&device);
-  auto worker = [&]() {
-    (void)hipSetDevice(device);  // the current device is per host thread
-    for (int k = next.fetch_add(1); k < n; k = next.fetch_add(1)) {
-      const int i = order[k];
+  if (e->batched && !e->groups.empty()) {
+    // one host thread per member: the members of a size-class group meet at every step's
+    // psi solve, which the last to arrive runs for all of them; a member whose run ends
+    // (or fails) leaves its group, so the others never wait for it
+    for (auto& g : e->groups) g->begin_run();
+    auto run_one = [&](int i) {
+      (void)hipSetDevice(device);
       int32_t kit = 0;
       rcs[i] = sav_run(e->members[i], t_nodes, n_nodes, n0, energy_hist ? energy_hist + (long long)i * m : nullptr,
                        area_hist ? area_hist + (long long)i * m : nullptr, &kit);
       if (krylov_iters) krylov_iters[i] = kit;
-    }
-  };
-  std::vector<std::thread> pool;
-  for (int t = 1; t < n_threads; ++t) pool.emplace_back(worker);
-  worker();
-  for (auto& th : pool) th.join();
+      if (e->group_of[i] >= 0) e->groups[e->group_of[i]]->leave(e->slot_of[i]);
+    };
+    std::vector<std::thread> ts;
+    for (int k = 1; k < n; ++k) ts.emplace_back(run_one, order[k]);
+    run_one(order[0]);
+    for (auto& th : ts) th.join();
+  } else {
+    auto worker = [&]() {
+      (void)hipSetDevice(device);  // the current device is per host thread
+      for (int k = next.fetch_add(1); k < n; k = next.fetch_add(1)) {
+        const int i = order[k];
+        int32_t kit = 0;
+        rcs[i] = sav_run(e->members[i], t_nodes, n_nodes, n0, energy_hist ? energy_hist + (long long)i * m : nullptr,
+                         area_hist ? area_hist + (long long)i * m : nullptr, &kit);
+        if (krylov_iters) krylov_iters[i] = kit;
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < n_threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+  }
   int worst = SAV_OK;
   e->err.clear();
   for (int i = 0; i < n; ++i) {

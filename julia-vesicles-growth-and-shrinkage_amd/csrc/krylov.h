@@ -7,6 +7,12 @@
 
 namespace sav {
 
+// Systems one batched solve can cover: the three solves of a step for each of up to 64
+// members of a size-class group (sav_ensemble_set_batched).  System z of a batch belongs
+// to member z / 3: the kernels take its mobility at M + (z / 3) * nfull and its step scalars
+// at ds + z / 3 (a handle's own solves have z < 3).
+constexpr int KRY_MAX_SYS = 3 * 64;
+
 template <typename R>
 struct KrylovCtx {
   int Nx, Ny, N;
@@ -30,7 +36,7 @@ struct KrylovCtx {
   KScal* ks;                  // device, one per solve slot
   KScal* hks;                 // pinned host mirror
   hipStream_t st;
-  int guess[3];               // expected iteration count per slot (batching of convergence checks)
+  int guess[KRY_MAX_SYS];     // expected iteration count per slot (batching of convergence checks)
   // optional event timing of one matvec kernel (1: column inverse, 2: rows, 3: column forward):
   // every ev_stride-th launch of it carries start/stop events (ev_seq counts its launches)
   int timed = 0;
@@ -54,6 +60,7 @@ struct KrylovCtx {
   // cache resident); chosen after the column width at the first solve.  seq = (order == 2).
   int order = 0;
   int seq = 0;
+  int tune_nsys = 3;          // systems the matvec autotuner times (a step's batch; a group's)
   double tune_order_ms[2] = {0.0, 0.0};  // measured matvec time in order 1 and 2 (0: not timed)
   double tune_ms[3] = {0.0, 0.0, 0.0};  // measured matvec time with 8, 16 and 4 columns (0: not timed)
   // preconditioned directions of the flexible solver

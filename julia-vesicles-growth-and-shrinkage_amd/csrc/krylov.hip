@@ -15,6 +15,7 @@
 // host's convergence check.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include <hip/hip_ext.h>
 
@@ -165,12 +166,14 @@ __global__ __launch_bounds__((KRowCfg<R, L>::NT)) void k_mv_rows(cplx<R>* __rest
                                                                 const R* __restrict__ M, long long nrows_half,
                                                                 double scale, const cplx<R>* __restrict__ twi,
                                                                 const cplx<R>* __restrict__ twf,
-                                                                const KScal* __restrict__ ks, long long vs) {
+                                                                const KScal* __restrict__ ks, long long vs,
+                                                                long long ms) {
   const int z = blockIdx.y;
   ks += z;
   if (ks->done) return;
   Bx += z * vs;
   By += z * vs;
+  M += (z / 3) * ms;  // the mobility of the member the system belongs to
   using FI = typename KRowCfg<R, L>::FI;
   using FF = typename KRowCfg<R, L>::FF;
   constexpr int TT = FI::TT, E = FI::E, ROWS = KRowCfg<R, L>::ROWS, RS = KRowCfg<R, L>::RS;
@@ -235,6 +238,7 @@ __global__ __launch_bounds__((KColCfg<R, L, CW, true>::NT)) void k_mv_cols_fwd(
   if (Cv) Cv += z * vs;
   out += z * vs;
   partials += z * ps;
+  ds += z / 3;  // the step scalars of the member the system belongs to
   using F = KColFFT<R, L, false>;
   constexpr int C = KColCfg<R, L, CW, true>::C;
   constexpr int E = F::E;
@@ -340,6 +344,7 @@ __global__ __launch_bounds__(256) void k_kry_init(const cplx<R>* __restrict__ b,
   p += z * vs;
   x += z * vs;
   partials += z * ps;
+  ds += z / 3;  // the step scalars of the member the system belongs to
   double acc[3] = {0.0, 0.0, 0.0};
   const double a_dt = ds->a / ds->dt, Mmax = ds->Mmax;
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
@@ -638,6 +643,8 @@ struct MvArgs {
   cplx<R>* out;
   const KScal* ks;
   double* partials;
+  const R* M;              // mobility of the batch's first member (system z's: M + (z / 3) nfull)
+  const DScal* ds;         // its step scalars (system z's: ds + z / 3)
   int nsys;
 };
 
@@ -681,7 +688,7 @@ static int launch_cols_fwd(KrylovCtx<R>& k, const MvArgs<R>& a, int mode, hipEve
   int nb3 = 0;
 #define KL(L, M)                                                                                                 \
   hipExtLaunchKernelGGL((k_mv_cols_fwd<R, L, M, CC::C>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, a.y,   \
-                        a.partner, a.out, k.Nx, k.S4, k.kx0, k.ky, k.tw_cf, k.ds, a.ks, a.partials, vs, ps)
+                        a.partner, a.out, k.Nx, k.S4, k.kx0, k.ky, k.tw_cf, a.ds, a.ks, a.partials, vs, ps)
   switch (k.Ny) {
 #define X(L)                                            \
   case L: {                                             \
@@ -716,8 +723,8 @@ static void launch_rows(KrylovCtx<R>& k, const MvArgs<R>& a, hipEvent_t e0, hipE
   case L: {                                                                                                   \
     using CC = KRowCfg<R, L>;                                                                                 \
     dim3 grid((unsigned)((nrh + CC::ROWS - 1) / CC::ROWS), a.nsys);                                           \
-    hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, k.M, nrh,    \
-                          scale, k.tw_ri, k.tw_rf, a.ks, k.vs);                                               \
+    hipExtLaunchKernelGGL((k_mv_rows<R, L>), grid, dim3(CC::NT), 0, k.st, e0, e1, 0, a.Bx, a.By, a.M, nrh,    \
+                          scale, k.tw_ri, k.tw_rf, a.ks, k.vs, k.nfull);                                      \
     break;                                                                                                    \
   }
     SAV_KRY_SIZES(X)
@@ -780,6 +787,8 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
   a.out = out;
   a.ks = k.ks + slot;
   a.partials = k.partials;
+  a.M = k.M;
+  a.ds = k.ds;
   a.nsys = nsys;
   if (!k.seq || nsys == 1) return matvec_kernels(k, a, mode, upd, -1);
   int nb3 = 0;
@@ -794,6 +803,8 @@ static int matvec_launch(KrylovCtx<R>& k, const cplx<R>* y, cplx<R>* out, int mo
     b.out = shifted(a.out, d);
     b.ks = a.ks + z;
     b.partials = a.partials + z * k.ps;
+    b.M = k.M + (z / 3) * k.nfull;
+    b.ds = k.ds + z / 3;
     b.nsys = 1;
     nb3 = matvec_kernels(k, b, mode, upd, z);
   }
@@ -814,15 +825,15 @@ static int effective_columns(int Ny) {
 }
 
 // Milliseconds per matvec of three pairs of matvecs in the shape of a step's batched solve
-// (three systems, KScal slots 0-2, on the right-hand sides in k.b, output into k.t; in FP64
-// with the s and p updates formed on load, as an iteration runs them), after one warm-up.
-// The slots are reinitialised by the solve that follows.
+// (k.tune_nsys systems, KScal slots from 0, on the right-hand sides in k.b, output into k.t;
+// in FP64 with the s and p updates formed on load, as an iteration runs them), after one
+// warm-up.  The slots are reinitialised by the solve that follows.
 template <typename R>
 static double time_matvec_pairs(KrylovCtx<R>& k, hipEvent_t a, hipEvent_t b) {
-  constexpr int nsys = 3;
+  const int nsys = k.tune_nsys;
   constexpr int u1 = sizeof(R) == 8 ? 1 : 0, u2 = sizeof(R) == 8 ? 2 : 0;
-  const KScal zs[nsys] = {};
-  SAV_HIP(hipMemcpyAsync(k.ks, zs, sizeof(zs), hipMemcpyHostToDevice, k.st));
+  const std::vector<KScal> zs(nsys, KScal{});
+  SAV_HIP(hipMemcpyAsync(k.ks, zs.data(), nsys * sizeof(KScal), hipMemcpyHostToDevice, k.st));
   matvec_launch(k, k.b, k.t, 1, 0, k.b, nsys, u1);  // warm-up
   SAV_HIP(hipEventRecord(a, k.st));
   for (int r = 0; r < 3; ++r) {
@@ -994,14 +1005,14 @@ int bicgstab_batch(KrylovCtx<R>& k, int slot, int nsys) {
     if (all_done || queued >= k.itmax) break;
     if (!logging) batch = 2;
   }
-  int iters[4] = {0, 0, 0, 0};
+  std::vector<int> iters(nsys, 0);
   int maxit = 0;
   for (int z = 0; z < nsys; ++z) {
     iters[z] = k.hks[slot + z].iter;
     k.guess[slot + z] = iters[z] > 0 ? iters[z] : 1;
     maxit = std::max(maxit, iters[z]);
   }
-  if (k.timed) tev_collect(k, iters, nsys);
+  if (k.timed) tev_collect(k, iters.data(), nsys);
   return maxit;
 }
 

@@ -113,5 +113,34 @@ std::unique_ptr<ISolver> make_solver(const HostCfg& cfg);
 
 // Library-internal: the step fork of a handle's solver (see ISolver::set_step_fork).
 void set_step_fork(sav_handle_t* h, bool on);
+// Library-internal: the solver behind a handle.
+ISolver* handle_solver(sav_handle_t* h);
+
+// A size-class group of ensemble members (no reference counterpart).  Members with the same
+// 2D grid, vesicle count, storage precision (FP64 or FP32 fields; not mixed), S4 and
+// psi-solve settings (diagonal preconditioner, tol, atol, itmax) run the three psi solves of
+// a step as ONE batched BiCGSTAB over all their systems: every Krylov and matvec launch
+// covers the whole size class, with the system (3 per member) on a grid dimension.  Their
+// Krylov vectors, mobility fields and step scalars live in pools the group owns, one slice
+// per member; each system's iterates are those of its member's own solve (bit-identical).
+// Members step in their own host threads and meet at the solve: the last to arrive runs it.
+class ISolveGroup {
+ public:
+  virtual ~ISolveGroup() = default;
+  // Starts a run: every member takes part again.
+  virtual void begin_run() = 0;
+  // Member m's run has ended (or failed): the others no longer wait for it.
+  virtual void leave(int m) = 0;
+  // Couples (or decouples, for stepping members one at a time) the members' solves; the
+  // pools stay in use either way.
+  virtual void set_coupled(bool on) = 0;
+  virtual int size() const = 0;
+  // batched solves run so far, and the matvec tile width and order the group tuned
+  virtual void info(long long* solves, int* columns, int* order) const = 0;
+};
+// Builds the group of `members` (at most 64, compatible as above); throws
+// std::invalid_argument if they do not qualify.  The group must outlive the members' use of
+// their pools: destroy the members first.
+std::unique_ptr<ISolveGroup> make_solve_group(const std::vector<ISolver*>& members);
 
 }  // namespace sav

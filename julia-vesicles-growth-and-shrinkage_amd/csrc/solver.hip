@@ -10,8 +10,11 @@
 // swap and the back-transform of a new state is a single batched irfft.
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
+#include <stdexcept>
+#include <string>
 #include <type_traits>
 
 #include "krylov.h"
@@ -113,6 +116,9 @@ static void* noalloc_scratch_address(int device) {
 }
 
 template <typename R>
+class SolveGroup;
+
+template <typename R>
 class Solver final : public ISolver {
  public:
   using C = cplx<R>;
@@ -187,8 +193,13 @@ class Solver final : public ISolver {
   // inner relative tolerance: at 1024^2 (two vesicles, 100 steps) 1e-5 gave 175.8 SAV steps/s,
   // 1e-4 159.4, 3e-4 152.7, 1e-6 163.7 (fewer rounds against more inner iterations)
   static constexpr double MIXED_INNER_RTOL = 1e-5;
-  KScal* hks = nullptr;
+  KScal* hks = nullptr;       // this handle's Krylov scalars on the host (a group's slice once grouped)
+  KScal* hks_own = nullptr;   // its own pinned allocation
   DStateScal* hst = nullptr;  // pinned: scalars of the state a step produced
+  // size-class group (ISolveGroup): the psi solves of a step run as one batched solve with
+  // the other members' while `group` is set; the pools stay in use after decoupling
+  SolveGroup<R>* group = nullptr;
+  int group_m = -1;
   C* w_bphys_c = nullptr;
   float last_ms = 0.f;
 
@@ -200,6 +211,41 @@ class Solver final : public ISolver {
     SAV_HIP(hipMemsetAsync(p, 0, bytes, s));
     allocs.push_back(p);
     return reinterpret_cast<T*>(p);
+  }
+  void dfree(void* p) {
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it == allocs.end()) return;
+    allocs.erase(it);
+    SAV_HIP(hipFree(p));
+  }
+
+  // Moves this handle's psi-solve buffers into the slices of a size-class group's pools:
+  // Krylov vectors (vec: x, r, p, v, s, t, b, Bx, By of its three systems), mobility, step
+  // scalars (copied over) and Krylov scalars; its own Krylov vectors are freed.
+  void adopt_group(SolveGroup<R>* grp, int m, C* const* vec, R* Mp, DScal* dsp, KScal* ksp, KScal* hksp) {
+    SAV_HIP(hipStreamSynchronize(s));
+    drop_graphs();  // captured steps hold the previous addresses
+    SAV_HIP(hipMemcpyAsync(dsp, ds, sizeof(DScal), hipMemcpyDeviceToDevice, s));
+    SAV_HIP(hipStreamSynchronize(s));
+    ds = dsp;
+    kc.ds = dsp;
+    kc32.ds = dsp;
+    w_M = Mp;
+    kc.M = Mp;
+    C** own[9] = {&kc.x, &kc.r, &kc.p, &kc.v, &kc.s, &kc.t, &kc.b, &kc.Bx, &kc.By};
+    for (int i = 0; i < 9; ++i) {
+      dfree(*own[i]);
+      *own[i] = vec[i];
+    }
+    kc.ks = ksp;
+    kc.hks = hksp;
+    hks = hksp;
+    group = grp;
+    group_m = m;
+  }
+  void group_solve();
+  bool group_ready() const {
+    return nd == 2 && !mixed && !use_learned && !collecting && precond_kind == SAV_PRECOND_DIAG;
   }
 
   explicit Solver(const HostCfg& h) : hc(h), c(h.c) {
@@ -288,6 +334,7 @@ class Solver final : public ISolver {
     tmp = dalloc<C>((16LL * N + 16) * H);
     // Krylov workspace (2D)
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks), 4 * sizeof(KScal), hipHostMallocDefault));
+    hks_own = hks;
     SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hst), sizeof(DStateScal), hipHostMallocDefault));
     std::memset(hks, 0, 4 * sizeof(KScal));
     if (nd == 2) {
@@ -382,7 +429,7 @@ class Solver final : public ISolver {
     for (auto e : tev) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (hs) hipHostFree(hs);
-    if (hks) hipHostFree(hks);
+    if (hks_own) hipHostFree(hks_own);
     if (hks32) hipHostFree(hks32);
     if (hst) hipHostFree(hst);
     for (auto& gr : graphs) hipGraphExecDestroy(gr.exec);
@@ -734,6 +781,8 @@ class Solver final : public ISolver {
       refined_solve(0, 3);
     } else if (capture_iters > 0) {
       bicgstab_queue<R>(kc, 0, 3, capture_iters);
+    } else if (group) {
+      group_solve();  // one batched solve with the group's other members
     } else {
       bicgstab_batch<R>(kc, 0, 3);
     }
@@ -2273,7 +2322,7 @@ class Solver final : public ISolver {
 
   bool graph_usable() const {
     // a mixed-precision step decides its rounds on the host: eager only
-    return graph_on && kc.timed == 0 &&
+    return graph_on && kc.timed == 0 && group == nullptr &&
            (nd == 3 || (!mixed && !use_learned && !collecting && kc.colw != 0 && kc.order != 0));
   }
   void drop_graphs() {
@@ -2594,6 +2643,198 @@ class Solver final : public ISolver {
     return 0;
   }
 };
+
+// ---------------------------------------------------------------------------
+// Size-class group of ensemble members (see ISolveGroup in solver.h).
+// ---------------------------------------------------------------------------
+template <typename R>
+class SolveGroup final : public ISolveGroup {
+ public:
+  using C = cplx<R>;
+  std::vector<Solver<R>*> mem;
+  int K = 0;
+  KrylovCtx<R> gk{};  // one batched solve over 3 K systems, system 3 m + z = member m's solve z
+  hipStream_t st = nullptr;
+  std::vector<hipEvent_t> ready;  // member m's right-hand sides are queued
+  hipEvent_t solved = nullptr;
+  std::vector<void*> allocs;
+  KScal* hks = nullptr;  // pinned, 3 K + 1
+  // rendezvous of the members' host threads
+  std::mutex mu;
+  std::condition_variable cv;
+  int parties = 0, arrived = 0;
+  long long gen = 0, solves = 0;
+  std::vector<char> active;
+  std::vector<int> left;  // members that left since the last solve (their systems are cleared)
+  bool coupled = true;
+  std::string fail;
+
+  template <class T>
+  T* alloc(size_t n) {
+    void* q = nullptr;
+    SAV_HIP(hipMalloc(&q, n * sizeof(T) + 256));
+    SAV_HIP(hipMemsetAsync(q, 0, n * sizeof(T) + 256, st));
+    allocs.push_back(q);
+    return reinterpret_cast<T*>(q);
+  }
+
+  explicit SolveGroup(const std::vector<Solver<R>*>& members) : mem(members), K((int)members.size()) {
+    const Solver<R>& m0 = *mem[0];
+    SAV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const long long nf = m0.kc.nfull;
+    gk = m0.kc;  // tables, sizes, tolerances
+    gk.vs = nf;
+    gk.ps = m0.kc.ps;
+    C** vec[9] = {&gk.x, &gk.r, &gk.p, &gk.v, &gk.s, &gk.t, &gk.b, &gk.Bx, &gk.By};
+    for (auto v : vec) *v = alloc<C>(3LL * K * nf);
+    R* Mp = alloc<R>((long long)K * nf);
+    DScal* dsp = alloc<DScal>(K);
+    gk.M = Mp;
+    gk.ds = dsp;
+    gk.partials = alloc<double>(3LL * K * gk.ps);
+    gk.ks = alloc<KScal>(3 * K + 1);
+    SAV_HIP(hipHostMalloc(reinterpret_cast<void**>(&hks), (3 * K + 1) * sizeof(KScal), hipHostMallocDefault));
+    std::memset(hks, 0, (3 * K + 1) * sizeof(KScal));
+    gk.hks = hks;
+    gk.st = st;
+    gk.colw = 0;
+    gk.order = 0;
+    gk.seq = 0;
+    gk.tune_nsys = 3 * K;
+    gk.timed = 0;
+    gk.ev = nullptr;
+    gk.ev_cap = gk.ev_used = 0;
+    gk.snap = nullptr;
+    gk.outer = nullptr;
+    for (int z = 0; z < KRY_MAX_SYS; ++z) gk.guess[z] = 8;
+    SAV_HIP(hipStreamSynchronize(st));
+    ready.resize(K);
+    for (auto& e : ready) SAV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    SAV_HIP(hipEventCreateWithFlags(&solved, hipEventDisableTiming));
+    for (int m = 0; m < K; ++m) {
+      C* v[9];
+      for (int i = 0; i < 9; ++i) v[i] = *vec[i] + 3LL * m * nf;
+      mem[m]->adopt_group(this, m, v, Mp + (long long)m * nf, dsp + m, gk.ks + 3 * m, hks + 3 * m);
+    }
+    active.assign(K, 1);
+    parties = K;
+  }
+
+  ~SolveGroup() override {
+    if (st) hipStreamSynchronize(st);
+    for (auto e : ready) hipEventDestroy(e);
+    if (solved) hipEventDestroy(solved);
+    for (void* q : allocs) hipFree(q);
+    if (hks) hipHostFree(hks);
+    if (st) hipStreamDestroy(st);
+  }
+
+  void begin_run() override {
+    std::lock_guard<std::mutex> lk(mu);
+    active.assign(K, 1);
+    left.clear();
+    parties = K;
+    arrived = 0;
+    fail.clear();
+  }
+
+  void set_coupled(bool on) override {
+    coupled = on;
+    for (int m = 0; m < K; ++m) mem[m]->group = on ? this : nullptr;
+  }
+
+  int size() const override { return K; }
+  void info(long long* n, int* columns, int* order) const override {
+    if (n) *n = solves;
+    if (columns) *columns = gk.colw;
+    if (order) *order = gk.order;
+  }
+
+  // The batched solve of every member still taking part (lock not held; the others wait).
+  void run_solve() {
+    for (int m : left) {
+      // a member that left: zero right-hand sides, its systems finish at once
+      SAV_HIP(hipMemsetAsync(gk.b + 3LL * m * gk.vs, 0, 3 * gk.vs * sizeof(C), st));
+    }
+    left.clear();
+    for (int m = 0; m < K; ++m)
+      if (active[m]) SAV_HIP(hipStreamWaitEvent(st, ready[m], 0));
+    bicgstab_batch<R>(gk, 0, 3 * K);
+    SAV_HIP(hipEventRecord(solved, st));
+    ++solves;
+  }
+
+  // With `lk` held and every member still taking part arrived: run the solve, wake them.
+  void release(std::unique_lock<std::mutex>& lk) {
+    arrived = 0;
+    lk.unlock();
+    try {
+      run_solve();
+    } catch (const std::exception& e) {
+      fail = e.what();
+    }
+    lk.lock();
+    ++gen;
+    cv.notify_all();
+  }
+
+  // Member m's right-hand sides are queued on its stream: wait for the group's solve, after
+  // which its stream waits for the solution (its Krylov scalars are then in hks).
+  void solve_point(int m) {
+    Solver<R>* me = mem[m];
+    SAV_HIP(hipEventRecord(ready[m], me->s));
+    std::unique_lock<std::mutex> lk(mu);
+    const long long g0 = gen;
+    if (++arrived == parties)
+      release(lk);
+    else
+      cv.wait(lk, [&] { return gen != g0; });
+    if (!fail.empty()) throw HipError("size-class group solve: " + fail, SAV_E_HIP);
+    lk.unlock();
+    SAV_HIP(hipStreamWaitEvent(me->s, solved, 0));
+  }
+
+  void leave(int m) override {
+    std::unique_lock<std::mutex> lk(mu);
+    if (m < 0 || m >= K || !active[m]) return;
+    active[m] = 0;
+    left.push_back(m);
+    --parties;
+    if (parties > 0 && arrived == parties) release(lk);
+  }
+};
+
+template <typename R>
+void Solver<R>::group_solve() {
+  group->solve_point(group_m);
+}
+
+template <typename R>
+static std::unique_ptr<ISolveGroup> make_group_t(const std::vector<ISolver*>& members) {
+  std::vector<Solver<R>*> ms;
+  for (ISolver* i : members) {
+    auto* q = dynamic_cast<Solver<R>*>(i);
+    if (!q) throw std::invalid_argument("size-class group: members of one storage precision");
+    ms.push_back(q);
+  }
+  const Solver<R>& a = *ms[0];
+  for (Solver<R>* q : ms) {
+    if (!q->group_ready() || q->group)
+      throw std::invalid_argument("size-class group: 2D, not mixed, diagonal preconditioner, not grouped yet");
+    const sav_config_t &x = a.c, &y = q->c;
+    if (y.Nx != x.Nx || y.Ny != x.Ny || y.N != x.N || y.S4 != x.S4 || y.tol != x.tol || y.atol != x.atol ||
+        y.itmax != x.itmax || y.device != x.device)
+      throw std::invalid_argument("size-class group: members differ in grid, N, S4 or the psi-solve settings");
+  }
+  return std::unique_ptr<ISolveGroup>(new SolveGroup<R>(ms));
+}
+
+std::unique_ptr<ISolveGroup> make_solve_group(const std::vector<ISolver*>& members) {
+  if (members.size() < 2 || members.size() * 3 > (size_t)KRY_MAX_SYS)
+    throw std::invalid_argument("size-class group: 2 to 64 members");
+  if (dynamic_cast<Solver<double>*>(members[0])) return make_group_t<double>(members);
+  return make_group_t<float>(members);
+}
 
 // Global maximum of a field into a device double.
 template <typename R>

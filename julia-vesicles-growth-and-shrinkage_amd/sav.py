@@ -81,7 +81,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_set_column_tile", "sav_autotune_info", "sav_column_tile_info", "sav_set_matvec_order", "sav_matvec_order_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_transform_tile_fwd_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
            "sav_precond_samples", "sav_precond_train", "sav_precond_train_target", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
-           "sav_ensemble_member", "sav_ensemble_run")
+           "sav_ensemble_member", "sav_ensemble_run", "sav_ensemble_set_batched", "sav_ensemble_batched_info")
 
 _lib_handle = None
 
@@ -158,6 +158,8 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_ensemble_member.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_ensemble_member.restype = ct.c_void_p
     lib.sav_ensemble_run.argtypes = [ct.c_void_p, _dp, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _fp, _fp]
+    lib.sav_ensemble_set_batched.argtypes = [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int32)]
+    lib.sav_ensemble_batched_info.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p]
     if path is None:
         _lib_handle = lib
     return lib
@@ -584,6 +586,23 @@ class Ensemble:
         """Graph replay of steps on every member (Operators.set_graph)."""
         for m in self.members:
             m.set_graph(on)
+
+    def set_batched(self, on: bool) -> int:
+        """Size classes: members with the same grid, N and solve settings run their psi solves
+        as one batched solve per step (sav_ensemble_set_batched); returns the group count."""
+        ng = ct.c_int32(0)
+        rc = self.lib.sav_ensemble_set_batched(self.e, int(bool(on)), ct.byref(ng))
+        if rc != 0:
+            raise SavError(rc, "ensemble_set_batched: " + self.lib.sav_ensemble_last_error(self.e).decode())
+        return ng.value
+
+    def batched_info(self):
+        """(group of each member (-1: none), batched solves run by each member's group)."""
+        n = len(self.confs)
+        g = np.zeros(n, dtype=np.int32)
+        ns = np.zeros(n, dtype=np.int64)
+        self.lib.sav_ensemble_batched_info(self.e, g.ctypes.data, ns.ctypes.data)
+        return g, ns
 
     def run(self, t_nodes: np.ndarray, n0: int = 1, threads: int = 8):
         t = np.ascontiguousarray(t_nodes, dtype=np.float64)
