@@ -333,8 +333,9 @@ int sav_ensemble_run(sav_ensemble_t* e, const double* t_nodes, int n_nodes, int 
  * covers the size class, the system on a grid dimension).  Each member's iterates, iteration
  * counts and states stay bit-identical to its own solves.  sav_ensemble_run then steps every
  * member in a host thread of its own (n_threads is not used); members meet at each step's
- * solve, which the last to arrive runs.  The groups' pools replace the members' own psi-solve
- * buffers and stay in use with on = 0 (members then solve on their own again).  n_groups
+ * solve, which the last to arrive runs.  Outside sav_ensemble_run, and with on = 0, members
+ * solve on their own (a member may be stepped alone through its handle).  The groups' pools
+ * replace the members' own psi-solve buffers and stay in use.  n_groups
  * returns the number of groups.  sav_ensemble_batched_info: each member's group (-1: none)
  * and the number of batched solves its group ran (either array may be NULL). */
 int sav_ensemble_set_batched(sav_ensemble_t* e, int on, int32_t* n_groups);

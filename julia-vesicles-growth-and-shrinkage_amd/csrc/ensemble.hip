@@ -106,8 +106,7 @@ int sav_ensemble_set_batched(sav_ensemble_t* e, int on, int32_t* n_groups) {
         }
       }
     }
-    for (auto& g : e->groups) g->set_coupled(on != 0);
-    e->batched = on != 0;
+    e->batched = on != 0;  // the groups' solves are coupled during sav_ensemble_run only
     if (n_groups) *n_groups = (int32_t)e->groups.size();
     return SAV_OK;
   } catch (const std::exception& ex) {
@@ -158,7 +157,10 @@ int sav_ensemble_run(sav_ensemble_t* e, const double* t_nodes, int n_nodes, int 
     // one host thread per member: the members of a size-class group meet at every step's
     // psi solve, which the last to arrive runs for all of them; a member whose run ends
     // (or fails) leaves its group, so the others never wait for it
-    for (auto& g : e->groups) g->begin_run();
+    for (auto& g : e->groups) {
+      g->begin_run();
+      g->set_coupled(true);
+    }
     auto run_one = [&](int i) {
       (void)hipSetDevice(device);
       int32_t kit = 0;
@@ -171,6 +173,7 @@ int sav_ensemble_run(sav_ensemble_t* e, const double* t_nodes, int n_nodes, int 
     for (int k = 1; k < n; ++k) ts.emplace_back(run_one, order[k]);
     run_one(order[0]);
     for (auto& th : ts) th.join();
+    for (auto& g : e->groups) g->set_coupled(false);  // members may be stepped alone again
   } else {
     auto worker = [&]() {
       (void)hipSetDevice(device);  // the current device is per host thread

@@ -131,8 +131,8 @@ class ISolveGroup {
   virtual void begin_run() = 0;
   // Member m's run has ended (or failed): the others no longer wait for it.
   virtual void leave(int m) = 0;
-  // Couples (or decouples, for stepping members one at a time) the members' solves; the
-  // pools stay in use either way.
+  // Couples the members' solves (only while all members are stepped together, or a member
+  // stepped alone would wait for the others) or decouples them; the pools stay in use.
   virtual void set_coupled(bool on) = 0;
   virtual int size() const = 0;
   // batched solves run so far, and the matvec tile width and order the group tuned

@@ -2680,6 +2680,7 @@ class SolveGroup final : public ISolveGroup {
 
   explicit SolveGroup(const std::vector<Solver<R>*>& members) : mem(members), K((int)members.size()) {
     const Solver<R>& m0 = *mem[0];
+    SAV_HIP(hipSetDevice(m0.c.device));
     SAV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     const long long nf = m0.kc.nfull;
     gk = m0.kc;  // tables, sizes, tolerances
@@ -2718,6 +2719,7 @@ class SolveGroup final : public ISolveGroup {
     }
     active.assign(K, 1);
     parties = K;
+    set_coupled(false);  // coupled only while sav_ensemble_run steps all members together
   }
 
   ~SolveGroup() override {

@@ -67,6 +67,13 @@ def test_batched_size_classes_match_single_runs(jv, precision):
     groups, solves = ens.batched_info()
     assert np.all(solves[groups >= 0] == 3)
     mid = [m.get_state(jv.SAV_PRESENT) for m in ens.members]
+    # between batched runs a member is stepped through its own handle (solves alone)
+    probe = ens.members[int(np.flatnonzero(groups >= 0)[0])]
+    saved = (probe.get_state(jv.SAV_PRESENT), probe.get_state(jv.SAV_OLD))
+    rcp, _ = probe.step(confs[0].dt, 4)
+    assert rcp >= 0
+    probe.set_state(jv.SAV_PRESENT, saved[0])
+    probe.set_state(jv.SAV_OLD, saved[1])
     ens.set_batched(False)
     rc2, E2, A2, kit2, st2 = ens.run(t[3:], 4, threads=4)
     assert rc2 >= 0
