@@ -173,8 +173,8 @@ class Solver final : public ISolver {
   double* partials = nullptr;
   double* red = nullptr;  // reduction outputs (512 doubles)
   // work
-  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_s2, *w_du, *w_dmn, *w_conv,
-      *w_coupl, *w_rhs2, *w_ut12, *w_s5, *w_mon, *w_gxy;
+  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_du, *w_dmn, *w_conv,
+      *w_coupl, *w_rhs2, *w_ut12;
   C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp;
   // scratch of the forked branch (made by ensure_fork)
   R* w_grad = nullptr;
@@ -311,16 +311,12 @@ class Solver final : public ISolver {
     w_H = dalloc<R>(4 * NP);
     w_w = dalloc<R>(3 * NP);
     w_adv = dalloc<R>(2 * NP);
-    w_s2 = dalloc<R>(8 * NP);
     w_du = dalloc<R>(9 * P);
     w_dmn = dalloc<R>(6 * NP);
     w_conv = dalloc<R>(3 * P);
     w_coupl = dalloc<R>(3 * P);
     w_rhs2 = dalloc<R>(3 * P);
     w_ut12 = dalloc<R>(6 * P);
-    w_s5 = dalloc<R>(4 * NP);
-    w_mon = dalloc<R>(6 * NP + 2 * P);
-    w_gxy = dalloc<R>(2 * NP);
     w_phat = dalloc<C>(NH);
     w_omh = dalloc<C>(NH);
     w_Hh = dalloc<C>(4 * NH);
@@ -1566,7 +1562,43 @@ class Solver final : public ISolver {
     });
     // grad G -> M * grad G -> rfft -> rhs12 = sum_d D1_d * (.)   (Solvers.jl:99-105)
     const C* Gh = w_Hh + NH;
-    {
+    if (nd == 3) {
+      // constant mobility M0_psi (Solvers_3d.jl:48-57): rfft(M0 irfft(X)) of X = D1_d G_hat is
+      // M0 times the part of X that a c2r keeps, the Hermitian part (X(k) + conj X(-k)) / 2 on
+      // the kx = 0 and kx = Nx/2 planes and X itself elsewhere, so the 2 nd N-field transforms
+      // of the reference reduce to one spectral pass (equal up to the transforms' round-off)
+      C* rhs12 = w_rhs + NH;
+      const double m0 = c.M0_psi;
+      const int Nx_ = Nx, Nxh_ = Nxh, Ny_ = Ny, Nz_ = Nz;
+      foreach_n(s, NH, [=] __device__(long long i) {
+        int ix, iy, iz, f;
+        ki(i, ix, iy, iz, f);
+        const double kk[3] = {sc.kx[ix], sc.ky[iy], sc.kz[iz]};
+        const CD gk = Gh[i];
+        const bool plane = ix == 0 || 2 * ix == Nx_;
+        CD gm = CD{0.0, 0.0};
+        double km[3] = {0.0, 0.0, 0.0};
+        if (plane) {
+          const int iy2 = iy ? Ny_ - iy : 0, iz2 = iz ? Nz_ - iz : 0;
+          gm = Gh[(((long long)f * Nz_ + iz2) * Ny_ + iy2) * Nxh_ + ix];
+          km[0] = sc.kx[ix];
+          km[1] = sc.ky[iy2];
+          km[2] = sc.kz[iz2];
+        }
+        CD acc = CD{0.0, 0.0};
+        for (int d = 0; d < 3; ++d) {
+          CD x = CD{-kk[d] * gk.y, kk[d] * gk.x};  // X(k) = i k_d G(k)
+          if (plane) {
+            const CD xm = CD{-km[d] * gm.y, km[d] * gm.x};  // X(-k)
+            x = CD{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)};
+          }
+          x = CD{m0 * x.x, m0 * x.y};
+          const CD t = CD{-kk[d] * x.y, kk[d] * x.x};
+          acc = d == 0 ? t : acc + t;
+        }
+        rhs12[i] = acc;
+      });
+    } else {
       std::vector<SpecJob> jl;
       for (int d = 0; d < nd; ++d) jl.push_back(job(Gh, w_d4 + d * NP, N, DX(d)));
       irfft_v(jl);
@@ -1659,56 +1691,68 @@ class Solver final : public ISolver {
   // Step 2 (Solvers.jl:144-234): 22 inner products, two 3x3 LU solves
   // ==========================================================================
   void step2() {
-    const long long NP = (long long)N * P;
-    // physical step-1 fields in the order phi12 phi13 phi14 psi12 phi11 psi11 phi21 psi21
-    const int order[8] = {PHI12, PHI13, PHI14, PSI12, PHI11, PSI11, PHI21, PSI21};
-    std::vector<SpecJob> jl;
-    for (int k = 0; k < 8; ++k) jl.push_back(job(cf(order[k]), w_s2 + k * NP, N));
-    irfft_v(jl);
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    // <H_i, b phi^n + c phi^{n-1}> and <G, b psi^n + c psi^{n-1}> over the physical fields
     const R *phN = fphys(ip, F_PHI), *phO = fphys(io, F_PHI), *psN = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
     const R *H1 = w_H, *G = w_H + NP, *H2 = w_H + 2 * NP, *H3 = w_H + 3 * NP;
-    const R* f = w_s2;
     const DScal* dsl = ds;
-    reduce_n<22>(s, NP, 1,
-                 [=] __device__(long long i, int, double(&acc)[22]) {
-                   const T b = (T)dsl->b, cc = (T)dsl->c;
-                   const double tph = (double)(b * phN[i] + cc * phO[i]);
-                   const double tps = (double)(b * psN[i] + cc * psO[i]);
-                   const double h1 = H1[i], h2 = H2[i], h3 = H3[i], gg = G[i];
-                   acc[0] += h1 * tph;
-                   acc[1] += h2 * tph;
-                   acc[2] += h3 * tph;
-                   acc[3] += gg * tps;
-                   const double p12 = f[i], p13 = f[NP + i], p14 = f[2 * NP + i], s12 = f[3 * NP + i];
-                   const double p11 = f[4 * NP + i], s11 = f[5 * NP + i], p21 = f[6 * NP + i], s21 = f[7 * NP + i];
-                   acc[4] += h1 * p12;
-                   acc[5] += h1 * p13;
-                   acc[6] += h1 * p14;
-                   acc[7] += h2 * p12;
-                   acc[8] += h2 * p13;
-                   acc[9] += h2 * p14;
-                   acc[10] += h3 * p12;
-                   acc[11] += h3 * p13;
-                   acc[12] += h3 * p14;
-                   acc[13] += gg * s12;
-                   acc[14] += h1 * p11;
-                   acc[15] += h2 * p11;
-                   acc[16] += h3 * p11;
-                   acc[17] += gg * s11;
-                   acc[18] += h1 * p21;
-                   acc[19] += h2 * p21;
-                   acc[20] += h3 * p21;
-                   acc[21] += gg * s21;
+    reduce_n<4>(s, NP, 1,
+                [=] __device__(long long i, int, double(&acc)[4]) {
+                  const T b = (T)dsl->b, cc = (T)dsl->c;
+                  const double tph = (double)(b * phN[i] + cc * phO[i]);
+                  const double tps = (double)(b * psN[i] + cc * psO[i]);
+                  acc[0] += (double)H1[i] * tph;
+                  acc[1] += (double)H2[i] * tph;
+                  acc[2] += (double)H3[i] * tph;
+                  acc[3] += (double)G[i] * tps;
+                },
+                partials, red);
+    // the 18 products with step 1's split solutions, by Parseval over the half spectra (the
+    // reference transforms the 8 split solutions back and sums in physical space): H_hat and
+    // G_hat are rffts of real fields, so sum_x h f = (1/P) sum_k w(kx) Re(conj(h_hat) f_hat)
+    // with w = 1 on the kx = 0 and kx = Nx/2 planes and 2 elsewhere, for any f_hat whose c2r is f
+    const C* Hh = w_Hh;  // [H1 | G | H2 | H3]
+    const C* cch = cache;
+    const int Nxh_ = Nxh, Nx_ = Nx;
+    reduce_n<18>(s, NH, 1,
+                 [=] __device__(long long i, int, double(&acc)[18]) {
+                   const int ix = (int)(i % Nxh_);
+                   const double w = (ix == 0 || 2 * ix == Nx_) ? 1.0 : 2.0;
+                   const CD h1 = Hh[i], gg = Hh[NH + i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
+                   auto re = [&](const CD& a, int slot) {
+                     const CD f = cch[(long long)slot * NH + i];
+                     return w * ((double)a.x * (double)f.x + (double)a.y * (double)f.y);
+                   };
+                   acc[0] += re(h1, PHI12);
+                   acc[1] += re(h1, PHI13);
+                   acc[2] += re(h1, PHI14);
+                   acc[3] += re(h2, PHI12);
+                   acc[4] += re(h2, PHI13);
+                   acc[5] += re(h2, PHI14);
+                   acc[6] += re(h3, PHI12);
+                   acc[7] += re(h3, PHI13);
+                   acc[8] += re(h3, PHI14);
+                   acc[9] += re(gg, PSI12);
+                   acc[10] += re(h1, PHI11);
+                   acc[11] += re(h2, PHI11);
+                   acc[12] += re(h3, PHI11);
+                   acc[13] += re(gg, PSI11);
+                   acc[14] += re(h1, PHI21);
+                   acc[15] += re(h2, PHI21);
+                   acc[16] += re(h3, PHI21);
+                   acc[17] += re(gg, PSI21);
                  },
-                 partials, red);
+                 partials, red + 4);
     const PhysC q = pc();
     const int ndl = nd;
     const double* rd = red;
     DScal* dsw = ds;
     const DStateScal* stN = dstate + ip;
     const DStateScal* stO = dstate + io;
+    const double invP = 1.0 / (double)P;
     single(s, [=] __device__() {
-      auto idot = [&](int k) { return integ(rd[k], q, ndl); };
+      // red[0..3] physical sums, red[4..21] spectral sums (x 1/P)
+      auto idot = [&](int k) { return integ(k < 4 ? rd[k] : rd[k] * invP, q, ndl); };
       const double a = dsw->a, b = dsw->b, cc = dsw->c;
       const double U1 = -(b * stN->R1 + cc * stO->R1) / a + idot(0) / (2.0 * a);
       const double U2 = -(b * stN->R2 + cc * stO->R2) / a + idot(1) / (2.0 * a);
@@ -1884,31 +1928,41 @@ class Solver final : public ISolver {
   // Step 5 (Solvers.jl:344-423; Solvers_3d.jl:299-392)
   // ==========================================================================
   void step5() {
-    const long long NP = (long long)N * P, NH = (long long)N * H;
-    irfft({job(s3 + 1 * NH, w_s5, N), job(s3 + 3 * NH, w_s5 + NP, N), job(s3 + 5 * NH, w_s5 + 2 * NP, N),
-           job(s3 + 7 * NH, w_s5 + 3 * NP, N)});
-    const R *adv = w_adv, *f5 = w_s5, *conv = w_conv, *coupl = w_coupl, *ut = w_ut12;
+    const long long NH = (long long)N * H;
+    // sum (u* . grad phi*) mu_j and sum (u* . grad psi*) nu_j by Parseval over the half spectra
+    // (the reference transforms mu_j, nu_j back): w_advh holds the rffts of the two advection
+    // fields (step 1), s3 the spectra mu_1, nu_1, mu_2, nu_2 in slots 1, 3, 5, 7
+    const C* ah = w_advh;
+    const C* o = s3;
+    const int Nxh_ = Nxh, Nx_ = Nx;
+    reduce_n<4>(s, NH, 1,
+                [=] __device__(long long i, int, double(&acc)[4]) {
+                  const int ix = (int)(i % Nxh_);
+                  const double w = (ix == 0 || 2 * ix == Nx_) ? 1.0 : 2.0;
+                  const CD ap = ah[i], as = ah[NH + i];
+                  const CD m1 = o[NH + i], n1 = o[3 * NH + i], m2 = o[5 * NH + i], n2 = o[7 * NH + i];
+                  acc[0] += w * ((double)ap.x * (double)m1.x + (double)ap.y * (double)m1.y);  // t11
+                  acc[1] += w * ((double)as.x * (double)n1.x + (double)as.y * (double)n1.y);  // t12
+                  acc[2] += w * ((double)ap.x * (double)m2.x + (double)ap.y * (double)m2.y);  // t21
+                  acc[3] += w * ((double)as.x * (double)n2.x + (double)as.y * (double)n2.y);  // t22
+                },
+                partials, red);
+    const R *conv = w_conv, *coupl = w_coupl, *ut = w_ut12;
     const long long Pl = P;
     const int ndd = nd;
-    reduce_n<16>(s, NP, 1,
-                 [=] __device__(long long i, int, double(&acc)[16]) {
-                   const double ap = adv[i], as = adv[NP + i];
-                   acc[0] += ap * (double)f5[i];           // t11
-                   acc[1] += as * (double)f5[NP + i];      // t12
-                   acc[2] += ap * (double)f5[2 * NP + i];  // t21
-                   acc[3] += as * (double)f5[3 * NP + i];  // t22
-                   if (i < Pl) {
-                     for (int d = 0; d < ndd; ++d) {
-                       const double u1 = ut[d * Pl + i], u2 = ut[(ndd + d) * Pl + i];
-                       const double cp = coupl[d * Pl + i], cv = conv[d * Pl + i];
-                       acc[4 + d] += cp * u1;   // t13 parts
-                       acc[7 + d] += cv * u1;   // t14 parts
-                       acc[10 + d] += cp * u2;  // t23 parts
-                       acc[13 + d] += cv * u2;  // t24 parts
-                     }
+    reduce_n<12>(s, P, 1,
+                 [=] __device__(long long i, int, double(&acc)[12]) {
+                   for (int d = 0; d < ndd; ++d) {
+                     const double u1 = ut[d * Pl + i], u2 = ut[(ndd + d) * Pl + i];
+                     const double cp = coupl[d * Pl + i], cv = conv[d * Pl + i];
+                     acc[d] += cp * u1;      // t13 parts
+                     acc[3 + d] += cv * u1;  // t14 parts
+                     acc[6 + d] += cp * u2;  // t23 parts
+                     acc[9 + d] += cv * u2;  // t24 parts
                    }
                  },
-                 partials, red);
+                 partials, red + 4);
+    const double invP = 1.0 / (double)P;
     const PhysC q = pc();
     const double* rd = red;
     DScal* dsw = ds;
@@ -1925,8 +1979,9 @@ class Solver final : public ISolver {
       }
       double dvol = q.dx * q.dy;
       if (ndd == 3) dvol = dvol * q.dz;
-      const double th1 = dvol * (lam * (rd[0] + rd[1] + t13) + t14);
-      const double th2 = dvol * (lam * (rd[2] + rd[3] + t23) + t24);
+      // red[0..3] are spectral sums (x 1/P), red[4..15] physical
+      const double th1 = dvol * (lam * (rd[0] * invP + rd[1] * invP + t13) + t14);
+      const double th2 = dvol * (lam * (rd[2] * invP + rd[3] * invP + t23) + t24);
       const double a = dsw->a, b = dsw->b, cc = dsw->c, dt = dsw->dt5;
       const double rhsQ = -(b * stN->Q + cc * stO->Q) / dt + th1;
       const double lhsQ = (a / dt) - th2;
@@ -2009,98 +2064,127 @@ class Solver final : public ISolver {
   // (simulation.jl:147) for the present/old pair.
   // ==========================================================================
   void monitor() {
-    const long long NP = (long long)N * P;
     const C *phh = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *ph = phalf(ip);
-    R* m = w_mon;
     const bool s2 = c.S2 != 0.0;
-    std::vector<SpecJob> jl = {job(phh, m, N, MUL_LAP), job(phh, m + NP, N, MUL_LAP, phhO),
-                               job(ph, m + 6 * NP, 1, MUL_DX), job(ph, m + 6 * NP + P, 1, MUL_DY)};
-    if (s2) {
-      jl.push_back(job(phh, m + 2 * NP, N, MUL_DX));
-      jl.push_back(job(phh, m + 3 * NP, N, MUL_DY));
-      jl.push_back(job(phh, m + 4 * NP, N, MUL_DX, phhO));
-      jl.push_back(job(phh, m + 5 * NP, N, MUL_DY, phhO));
-    }
     // area: f_surf of rfft(phi) (calculate_area)
     const R* phi = fphys(ip, F_PHI);
     rfft({job(phi, w_phat, N)});
-    jl.push_back(job(w_phat, w_gxy, N, MUL_DX));
-    jl.push_back(job(w_phat, w_gxy + NP, N, MUL_DY));
-    irfft_v(jl);
     const PhysC q = pc();
-    const R* gxy = w_gxy;
-    const long long Pl = P;
-    double* redA = red + 256;
-    reduce_n<1>(s, P, N,
-                [=] __device__(long long i, int n, double(&acc)[1]) {
+    const R *u = uphys(ip), *uo = uphys(io), *phO = fphys(io, F_PHI), *ps = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
+    const long long Pl = P, Hl = H;
+    const int ndl = nd;
+    // physical sums, group n = vesicle n: the g(phi) part of f_surf, phi^2, (2 phi - phi_old)^2,
+    // psi^2, (2 psi - psi_old)^2; group 0 also |u|^2 and |2u - u_old|^2 -> red[n*7 + k]
+    double* redP = red;
+    reduce_n<7>(s, P, N,
+                [=] __device__(long long i, int n, double(&acc)[7]) {
                   const long long k = (long long)n * Pl + i;
-                  const T p = phi[k], gx = gxy[k], gy = gxy[NP + k];
+                  const T p = phi[k];
                   const T eps = (T)q.eps;
                   const T gg = T(0.25) * (p * p - T(1)) * (p * p - T(1));
-                  const T fs = (T)(3.0 * SQ2 / 4.0) * (gg / eps) + eps * (gx * gx + gy * gy) / T(2);
-                  acc[0] += (double)fs;
+                  acc[0] += (double)((T)(3.0 * SQ2 / 4.0) * (gg / eps));
+                  const double pd = p, pst = (double)(T(2) * phi[k] - phO[k]);
+                  const double y = ps[k], yst = (double)(T(2) * ps[k] - psO[k]);
+                  acc[1] += pd * pd;
+                  acc[2] += pst * pst;
+                  acc[3] += y * y;
+                  acc[4] += yst * yst;
+                  if (n == 0)
+                    for (int d = 0; d < ndl; ++d) {
+                      const double a = u[d * Pl + i];
+                      const double st = (double)(T(2) * u[d * Pl + i] - uo[d * Pl + i]);
+                      acc[5] += a * a;
+                      acc[6] += st * st;
+                    }
                 },
-                partials, redA);
-    const R *u = uphys(ip), *uo = uphys(io), *phO = fphys(io, F_PHI), *ps = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
-    const long long nu = (long long)nd * P;
-    reduce_n<14>(s, NP > nu ? NP : nu, 1,
-                 [=] __device__(long long i, int, double(&acc)[14]) {
-                   if (i < nu) {
-                     const double a = u[i], b = uo[i];
-                     const double st = (double)(T(2) * u[i] - uo[i]);
-                     acc[0] += a * a;
-                     acc[1] += st * st;
-                     (void)b;
-                   }
-                   if (i < Pl) {
-                     const double gx = m[6 * NP + i], gy = m[6 * NP + Pl + i];
-                     acc[2] += gx * gx;
-                     acc[3] += gy * gy;
-                   }
-                   if (i < NP) {
-                     const double l0 = m[i], l1 = m[NP + i];
-                     acc[4] += l0 * l0;
-                     acc[5] += l1 * l1;
-                     if (s2) {
-                       const double a = m[2 * NP + i], b = m[3 * NP + i], cc = m[4 * NP + i], d = m[5 * NP + i];
-                       acc[6] += a * a;
-                       acc[7] += b * b;
-                       acc[8] += cc * cc;
-                       acc[9] += d * d;
-                     }
-                     const double p = phi[i], pst = (double)(T(2) * phi[i] - phO[i]);
-                     const double y = ps[i], yst = (double)(T(2) * ps[i] - psO[i]);
-                     acc[10] += p * p;
-                     acc[11] += pst * pst;
-                     acc[12] += y * y;
-                     acc[13] += yst * yst;
-                   }
-                 },
-                 partials, red);
+                partials, redP);
+    // the squared norms of derivative fields by Parseval over the half spectra (the reference
+    // transforms each back): sum_x c2r(X)^2 = (1/P) [sum off the kx = 0, Nx/2 planes of 2 |X|^2
+    // + sum on them of |(X(k) + conj X(-k)) / 2|^2], the part of X that a c2r keeps.  Group n:
+    // |grad phi_n|^2 of rfft(phi) (x and y, as f_surf), (Lap phi)^2, (Lap phi*)^2, and with S2
+    // (dx phi)^2, (dy phi)^2, (dx phi*)^2, (dy phi*)^2 (phi* = 2 phi - phi_old); group 0 also
+    // (dx p)^2, (dy p)^2 -> red[256 + n*9 + k]
+    double* redS = red + 256;
+    const C* pa = w_phat;
+    const SpecC sc = spc();
+    const int Nx_ = Nx, Nxh_ = Nxh, Ny_ = Ny, Nz_ = Nz;
+    const KIdx ki{Nxh, Ny, Nz};
+    reduce_n<9>(s, H, N,
+                [=] __device__(long long i, int n, double(&acc)[9]) {
+                  int ix, iy, iz, f;
+                  ki(i, ix, iy, iz, f);
+                  const bool plane = ix == 0 || 2 * ix == Nx_;
+                  int iy2 = iy, iz2 = iz;
+                  long long im = i;
+                  if (plane) {
+                    iy2 = iy ? Ny_ - iy : 0;
+                    iz2 = iz ? Nz_ - iz : 0;
+                    im = ((long long)iz2 * Ny_ + iy2) * Nxh_ + ix;
+                  }
+                  const long long k = (long long)n * Hl + i, km = (long long)n * Hl + im;
+                  const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz], kym = sc.ky[iy2];
+                  const double l = -(kx * kx + ky * ky + kz * kz);  // -k^2 is even in k
+                  auto sq = [&](const CD& x, const CD& xm) {
+                    if (!plane) return 2.0 * (x.x * x.x + x.y * x.y);
+                    const double re = 0.5 * (x.x + xm.x), imv = 0.5 * (x.y - xm.y);
+                    return re * re + imv * imv;
+                  };
+                  auto dk = [](double kk, const CD& a) { return CD{-kk * a.y, kk * a.x}; };
+                  auto sc2 = [](double v, const CD& a) { return CD{v * a.x, v * a.y}; };
+                  const CD a = pa[k], am = pa[km];
+                  acc[0] += sq(dk(kx, a), dk(kx, am)) + sq(dk(ky, a), dk(kym, am));
+                  const CD x = phh[k], xo = phhO[k], xm = phh[km], xom = phhO[km];
+                  const CD st = CD{2.0 * x.x - xo.x, 2.0 * x.y - xo.y};
+                  const CD stm = CD{2.0 * xm.x - xom.x, 2.0 * xm.y - xom.y};
+                  acc[1] += sq(sc2(l, x), sc2(l, xm));
+                  acc[2] += sq(sc2(l, st), sc2(l, stm));
+                  if (s2) {
+                    acc[3] += sq(dk(kx, x), dk(kx, xm));
+                    acc[4] += sq(dk(ky, x), dk(kym, xm));
+                    acc[5] += sq(dk(kx, st), dk(kx, stm));
+                    acc[6] += sq(dk(ky, st), dk(kym, stm));
+                  }
+                  if (n == 0) {
+                    const CD pp = ph[i], ppm = ph[im];
+                    acc[7] += sq(dk(kx, pp), dk(kx, ppm));
+                    acc[8] += sq(dk(ky, pp), dk(kym, ppm));
+                  }
+                },
+                partials, redS);
+    const double invP = 1.0 / (double)P;
     const double* rd = red;
     DScal* dsw = ds;
     const DStateScal* stN = dstate + ip;
     const DStateScal* stO = dstate + io;
     const double lam = (double)c.lamda;
     const double* A0 = dA0;
-    const int ndl = nd, Nl = N;
+    const int Nl = N;
     single(s, [=] __device__() {
-      auto isq = [&](int k) { return integ(rd[k], q, ndl); };
-      const double kinetic = 0.25 * isq(0) + 0.25 * isq(1);
+      auto sumP = [&](int k) {
+        double t = rd[k];
+        for (int n = 1; n < Nl; ++n) t += rd[n * 7 + k];
+        return integ(t, q, ndl);
+      };
+      auto sumS = [&](int k) {
+        double t = rd[256 + k];
+        for (int n = 1; n < Nl; ++n) t += rd[256 + n * 9 + k];
+        return integ(t * invP, q, ndl);
+      };
+      const double kinetic = 0.25 * integ(rd[5], q, ndl) + 0.25 * integ(rd[6], q, ndl);
       const double dt = dsw->dt;
-      const double pressure = (dt * dt / 3.0) * (isq(2) + isq(3));
+      const double pressure = (dt * dt / 3.0) * (integ(rd[256 + 7] * invP, q, ndl) + integ(rd[256 + 8] * invP, q, ndl));
       auto se = [](double rn, double ro) { return 0.5 * rn * rn + 0.5 * (2 * rn - ro) * (2 * rn - ro); };
       const double Re = se(stN->R1, stO->R1) + se(stN->R2, stO->R2) + se(stN->R3, stO->R3);
       const double Qe = 0.5 * se(stN->Q, stO->Q);
-      const double S1t = 0.25 * q.S1 * (isq(4) + isq(5));
-      const double S2t = 0.25 * q.S2 * (isq(6) + isq(7) + isq(8) + isq(9));
-      const double S3t = 0.25 * q.S3 * (isq(10) + isq(11));
-      const double S4t = 0.25 * q.S4 * (isq(12) + isq(13));
+      const double S1t = 0.25 * q.S1 * (sumS(1) + sumS(2));
+      const double S2t = 0.25 * q.S2 * (sumS(3) + sumS(4) + sumS(5) + sumS(6));
+      const double S3t = 0.25 * q.S3 * (sumP(1) + sumP(2));
+      const double S4t = 0.25 * q.S4 * (sumP(3) + sumP(4));
       dsw->energy = (kinetic + pressure + lam * Re + Qe + lam * (S1t + S2t + S3t + S4t) -
                      lam * (q.C1 + q.C2 + q.C3) - 0.5);
       double num = 0.0, den = 0.0;
       for (int n = 0; n < Nl; ++n) {
-        const double A = integ(redA[n], q, ndl);
+        const double A = integ(rd[n * 7] + (0.5 * q.eps) * (rd[256 + n * 9] * invP), q, ndl);
         dsw->area_mon[n] = A;
         num += fabs(A - A0[n]);
         den += A0[n];
