@@ -173,8 +173,8 @@ class Solver final : public ISolver {
   double* partials = nullptr;
   double* red = nullptr;  // reduction outputs (512 doubles)
   // work
-  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_du, *w_dmn, *w_conv,
-      *w_coupl, *w_rhs2, *w_ut12;
+  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_du, *w_dmn,
+      *w_rhs2;
   C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp;
   // scratch of the forked branch (made by ensure_fork)
   R* w_grad = nullptr;
@@ -313,10 +313,7 @@ class Solver final : public ISolver {
     w_adv = dalloc<R>(2 * NP);
     w_du = dalloc<R>(9 * P);
     w_dmn = dalloc<R>(6 * NP);
-    w_conv = dalloc<R>(3 * P);
-    w_coupl = dalloc<R>(3 * P);
     w_rhs2 = dalloc<R>(3 * P);
-    w_ut12 = dalloc<R>(6 * P);
     w_phat = dalloc<C>(NH);
     w_omh = dalloc<C>(NH);
     w_Hh = dalloc<C>(4 * NH);
@@ -1875,15 +1872,18 @@ class Solver final : public ISolver {
       irfft_v(jl);
     }
     const R *us = w_us, *du = w_du, *dmn = w_dmn, *phs = w_phis, *pss = w_psis;
-    R *conv = w_conv, *coupl = w_coupl, *rhs2 = w_rhs2;
+    R* rhs2 = w_rhs2;
     const int ndd = nd, Nl = N;
     const long long Pl = P;
     const T lam = (T)c.lamda;
+    // rhs2 = -(u* . grad) u* - lamda sum_n (phi* grad mu* + psi* grad nu*); step 5 needs only its
+    // spectrum, so the convection and coupling fields are not kept
     foreach_n(s, P, [=] __device__(long long i) {
+      R conv[3];
       for (int cc = 0; cc < ndd; ++cc) {
         R a = us[i] * du[(cc * ndd) * Pl + i];
         for (int d = 1; d < ndd; ++d) a = a + us[d * Pl + i] * du[(cc * ndd + d) * Pl + i];
-        conv[cc * Pl + i] = a;
+        conv[cc] = a;
       }
       for (int d = 0; d < ndd; ++d) {
         R sum = T(0);
@@ -1892,8 +1892,7 @@ class Solver final : public ISolver {
           const T t = phs[k] * dmn[d * NP + k] + pss[k] * dmn[(ndd + d) * NP + k];
           sum = n == 0 ? t : sum + t;
         }
-        coupl[d * Pl + i] = sum;
-        rhs2[d * Pl + i] = -conv[d * Pl + i] - lam * sum;
+        rhs2[d * Pl + i] = -conv[d] - lam * sum;
       }
     });
     rfft({job(w_rhs2, w_rhs2h, nd)});
@@ -1921,7 +1920,6 @@ class Solver final : public ISolver {
       const CD r2 = r2h[i];
       uth[ndd * Hl + i] = CD{r2.x / lh, r2.y / lh};
     });
-    irfft({job(w_uth, w_ut12, 2 * nd)});
   }
 
   // ==========================================================================
@@ -1929,39 +1927,36 @@ class Solver final : public ISolver {
   // ==========================================================================
   void step5() {
     const long long NH = (long long)N * H;
-    // sum (u* . grad phi*) mu_j and sum (u* . grad psi*) nu_j by Parseval over the half spectra
-    // (the reference transforms mu_j, nu_j back): w_advh holds the rffts of the two advection
-    // fields (step 1), s3 the spectra mu_1, nu_1, mu_2, nu_2 in slots 1, 3, 5, 7
+    // The theta sums (Solvers.jl:344-423) by Parseval over the half spectra; the reference
+    // transforms mu_j, nu_j and u~_j back and sums in physical space.  w_advh holds the rffts of
+    // the two advection fields u* . grad phi*, u* . grad psi* (step 1), s3 the spectra mu_1, nu_1,
+    // mu_2, nu_2 in slots 1, 3, 5, 7, w_rhs2h the rfft of step 4's rhs2 = -(u* . grad) u* - lamda
+    // sum_n (phi* grad mu* + psi* grad nu*) and w_uth the spectra of u~_1 and u~_2, so
+    // lamda sum (coupling . u~_j) + sum (convection . u~_j) = -sum (rhs2 . u~_j).
     const C* ah = w_advh;
     const C* o = s3;
-    const int Nxh_ = Nxh, Nx_ = Nx;
-    reduce_n<4>(s, NH, 1,
-                [=] __device__(long long i, int, double(&acc)[4]) {
+    const C* r2h = w_rhs2h;
+    const C* uth = w_uth;
+    const int Nxh_ = Nxh, Nx_ = Nx, ndd = nd;
+    const long long Hl = H;
+    reduce_n<6>(s, NH, 1,
+                [=] __device__(long long i, int, double(&acc)[6]) {
                   const int ix = (int)(i % Nxh_);
                   const double w = (ix == 0 || 2 * ix == Nx_) ? 1.0 : 2.0;
+                  auto re = [&](const CD& a, const CD& b) { return w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y); };
                   const CD ap = ah[i], as = ah[NH + i];
-                  const CD m1 = o[NH + i], n1 = o[3 * NH + i], m2 = o[5 * NH + i], n2 = o[7 * NH + i];
-                  acc[0] += w * ((double)ap.x * (double)m1.x + (double)ap.y * (double)m1.y);  // t11
-                  acc[1] += w * ((double)as.x * (double)n1.x + (double)as.y * (double)n1.y);  // t12
-                  acc[2] += w * ((double)ap.x * (double)m2.x + (double)ap.y * (double)m2.y);  // t21
-                  acc[3] += w * ((double)as.x * (double)n2.x + (double)as.y * (double)n2.y);  // t22
+                  acc[0] += re(ap, o[NH + i]);      // t11
+                  acc[1] += re(as, o[3 * NH + i]);  // t12
+                  acc[2] += re(ap, o[5 * NH + i]);  // t21
+                  acc[3] += re(as, o[7 * NH + i]);  // t22
+                  if (i < Hl)
+                    for (int d = 0; d < ndd; ++d) {
+                      const CD r = r2h[d * Hl + i];
+                      acc[4] += re(r, uth[d * Hl + i]);
+                      acc[5] += re(r, uth[(ndd + d) * Hl + i]);
+                    }
                 },
                 partials, red);
-    const R *conv = w_conv, *coupl = w_coupl, *ut = w_ut12;
-    const long long Pl = P;
-    const int ndd = nd;
-    reduce_n<12>(s, P, 1,
-                 [=] __device__(long long i, int, double(&acc)[12]) {
-                   for (int d = 0; d < ndd; ++d) {
-                     const double u1 = ut[d * Pl + i], u2 = ut[(ndd + d) * Pl + i];
-                     const double cp = coupl[d * Pl + i], cv = conv[d * Pl + i];
-                     acc[d] += cp * u1;      // t13 parts
-                     acc[3 + d] += cv * u1;  // t14 parts
-                     acc[6 + d] += cp * u2;  // t23 parts
-                     acc[9 + d] += cv * u2;  // t24 parts
-                   }
-                 },
-                 partials, red + 4);
     const double invP = 1.0 / (double)P;
     const PhysC q = pc();
     const double* rd = red;
@@ -1970,18 +1965,11 @@ class Solver final : public ISolver {
     const DStateScal* stO = dstate + io;
     const double lam = (double)c.lamda;
     single(s, [=] __device__() {
-      double t13 = rd[4], t14 = rd[7], t23 = rd[10], t24 = rd[13];
-      for (int d = 1; d < ndd; ++d) {
-        t13 += rd[4 + d];
-        t14 += rd[7 + d];
-        t23 += rd[10 + d];
-        t24 += rd[13 + d];
-      }
       double dvol = q.dx * q.dy;
       if (ndd == 3) dvol = dvol * q.dz;
-      // red[0..3] are spectral sums (x 1/P), red[4..15] physical
-      const double th1 = dvol * (lam * (rd[0] * invP + rd[1] * invP + t13) + t14);
-      const double th2 = dvol * (lam * (rd[2] * invP + rd[3] * invP + t23) + t24);
+      // spectral sums: x 1/P
+      const double th1 = dvol * (lam * (rd[0] * invP + rd[1] * invP) - rd[4] * invP);
+      const double th2 = dvol * (lam * (rd[2] * invP + rd[3] * invP) - rd[5] * invP);
       const double a = dsw->a, b = dsw->b, cc = dsw->c, dt = dsw->dt5;
       const double rhsQ = -(b * stN->Q + cc * stO->Q) / dt + th1;
       const double lhsQ = (a / dt) - th2;
@@ -2199,6 +2187,7 @@ class Solver final : public ISolver {
   void set_state(int which, const double* const* a, const double* scal, const double* A0, double dt) override {
     const int b = which == SAV_PRESENT ? ip : io;
     const long long NP = (long long)N * P, NH = (long long)N * H;
+    lazy_phys[b] = false;
     // order: phi, phi_hat, psi, psi_hat, u, u_hat, p, p_hat, mu, mu_hat, nu, nu_hat
     put_real(a[0], fphys(b, F_PHI), NP);
     put_cplx(a[1], fhalf(b, F_PHI), NH);
@@ -2225,6 +2214,7 @@ class Solver final : public ISolver {
   void get_state(int which, double* const* a, double* scal, double* A0, double* dt) override {
     const int b = which == SAV_PRESENT ? ip : io;
     const long long NP = (long long)N * P, NH = (long long)N * H;
+    materialize(b);
     get_real(a[0], fphys(b, F_PHI), NP);
     get_cplx(a[1], fhalf(b, F_PHI), NH);
     get_real(a[2], fphys(b, F_PSI), NP);
@@ -2290,6 +2280,7 @@ class Solver final : public ISolver {
 
   int init_sav() override {
     const long long NH = (long long)N * H, NP = (long long)N * P;
+    materialize(ip);  // old := present copies the physical fields too
     set_bdf(1.0, -1.0, 0.0, present_dt > 0 ? present_dt : c.dt);
     functionals(fphys(ip, F_PHI), fphys(ip, F_PSI), true, sizeof(R) == 8 ? nullptr : fhalf(ip, F_PHI));
     int rc = check_flags();
@@ -2332,6 +2323,7 @@ class Solver final : public ISolver {
     SAV_HIP(hipMemcpyAsync(shalf[io], shalf[ip], nhalf * sizeof(C), hipMemcpyDeviceToDevice, s));
     SAV_HIP(hipMemcpyAsync(dstate + io, dstate + ip, sizeof(DStateScal), hipMemcpyDeviceToDevice, s));
     SAV_HIP(hipStreamSynchronize(s));
+    lazy_phys[io] = false;
     (void)NP;
     return 0;
   }
@@ -2350,9 +2342,18 @@ class Solver final : public ISolver {
   }
   // After rotate(): the back-transform of the new state (simulation.jl:99-104) and the
   // energy/area monitor (simulation.jl:146-148).
+  // phi, psi and u, which the next step and the monitor read, are transformed here; mu, nu and p
+  // (read only by the host) when the state is read (materialize).
   void step_tail(bool monitor_on) {
-    irfft({job(shalf[ip], sphys[ip], 4 * N + nd + 1)});
+    irfft({job(shalf[ip], sphys[ip], 2 * N), job(uhalf(ip), uphys(ip), nd)});
     if (monitor_on) monitor();
+  }
+  // lazy_phys[b]: the physical mu, nu and p of buffer b are not yet back-transformed from its spectra
+  bool lazy_phys[3] = {false, false, false};
+  void materialize(int b) {
+    if (!lazy_phys[b]) return;
+    irfft({job(fhalf(b, F_MU), fphys(b, F_MU), 2 * N), job(phalf(b), pphys(b), 1)});
+    lazy_phys[b] = false;
   }
 
   // Step statistics and status from hs, hks, hst (copied from the device by the caller).
@@ -2492,6 +2493,7 @@ class Solver final : public ISolver {
         kc.guess[z] = kit[z] > 0 ? kit[z] : 1;
       }
     rotate();
+    lazy_phys[ip] = true;
     rc = finish_step(st);
     return true;
   }
@@ -2541,6 +2543,7 @@ class Solver final : public ISolver {
     // W + C < 0 or non-finite step is then not committed: present and old are intact.
     rotate();
     step_tail(monitor_on);
+    lazy_phys[ip] = true;  // (a step that is not committed leaves it on the discarded buffer)
     SAV_HIP(hipMemcpyAsync(hs, ds, sizeof(DScal), hipMemcpyDeviceToHost, s));
     SAV_HIP(hipMemcpyAsync(hst, dstate + ip, sizeof(DStateScal), hipMemcpyDeviceToHost, s));
     SAV_HIP(hipStreamSynchronize(s));
