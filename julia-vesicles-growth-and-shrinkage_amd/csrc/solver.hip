@@ -661,16 +661,43 @@ class Solver final : public ISolver {
   };
   SpecC spc() const { return SpecC{d_kx, d_ky, d_kz, c.S1, c.S2, c.S3, c.S4, c.M_phi, c.eta, c.M0_psi}; }
 
+  // Right-hand sides of the psi solves: b_z = fft(irfft(rhs_z)) (Solvers.jl:70-71) for nsys
+  // systems of N half spectra (rhs_z at z N H, b_z at z vs), written directly: the full
+  // spectrum of the real field irfft(X) is, for kx <= Nx/2, the part of X that a c2r keeps
+  // (the Hermitian part (X(k) + conj X(-k)) / 2 on the kx = 0 and kx = Nx/2 columns, X
+  // itself elsewhere) and beyond that its conjugate at (Nx - kx, -ky); equal to the two
+  // transforms up to their round-off, in one pass.
+  void full_rhs(const C* rhs, C* b, int nsys, long long vs) {
+    const long long NP = (long long)N * P, nh = (long long)N * H, Hl = H;
+    const int Nx_ = Nx, Ny_ = Ny, Nxh_ = Nxh;
+    foreach_n(s, nsys * NP, [=] __device__(long long i) {
+      const long long z = i / NP, j = i % NP;
+      const int ix = (int)(j % Nx_);
+      const long long r = j / Nx_;
+      const int iy = (int)(r % Ny_);
+      const long long f = r / Ny_;
+      const int my = iy ? Ny_ - iy : 0;
+      const C* X = rhs + z * nh + f * Hl;
+      CD v;
+      if (ix < Nxh_) {
+        v = X[(long long)iy * Nxh_ + ix];
+        if (ix == 0 || 2 * ix == Nx_) {
+          const CD m = X[(long long)my * Nxh_ + ix];
+          v = CD{T(0.5) * (v.x + m.x), T(0.5) * (v.y - m.y)};
+        }
+      } else {
+        const CD m = X[(long long)my * Nxh_ + (Nx_ - ix)];
+        v = CD{m.x, -m.y};
+      }
+      b[z * vs + j] = v;
+    });
+  }
+
   // ==========================================================================
   // psi solve (2D): out = solve_psi!(rhs) -- Solvers.jl:68-81
   // ==========================================================================
   int psi_solve(const C* rhs, C* out, int slot) {
-    const long long NP = (long long)N * P;
-    R* rphys = w_d4 + 4 * NP;  // scratch (d4 slots 4,5 are free during solves)
-    irfft({job(rhs, rphys, N)});
-    C* b = kc.b;
-    foreach_n(s, NP, [=] __device__(long long i) { b[i] = CD{rphys[i], T(0)}; });
-    fft_full_2d<R>(g, b, N, false, s);
+    full_rhs(rhs, kc.b, 1, (long long)N * P);
     int it;
     if (mixed) {
       mobility_fp32();
@@ -759,16 +786,9 @@ class Solver final : public ISolver {
   // batched BiCGSTAB: systems z = 0, 1, 2 use KScal slots 0, 1, 2 and the vectors at
   // offset z * kc.vs.  Each system's iterates equal those of its own solve.
   void psi_solve3(const C* rhs, C* o0, C* o1, C* o2) {
-    const long long NP = (long long)N * P, nf = kc.nfull;
-    R* rphys = reinterpret_cast<R*>(kc.Bx);  // 3 N real fields of scratch before the solve
-    irfft({job(rhs, rphys, 3 * N)});
-    C* b = kc.b;
+    const long long nf = kc.nfull;
     const long long vs = kc.vs;
-    foreach_n(s, 3 * NP, [=] __device__(long long i) {
-      const long long z = i / NP, j = i % NP;
-      b[z * vs + j] = CD{rphys[i], T(0)};
-    });
-    fft_full_2d<R>(g, b, 3 * N, false, s);  // vs == nf: the three systems are contiguous
+    full_rhs(rhs, kc.b, 3, vs);
     if (mixed) {
       mobility_fp32();
       refined_solve(0, 3);
