@@ -515,6 +515,32 @@ class Solver final : public ISolver {
     return v;
   }
 
+  // out = the part of X = 2 a - b (X = a when b is null) that a c2r keeps: X itself off the
+  // kx = 0 and kx = Nx/2 planes, (X(k) + conj X(-k)) / 2 on them; `count` fields of H bins.
+  // rfft(irfft(X)) equals it up to the transforms' round-off.
+  void hermitian_part(const C* a, const C* b, C* out, int count) {
+    const int Nx_ = Nx, Nxh_ = Nxh, Ny_ = Ny, Nz_ = Nz;
+    const long long Hl = H;
+    const KIdx ki{Nxh, Ny, Nz};
+    foreach_n(s, (long long)count * H, [=] __device__(long long i) {
+      int ix, iy, iz, f;
+      ki(i, ix, iy, iz, f);
+      auto ld = [&](long long j) {
+        const CD x = a[j];
+        if (!b) return x;
+        const CD y = b[j];
+        return CD{T(2) * x.x - y.x, T(2) * x.y - y.y};
+      };
+      CD v = ld(i);
+      if (ix == 0 || 2 * ix == Nx_) {
+        const int iy2 = iy ? Ny_ - iy : 0, iz2 = iz ? Nz_ - iz : 0;
+        const CD m = ld((long long)f * Hl + ((long long)iz2 * Ny_ + iy2) * Nxh_ + ix);
+        v = CD{T(0.5) * (v.x + m.x), T(0.5) * (v.y - m.y)};
+      }
+      out[i] = v;
+    });
+  }
+
   // =========================================================================
   // Functionals at (phi, psi): fills w_H = [H1 | G | H2 | H3] (physical), the
   // scalars W1..W3, R1s..R3s, area[n] (Utils.jl:36-249).  When set_A0 is true
@@ -524,19 +550,20 @@ class Solver final : public ISolver {
     const long long NP = (long long)N * P;
     const PhysC q = pc();
     const int ndl = nd;
-    // derivatives gx, gy, lap, bih of phi.  FP64: from rfft(phi) as the reference does
-    // (Utils.jl:37, 149, 167, 209).  FP32 (`spec` given): from the state spectra
-    // 2 phi_hat^n - phi_hat^{n-1}, because the FFT roundoff floor of rfft(phi) in FP32
-    // (~1e-7 of the largest coefficient at every k) would be multiplied by S1 k^4 in
-    // the biharmonic term; spectra stored in FP32 keep their own relative precision.
-    const C* src = w_phat;
-    const C* src2 = nullptr;
+    // derivatives gx, gy, lap, bih of phi from rfft(phi) (Utils.jl:37, 149, 167, 209).  In a
+    // step (`spec` given) phi = 2 phi^n - phi^{n-1} is the extrapolation of two states whose
+    // physical fields are the c2r of their spectra, so rfft(phi) is the part of
+    // 2 phi_hat^n - phi_hat^{n-1} that a c2r keeps (its Hermitian part on the kx = 0 and
+    // kx = Nx/2 planes), formed here in one pass instead of a transform.  In FP32 this also
+    // keeps the FFT roundoff floor of rfft(phi) (~1e-7 of the largest coefficient at every k)
+    // out of the S1 k^4 biharmonic term.
     if (spec) {
-      src = spec;
-      src2 = spec_old;
+      hermitian_part(spec, spec_old, w_phat, N);
     } else {
       rfft({job(phi, w_phat, N)});
     }
+    const C* src = w_phat;
+    const C* src2 = nullptr;
     R* gx = w_d4;
     R* gy = w_d4 + NP;
     R* lap = w_d4 + 2 * NP;
@@ -1534,10 +1561,7 @@ class Solver final : public ISolver {
       max_reduce(M, NP, &ds->Mmax);
       stencil_stale = true;
     }
-    if (sizeof(R) == 8)
-      functionals(phs, pss, false);
-    else
-      functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));
+    functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));
     // rfft of H1, G, H2, H3 -> w_Hh (same order)
     rfft({job(w_H, w_Hh, 4 * N)});
     // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115)
@@ -2074,9 +2098,9 @@ class Solver final : public ISolver {
   void monitor() {
     const C *phh = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *ph = phalf(ip);
     const bool s2 = c.S2 != 0.0;
-    // area: f_surf of rfft(phi) (calculate_area)
+    // area: f_surf of rfft(phi) (calculate_area); phi is the c2r of phi_hat, so rfft(phi) is the
+    // Hermitian part of phi_hat on the kx = 0 and kx = Nx/2 planes, phi_hat elsewhere
     const R* phi = fphys(ip, F_PHI);
-    rfft({job(phi, w_phat, N)});
     const PhysC q = pc();
     const R *u = uphys(ip), *uo = uphys(io), *phO = fphys(io, F_PHI), *ps = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
     const long long Pl = P, Hl = H;
@@ -2113,7 +2137,6 @@ class Solver final : public ISolver {
     // (dx phi)^2, (dy phi)^2, (dx phi*)^2, (dy phi*)^2 (phi* = 2 phi - phi_old); group 0 also
     // (dx p)^2, (dy p)^2 -> red[256 + n*9 + k]
     double* redS = red + 256;
-    const C* pa = w_phat;
     const SpecC sc = spc();
     const int Nx_ = Nx, Nxh_ = Nxh, Ny_ = Ny, Nz_ = Nz;
     const KIdx ki{Nxh, Ny, Nz};
@@ -2139,9 +2162,13 @@ class Solver final : public ISolver {
                   };
                   auto dk = [](double kk, const CD& a) { return CD{-kk * a.y, kk * a.x}; };
                   auto sc2 = [](double v, const CD& a) { return CD{v * a.x, v * a.y}; };
-                  const CD a = pa[k], am = pa[km];
-                  acc[0] += sq(dk(kx, a), dk(kx, am)) + sq(dk(ky, a), dk(kym, am));
                   const CD x = phh[k], xo = phhO[k], xm = phh[km], xom = phhO[km];
+                  CD a = x, am = xm;  // rfft(phi) at k and -k
+                  if (plane) {
+                    a = CD{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)};
+                    am = CD{a.x, -a.y};
+                  }
+                  acc[0] += sq(dk(kx, a), dk(kx, am)) + sq(dk(ky, a), dk(kym, am));
                   const CD st = CD{2.0 * x.x - xo.x, 2.0 * x.y - xo.y};
                   const CD stm = CD{2.0 * xm.x - xom.x, 2.0 * xm.y - xom.y};
                   acc[1] += sq(sc2(l, x), sc2(l, xm));
