@@ -175,7 +175,7 @@ class Solver final : public ISolver {
   // work
   R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_du, *w_dmn,
       *w_rhs2;
-  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *s3, *w_rhs2h, *w_uth, *tmp;
+  C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *w_rhs2h, *w_uth, *tmp;
   // scratch of the forked branch (made by ensure_fork)
   R* w_grad = nullptr;
   C* tmp4 = nullptr;
@@ -321,7 +321,6 @@ class Solver final : public ISolver {
     w_wh = dalloc<C>(3 * NH);
     w_rhs = dalloc<C>(3 * NH);
     w_advh = dalloc<C>(2 * NH);
-    s3 = dalloc<C>(8 * NH);
     w_rhs2h = dalloc<C>(3 * H);
     w_uth = dalloc<C>(6 * H);
     tmp = dalloc<C>((16LL * N + 16) * H);
@@ -1877,26 +1876,26 @@ class Solver final : public ISolver {
   // ==========================================================================
   // Step 3 (Solvers.jl:242-259)
   // ==========================================================================
-  void step3() {
-    const long long NH = (long long)N * H;
-    const C* k = cache;
-    C* o = s3;
-    const DScal* dsl = ds;
-    foreach_n(s, NH, [=] __device__(long long i) {
-      const T R11 = (T)dsl->Rs[0], R21 = (T)dsl->Rs[1], R31 = (T)dsl->Rs[2];
-      const T R12 = (T)dsl->Rs[3], R22 = (T)dsl->Rs[4], R32 = (T)dsl->Rs[5];
-      auto at = [&](int f) { return k[f * NH + i]; };
-      auto ax = [](C a, R r, C b) { return CD{a.x + r * b.x, a.y + r * b.y}; };
-      o[0 * NH + i] = ax(ax(ax(at(PHI11), R11, at(PHI12)), R21, at(PHI13)), R31, at(PHI14));
-      o[1 * NH + i] = ax(ax(ax(at(MU11), R11, at(MU12)), R21, at(MU13)), R31, at(MU14));
-      o[2 * NH + i] = ax(at(PSI11), R31, at(PSI12));
-      o[3 * NH + i] = ax(at(NU11), R31, at(NU12));
-      o[4 * NH + i] = ax(ax(ax(at(PHI21), R12, at(PHI12)), R22, at(PHI13)), R32, at(PHI14));
-      o[5 * NH + i] = ax(ax(ax(at(MU21), R12, at(MU12)), R22, at(MU13)), R32, at(MU14));
-      o[6 * NH + i] = ax(at(PSI21), R32, at(PSI12));
-      o[7 * NH + i] = ax(at(NU21), R32, at(NU12));
-    });
-  }
+  // The combinations of step 3 (Solvers.jl:242-259): x_1 = x_11 + R11 x_12 + R21 x_13 + R31 x_14
+  // for phi and mu (x_11 + R31 x_12 for psi and nu) and x_2 alike with x_21, R12, R22, R32.
+  // They are formed where they are used, in step 5's sums and step 6's update, so the eight
+  // combined half spectra are never stored.
+  struct Step3 {
+    const C* k;
+    long long NH;
+    __device__ __forceinline__ CD at(int f, long long i) const { return k[f * NH + i]; }
+    static __device__ __forceinline__ CD ax(const CD& a, T r, const CD& b) { return CD{a.x + r * b.x, a.y + r * b.y}; }
+    // which: 0 phi, 1 mu, 2 psi, 3 nu; part: 0 -> x_1, 1 -> x_2
+    __device__ __forceinline__ CD comb(int which, int part, long long i, const double* Rs) const {
+      const T Ra = (T)Rs[3 * part + 0], Rb = (T)Rs[3 * part + 1], Rc = (T)Rs[3 * part + 2];
+      switch (which) {
+        case 0: return ax(ax(ax(at(part ? PHI21 : PHI11, i), Ra, at(PHI12, i)), Rb, at(PHI13, i)), Rc, at(PHI14, i));
+        case 1: return ax(ax(ax(at(part ? MU21 : MU11, i), Ra, at(MU12, i)), Rb, at(MU13, i)), Rc, at(MU14, i));
+        case 2: return ax(at(part ? PSI21 : PSI11, i), Rc, at(PSI12, i));
+        default: return ax(at(part ? NU21 : NU11, i), Rc, at(NU12, i));
+      }
+    }
+  };
 
   // ==========================================================================
   // Step 4 (Solvers.jl:267-337; Solvers_3d.jl:221-292)
@@ -1973,12 +1972,13 @@ class Solver final : public ISolver {
     const long long NH = (long long)N * H;
     // The theta sums (Solvers.jl:344-423) by Parseval over the half spectra; the reference
     // transforms mu_j, nu_j and u~_j back and sums in physical space.  w_advh holds the rffts of
-    // the two advection fields u* . grad phi*, u* . grad psi* (step 1), s3 the spectra mu_1, nu_1,
-    // mu_2, nu_2 in slots 1, 3, 5, 7, w_rhs2h the rfft of step 4's rhs2 = -(u* . grad) u* - lamda
-    // sum_n (phi* grad mu* + psi* grad nu*) and w_uth the spectra of u~_1 and u~_2, so
-    // lamda sum (coupling . u~_j) + sum (convection . u~_j) = -sum (rhs2 . u~_j).
+    // the two advection fields u* . grad phi*, u* . grad psi* (step 1), the step-3 combinations
+    // give the spectra mu_1, nu_1, mu_2, nu_2, w_rhs2h is the rfft of step 4's rhs2 =
+    // -(u* . grad) u* - lamda sum_n (phi* grad mu* + psi* grad nu*) and w_uth the spectra of
+    // u~_1 and u~_2, so lamda sum (coupling . u~_j) + sum (convection . u~_j) = -sum (rhs2 . u~_j).
     const C* ah = w_advh;
-    const C* o = s3;
+    const Step3 k3{cache, NH};
+    const DScal* dsl = ds;
     const C* r2h = w_rhs2h;
     const C* uth = w_uth;
     const int Nxh_ = Nxh, Nx_ = Nx, ndd = nd;
@@ -1989,10 +1989,10 @@ class Solver final : public ISolver {
                   const double w = (ix == 0 || 2 * ix == Nx_) ? 1.0 : 2.0;
                   auto re = [&](const CD& a, const CD& b) { return w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y); };
                   const CD ap = ah[i], as = ah[NH + i];
-                  acc[0] += re(ap, o[NH + i]);      // t11
-                  acc[1] += re(as, o[3 * NH + i]);  // t12
-                  acc[2] += re(ap, o[5 * NH + i]);  // t21
-                  acc[3] += re(as, o[7 * NH + i]);  // t22
+                  acc[0] += re(ap, k3.comb(1, 0, i, dsl->Rs));  // t11: mu_1
+                  acc[1] += re(as, k3.comb(3, 0, i, dsl->Rs));  // t12: nu_1
+                  acc[2] += re(ap, k3.comb(1, 1, i, dsl->Rs));  // t21: mu_2
+                  acc[3] += re(as, k3.comb(3, 1, i, dsl->Rs));  // t22: nu_2
                   if (i < Hl)
                     for (int d = 0; d < ndd; ++d) {
                       const CD r = r2h[d * Hl + i];
@@ -2029,7 +2029,7 @@ class Solver final : public ISolver {
   // ==========================================================================
   void step67() {
     const long long NH = (long long)N * H;
-    const C* o = s3;
+    const Step3 k3{cache, NH};
     C *phN = fhalf(inew, F_PHI), *psN = fhalf(inew, F_PSI), *muN = fhalf(inew, F_MU), *nuN = fhalf(inew, F_NU);
     C* uN = uhalf(inew);
     C* pN = phalf(inew);
@@ -2044,8 +2044,8 @@ class Solver final : public ISolver {
       const T Q = (T)dsl->Q;
       for (int n = 0; n < Nl; ++n) {
         const long long k = (long long)n * Hl + i;
-        auto cmb = [&](int f) {
-          const CD a = o[f * NH + k], b = o[(f + 4) * NH + k];
+        auto cmb = [&](int f) {  // x^{n+1} = x_1 + Q x_2 (Solvers.jl:431-453)
+          const CD a = k3.comb(f, 0, k, dsl->Rs), b = k3.comb(f, 1, k, dsl->Rs);
           return CD{a.x + Q * b.x, a.y + Q * b.y};
         };
         phN[k] = cmb(0);
@@ -2471,7 +2471,6 @@ class Solver final : public ISolver {
       capture_iters = K > 0 ? K : 1;
       step1();
       step2();
-      step3();
       step4();
       step5();
       step67();
@@ -2578,7 +2577,6 @@ class Solver final : public ISolver {
     }
     fork_step4 = false;
     step2();
-    step3();
     if (forked)
       SAV_HIP(hipStreamWaitEvent(s, ev_join, 0));
     else
