@@ -1705,18 +1705,7 @@ class Solver final : public ISolver {
       });
       kit[0] = kit[1] = kit[2] = 0;
     }
-    // nu = S4 psi (+ G_hat for nu_12)
-    {
-      C* cch2 = cache;
-      const T S4 = (T)c.S4;
-      foreach_n(s, NH, [=] __device__(long long i) {
-        const CD a = cch2[PSI11 * NH + i], b = cch2[PSI12 * NH + i], d = cch2[PSI21 * NH + i];
-        const CD gh = Gh[i];
-        cch2[NU11 * NH + i] = CD{S4 * a.x, S4 * a.y};
-        cch2[NU12 * NH + i] = CD{S4 * b.x + gh.x, S4 * b.y + gh.y};
-        cch2[NU21 * NH + i] = CD{S4 * d.x, S4 * d.y};
-      });
-    }
+    // nu_11, nu_12, nu_21 = S4 psi (+ G_hat for nu_12) are written by step 2's pass
   }
 
   int kit[3] = {0, 0, 0};
@@ -1731,58 +1720,62 @@ class Solver final : public ISolver {
   // Step 2 (Solvers.jl:144-234): 22 inner products, two 3x3 LU solves
   // ==========================================================================
   void step2() {
-    const long long NP = (long long)N * P, NH = (long long)N * H;
-    // <H_i, b phi^n + c phi^{n-1}> and <G, b psi^n + c psi^{n-1}> over the physical fields
-    const R *phN = fphys(ip, F_PHI), *phO = fphys(io, F_PHI), *psN = fphys(ip, F_PSI), *psO = fphys(io, F_PSI);
-    const R *H1 = w_H, *G = w_H + NP, *H2 = w_H + 2 * NP, *H3 = w_H + 3 * NP;
-    const DScal* dsl = ds;
-    reduce_n<4>(s, NP, 1,
-                [=] __device__(long long i, int, double(&acc)[4]) {
-                  const T b = (T)dsl->b, cc = (T)dsl->c;
-                  const double tph = (double)(b * phN[i] + cc * phO[i]);
-                  const double tps = (double)(b * psN[i] + cc * psO[i]);
-                  acc[0] += (double)H1[i] * tph;
-                  acc[1] += (double)H2[i] * tph;
-                  acc[2] += (double)H3[i] * tph;
-                  acc[3] += (double)G[i] * tps;
-                },
-                partials, red);
-    // the 18 products with step 1's split solutions, by Parseval over the half spectra (the
-    // reference transforms the 8 split solutions back and sums in physical space): H_hat and
-    // G_hat are rffts of real fields, so sum_x h f = (1/P) sum_k w(kx) Re(conj(h_hat) f_hat)
-    // with w = 1 on the kx = 0 and kx = Nx/2 planes and 2 elsewhere, for any f_hat whose c2r is f
+    const long long NH = (long long)N * H;
+    // The 22 inner products (Solvers.jl:144-234) by Parseval over the half spectra; the
+    // reference transforms the eight split solutions back and sums in physical space.  H_hat
+    // and G_hat are rffts of real fields, so sum_x h f = (1/P) sum_k w(kx) Re(conj(h_hat) f_hat)
+    // with w = 1 on the kx = 0 and kx = Nx/2 planes and 2 elsewhere, for any f_hat whose c2r is
+    // f: the split solutions, and b phi^n + c phi^{n-1} (b psi^n + c psi^{n-1}) from the state
+    // spectra, whose c2r are the states' physical fields.  The same pass writes
+    // nu_11, nu_12, nu_21 = S4 psi (+ G_hat for nu_12) of step 1 (Solvers.jl:92, 107, 134).
     const C* Hh = w_Hh;  // [H1 | G | H2 | H3]
-    const C* cch = cache;
+    C* cch = cache;
+    const C *phN = fhalf(ip, F_PHI), *phO = fhalf(io, F_PHI), *psN = fhalf(ip, F_PSI), *psO = fhalf(io, F_PSI);
+    const DScal* dsl = ds;
     const int Nxh_ = Nxh, Nx_ = Nx;
-    reduce_n<18>(s, NH, 1,
-                 [=] __device__(long long i, int, double(&acc)[18]) {
+    const T S4 = (T)c.S4;
+    reduce_n<22>(s, NH, 1,
+                 [=] __device__(long long i, int, double(&acc)[22]) {
                    const int ix = (int)(i % Nxh_);
                    const double w = (ix == 0 || 2 * ix == Nx_) ? 1.0 : 2.0;
                    const CD h1 = Hh[i], gg = Hh[NH + i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
-                   auto re = [&](const CD& a, int slot) {
-                     const CD f = cch[(long long)slot * NH + i];
+                   auto re = [&](const CD& a, const CD& f) {
                      return w * ((double)a.x * (double)f.x + (double)a.y * (double)f.y);
                    };
-                   acc[0] += re(h1, PHI12);
-                   acc[1] += re(h1, PHI13);
-                   acc[2] += re(h1, PHI14);
-                   acc[3] += re(h2, PHI12);
-                   acc[4] += re(h2, PHI13);
-                   acc[5] += re(h2, PHI14);
-                   acc[6] += re(h3, PHI12);
-                   acc[7] += re(h3, PHI13);
-                   acc[8] += re(h3, PHI14);
-                   acc[9] += re(gg, PSI12);
-                   acc[10] += re(h1, PHI11);
-                   acc[11] += re(h2, PHI11);
-                   acc[12] += re(h3, PHI11);
-                   acc[13] += re(gg, PSI11);
-                   acc[14] += re(h1, PHI21);
-                   acc[15] += re(h2, PHI21);
-                   acc[16] += re(h3, PHI21);
-                   acc[17] += re(gg, PSI21);
+                   auto at = [&](int slot) { return (CD)cch[(long long)slot * NH + i]; };
+                   const T b = (T)dsl->b, cc = (T)dsl->c;
+                   const CD xn = phN[i], xo = phO[i], yn = psN[i], yo = psO[i];
+                   const CD tph = CD{b * xn.x + cc * xo.x, b * xn.y + cc * xo.y};
+                   const CD tps = CD{b * yn.x + cc * yo.x, b * yn.y + cc * yo.y};
+                   acc[0] += re(h1, tph);
+                   acc[1] += re(h2, tph);
+                   acc[2] += re(h3, tph);
+                   acc[3] += re(gg, tps);
+                   const CD p12 = at(PHI12), p13 = at(PHI13), p14 = at(PHI14), p11 = at(PHI11), p21 = at(PHI21);
+                   const CD s11 = at(PSI11), s12 = at(PSI12), s21 = at(PSI21);
+                   acc[4] += re(h1, p12);
+                   acc[5] += re(h1, p13);
+                   acc[6] += re(h1, p14);
+                   acc[7] += re(h2, p12);
+                   acc[8] += re(h2, p13);
+                   acc[9] += re(h2, p14);
+                   acc[10] += re(h3, p12);
+                   acc[11] += re(h3, p13);
+                   acc[12] += re(h3, p14);
+                   acc[13] += re(gg, s12);
+                   acc[14] += re(h1, p11);
+                   acc[15] += re(h2, p11);
+                   acc[16] += re(h3, p11);
+                   acc[17] += re(gg, s11);
+                   acc[18] += re(h1, p21);
+                   acc[19] += re(h2, p21);
+                   acc[20] += re(h3, p21);
+                   acc[21] += re(gg, s21);
+                   cch[NU11 * NH + i] = CD{S4 * s11.x, S4 * s11.y};
+                   cch[NU12 * NH + i] = CD{S4 * s12.x + gg.x, S4 * s12.y + gg.y};
+                   cch[NU21 * NH + i] = CD{S4 * s21.x, S4 * s21.y};
                  },
-                 partials, red + 4);
+                 partials, red);
     const PhysC q = pc();
     const int ndl = nd;
     const double* rd = red;
@@ -1791,8 +1784,7 @@ class Solver final : public ISolver {
     const DStateScal* stO = dstate + io;
     const double invP = 1.0 / (double)P;
     single(s, [=] __device__() {
-      // red[0..3] physical sums, red[4..21] spectral sums (x 1/P)
-      auto idot = [&](int k) { return integ(k < 4 ? rd[k] : rd[k] * invP, q, ndl); };
+      auto idot = [&](int k) { return integ(rd[k] * invP, q, ndl); };
       const double a = dsw->a, b = dsw->b, cc = dsw->c;
       const double U1 = -(b * stN->R1 + cc * stO->R1) / a + idot(0) / (2.0 * a);
       const double U2 = -(b * stN->R2 + cc * stO->R2) / a + idot(1) / (2.0 * a);
