@@ -173,7 +173,7 @@ class Solver final : public ISolver {
   double* partials = nullptr;
   double* red = nullptr;  // reduction outputs (512 doubles)
   // work
-  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_lom, *w_H, *w_w, *w_adv, *w_du, *w_dmn,
+  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_H, *w_w, *w_adv, *w_du, *w_dmn,
       *w_rhs2;
   C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *w_rhs2h, *w_uth, *tmp;
   // scratch of the forked branch (made by ensure_fork)
@@ -307,7 +307,6 @@ class Solver final : public ISolver {
     w_us = dalloc<R>(nd * P);
     w_d4 = dalloc<R>(6 * NP);
     w_om = dalloc<R>(NP);
-    w_lom = dalloc<R>(NP);
     w_H = dalloc<R>(4 * NP);
     w_w = dalloc<R>(3 * NP);
     w_adv = dalloc<R>(2 * NP);
@@ -541,15 +540,17 @@ class Solver final : public ISolver {
   }
 
   // =========================================================================
-  // Functionals at (phi, psi): fills w_H = [H1 | G | H2 | H3] (physical), the
-  // scalars W1..W3, R1s..R3s, area[n] (Utils.jl:36-249).  When set_A0 is true
-  // (SAV initialisation) A0 := area first, as simulation.jl:25-26 does.
+  // Functionals at (phi, psi): fills w_Hh = rfft of [H1 | G | H2 | H3] (and, with
+  // phys_H, w_H with the physical fields), the scalars W1..W3, R1s..R3s, area[n]
+  // (Utils.jl:36-249).  When set_A0 is true (SAV initialisation) A0 := area first, as
+  // simulation.jl:25-26 does.
   // =========================================================================
-  void functionals(const R* phi, const R* psi, bool set_A0, const C* spec = nullptr, const C* spec_old = nullptr) {
-    const long long NP = (long long)N * P;
+  void functionals(const R* phi, const R* psi, bool set_A0, const C* spec = nullptr, const C* spec_old = nullptr,
+                   bool phys_H = false) {
+    const long long NP = (long long)N * P, NH = (long long)N * H;
     const PhysC q = pc();
     const int ndl = nd;
-    // derivatives gx, gy, lap, bih of phi from rfft(phi) (Utils.jl:37, 149, 167, 209).  In a
+    // derivatives gx, gy, lap of phi from rfft(phi) (Utils.jl:37, 149, 167, 209).  In a
     // step (`spec` given) phi = 2 phi^n - phi^{n-1} is the extrapolation of two states whose
     // physical fields are the c2r of their spectra, so rfft(phi) is the part of
     // 2 phi_hat^n - phi_hat^{n-1} that a c2r keeps (its Hermitian part on the kx = 0 and
@@ -566,9 +567,7 @@ class Solver final : public ISolver {
     R* gx = w_d4;
     R* gy = w_d4 + NP;
     R* lap = w_d4 + 2 * NP;
-    R* bih = w_d4 + 3 * NP;
-    irfft({job(src, gx, N, MUL_DX, src2), job(src, gy, N, MUL_DY, src2), job(src, lap, N, MUL_LAP, src2),
-           job(src, bih, N, MUL_BIH, src2)});
+    irfft({job(src, gx, N, MUL_DX, src2), job(src, gy, N, MUL_DY, src2), job(src, lap, N, MUL_LAP, src2)});
     // per-vesicle sums: f_surf, f_bend, E2 integrand, f_osm, psi^2 ; omega -> w_om
     R* om = w_om;
     const double* pin = d_psi_in;
@@ -649,25 +648,21 @@ class Solver final : public ISolver {
       dsl->R2s = sqrt(W2 + q.C2);
       dsl->R3s = sqrt(W3 + q.C3);
     });
-    // Laplacian of omega
+    // H1, G, H2, H3 (Utils.jl:198-249).  The terms of H1 and H2 that are linear in omega and phi
+    // are added in spectral space: H1 = fac_n omega / R1 is rfft'd as fac_n omega_hat / R1,
+    // and in H2 -eps Lap omega and S1 Lap^2 phi - S2 Lap phi + S3 phi become
+    // eps k^2 omega_hat and (S1 k^4 + S2 k^2 + S3) phi_hat (the reference transforms Lap omega
+    // and Lap^2 phi back, then H1 and H2 forward).  G, H3 and the rest of H2 are transformed.
     rfft({job(om, w_omh, N)});
-    irfft({job(w_omh, w_lom, N, MUL_LAP)});
-    // H1, G, H2, H3 (Utils.jl:198-249)
-    R* H1 = w_H;
     R* G = w_H + NP;
-    R* H2 = w_H + 2 * NP;
+    R* H2n = w_H + 2 * NP;  // the part of H2 not linear in omega or phi
     R* H3 = w_H + 3 * NP;
-    const R* lom = w_lom;
     foreach_n(s, NP, [=] __device__(long long k) {
       const int n = (int)(k / Pl);
-      const T ph = phi[k], ps = psi[k], w = om[k], lw = lom[k], lp = lap[k], bh = bih[k];
+      const T ph = phi[k], ps = psi[k], w = om[k];
       const T eps = (T)q.eps;
-      const double fac = q.garea * (dsl->area[n] - A0[n]) * (3.0 * SQ2 / 4.0);
-      H1[k] = ((T)fac * w) / (T)dsl->R1s;
-      const T sb = (T)q.gs * (T)(3.0 * SQ2 / 4.0) * w +
-                   (T)q.gb * (T)(3.0 * SQ2 / (8.0 * q.eps)) * (w * (T(3) * (ph * ph) - T(1)) / eps - eps * lw);
-      const T lin = (T)q.S1 * bh - (T)q.S2 * lp + (T)q.S3 * ph;
-      H2[k] = (sb - lin) / (T)dsl->R2s;
+      const T sb = (T)q.gs * (T)(3.0 * SQ2 / 4.0) * w + (T)q.gb * (T)(3.0 * SQ2 / (8.0 * q.eps)) * (w * (T(3) * (ph * ph) - T(1)) / eps);
+      H2n[k] = sb / (T)dsl->R2s;
       const T pp = T(-1.5) * (ph * ph) + T(1.5);
       const T di = ps - (T)pin[n], dou = ps - (T)pout[n];
       const T fin = T(0.5) * (T)q.gin * (di * di) + (T)q.bin;
@@ -678,6 +673,29 @@ class Solver final : public ISolver {
       const T G2 = (T)q.S4 * ps;
       G[k] = (G1 - G2) / (T)dsl->R3s;
     });
+    rfft({job(G, w_Hh + NH, 3 * N)});
+    C* Hh = w_Hh;
+    const C* omh = w_omh;
+    const C* phh = w_phat;
+    const SpecC sc = spc();
+    const KIdx ki{Nxh, Ny, Nz};
+    const long long Hl = H;
+    foreach_n(s, NH, [=] __device__(long long i) {
+      int ix, iy, iz, f;
+      ki(i, ix, iy, iz, f);
+      const int n = (int)(i / Hl);
+      const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      const CD wv = omh[i], pv = phh[i];
+      const double fac = q.garea * (dsl->area[n] - A0[n]) * (3.0 * SQ2 / 4.0) / dsl->R1s;
+      Hh[i] = CD{fac * wv.x, fac * wv.y};
+      const double cw = q.gb * (3.0 * SQ2 / (8.0 * q.eps)) * q.eps * k2;
+      const double cp = sc.S1 * (k2 * k2) + sc.S2 * k2 + sc.S3;
+      const double r2 = 1.0 / dsl->R2s;
+      const CD h2 = Hh[2 * NH + i];
+      Hh[2 * NH + i] = CD{h2.x + (cw * wv.x - cp * pv.x) * r2, h2.y + (cw * wv.y - cp * pv.y) * r2};
+    });
+    if (phys_H) irfft({job(w_Hh, w_H, 4 * N)});
   }
 
   // ---- spectral operator helpers (device) ------------------------------------
@@ -1560,9 +1578,7 @@ class Solver final : public ISolver {
       max_reduce(M, NP, &ds->Mmax);
       stencil_stale = true;
     }
-    functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));
-    // rfft of H1, G, H2, H3 -> w_Hh (same order)
-    rfft({job(w_H, w_Hh, 4 * N)});
+    functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));  // -> w_Hh
     // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115)
     const C *phhN = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
     const C* Hh = w_Hh;
@@ -2324,7 +2340,6 @@ class Solver final : public ISolver {
     functionals(fphys(ip, F_PHI), fphys(ip, F_PSI), true, sizeof(R) == 8 ? nullptr : fhalf(ip, F_PHI));
     int rc = check_flags();
     if (rc) return rc;
-    rfft({job(w_H, w_Hh, 4 * N)});
     // mu_hat = L_phi phi_hat + R1 H1^ + R2 H2^ + R3 H3^ ; nu_hat = S4 psi_hat + R3 G^ (simulation.jl:35-47)
     const C* Hh = w_Hh;
     C* muh = fhalf(ip, F_MU);
@@ -2702,7 +2717,7 @@ class Solver final : public ISolver {
     put_real(phi, w_phis, NP);
     put_real(psi, w_psis, NP);
     SAV_HIP(hipMemsetAsync(&ds->flags, 0, sizeof(int), s));
-    functionals(w_phis, w_psis, false);
+    functionals(w_phis, w_psis, false, nullptr, nullptr, true);
     int rc = check_flags();
     get_real(H1, w_H, NP);
     get_real(G, w_H + NP, NP);
