@@ -1579,42 +1579,18 @@ class Solver final : public ISolver {
       stencil_stale = true;
     }
     functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));  // -> w_Hh
-    // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115)
-    const C *phhN = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
-    const C* Hh = w_Hh;
-    C* cch = cache;
+    // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115): phi_1j and mu_1j are pointwise in the
+    // state spectra and H_hat and are formed where they are used (Splits); here only the right-hand
+    // side of psi_11, -(b psi_hat^n + c psi_hat^{n-1}) / dt
+    const C *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
     C* rhs = w_rhs;
     const SpecC sc = spc();
     const KIdx ki{Nxh, Ny, Nz};
     const DScal* dsl = ds;
     foreach_n(s, NH, [=] __device__(long long i) {
-      int ix, iy, iz, f;
-      ki(i, ix, iy, iz, f);
-      const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
-      const double k2 = kx * kx + ky * ky + kz * kz;
-      const double lapd = -k2, bihd = k2 * k2;
-      const double Lphi = sc.S1 * bihd - sc.S2 * lapd + sc.S3;
-      const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
-      const T Lp = (T)Lphi, lh = (T)lhs;
       const T b = (T)dsl->b, cc = (T)dsl->c, dt = (T)dsl->dt;
-      const CD xN = phhN[i], xO = phhO[i];
-      CD t = CD{-(b * xN.x + cc * xO.x) / dt, -(b * xN.y + cc * xO.y) / dt};
-      const CD p11 = CD{t.x / lh, t.y / lh};
-      cch[PHI11 * NH + i] = p11;
-      cch[MU11 * NH + i] = CD{Lp * p11.x, Lp * p11.y};
       const CD yN = pshN[i], yO = pshO[i];
       rhs[i] = CD{-(b * yN.x + cc * yO.x) / dt, -(b * yN.y + cc * yO.y) / dt};
-      const T mM = -(T)sc.Mphi;
-      const CD h1 = Hh[i], h2 = Hh[2 * NH + i], h3 = Hh[3 * NH + i];
-      const CD p12 = CD{(mM * h1.x) / lh, (mM * h1.y) / lh};
-      const CD p13 = CD{(mM * h2.x) / lh, (mM * h2.y) / lh};
-      const CD p14 = CD{(mM * h3.x) / lh, (mM * h3.y) / lh};
-      cch[PHI12 * NH + i] = p12;
-      cch[MU12 * NH + i] = CD{Lp * p12.x + h1.x, Lp * p12.y + h1.y};
-      cch[PHI13 * NH + i] = p13;
-      cch[PHI14 * NH + i] = p14;
-      cch[MU13 * NH + i] = CD{Lp * p13.x + h2.x, Lp * p13.y + h2.y};
-      cch[MU14 * NH + i] = CD{Lp * p14.x + h3.x, Lp * p14.y + h3.y};
     });
     // grad G -> M * grad G -> rfft -> rhs12 = sum_d D1_d * (.)   (Solvers.jl:99-105)
     const C* Gh = w_Hh + NH;
@@ -1742,14 +1718,12 @@ class Solver final : public ISolver {
     // and G_hat are rffts of real fields, so sum_x h f = (1/P) sum_k w(kx) Re(conj(h_hat) f_hat)
     // with w = 1 on the kx = 0 and kx = Nx/2 planes and 2 elsewhere, for any f_hat whose c2r is
     // f: the split solutions, and b phi^n + c phi^{n-1} (b psi^n + c psi^{n-1}) from the state
-    // spectra, whose c2r are the states' physical fields.  The same pass writes
-    // nu_11, nu_12, nu_21 = S4 psi (+ G_hat for nu_12) of step 1 (Solvers.jl:92, 107, 134).
+    // spectra, whose c2r are the states' physical fields.
     const C* Hh = w_Hh;  // [H1 | G | H2 | H3]
-    C* cch = cache;
     const C *phN = fhalf(ip, F_PHI), *phO = fhalf(io, F_PHI), *psN = fhalf(ip, F_PSI), *psO = fhalf(io, F_PSI);
     const DScal* dsl = ds;
     const int Nxh_ = Nxh, Nx_ = Nx;
-    const T S4 = (T)c.S4;
+    const Splits sp = splits();
     reduce_n<22>(s, NH, 1,
                  [=] __device__(long long i, int, double(&acc)[22]) {
                    const int ix = (int)(i % Nxh_);
@@ -1758,7 +1732,6 @@ class Solver final : public ISolver {
                    auto re = [&](const CD& a, const CD& f) {
                      return w * ((double)a.x * (double)f.x + (double)a.y * (double)f.y);
                    };
-                   auto at = [&](int slot) { return (CD)cch[(long long)slot * NH + i]; };
                    const T b = (T)dsl->b, cc = (T)dsl->c;
                    const CD xn = phN[i], xo = phO[i], yn = psN[i], yo = psO[i];
                    const CD tph = CD{b * xn.x + cc * xo.x, b * xn.y + cc * xo.y};
@@ -1767,8 +1740,10 @@ class Solver final : public ISolver {
                    acc[1] += re(h2, tph);
                    acc[2] += re(h3, tph);
                    acc[3] += re(gg, tps);
-                   const CD p12 = at(PHI12), p13 = at(PHI13), p14 = at(PHI14), p11 = at(PHI11), p21 = at(PHI21);
-                   const CD s11 = at(PSI11), s12 = at(PSI12), s21 = at(PSI21);
+                   CD ph[4], mu[4];
+                   sp.phi_mu(i, ph, mu);
+                   const CD p11 = ph[0], p12 = ph[1], p13 = ph[2], p14 = ph[3], p21 = sp.at(PHI21, i);
+                   const CD s11 = sp.at(PSI11, i), s12 = sp.at(PSI12, i), s21 = sp.at(PSI21, i);
                    acc[4] += re(h1, p12);
                    acc[5] += re(h1, p13);
                    acc[6] += re(h1, p14);
@@ -1787,9 +1762,6 @@ class Solver final : public ISolver {
                    acc[19] += re(h2, p21);
                    acc[20] += re(h3, p21);
                    acc[21] += re(gg, s21);
-                   cch[NU11 * NH + i] = CD{S4 * s11.x, S4 * s11.y};
-                   cch[NU12 * NH + i] = CD{S4 * s12.x + gg.x, S4 * s12.y + gg.y};
-                   cch[NU21 * NH + i] = CD{S4 * s21.x, S4 * s21.y};
                  },
                  partials, red);
     const PhysC q = pc();
@@ -1884,26 +1856,80 @@ class Solver final : public ISolver {
   // ==========================================================================
   // Step 3 (Solvers.jl:242-259)
   // ==========================================================================
-  // The combinations of step 3 (Solvers.jl:242-259): x_1 = x_11 + R11 x_12 + R21 x_13 + R31 x_14
-  // for phi and mu (x_11 + R31 x_12 for psi and nu) and x_2 alike with x_21, R12, R22, R32.
-  // They are formed where they are used, in step 5's sums and step 6's update, so the eight
-  // combined half spectra are never stored.
-  struct Step3 {
-    const C* k;
+  // Step 1's split solutions where they are used.  phi_1j, mu_1j (j = 11, 12, 13, 14;
+  // Solvers.jl:86-115) are pointwise in the state spectra phi_hat^n, phi_hat^{n-1} and in
+  // H1_hat, H2_hat, H3_hat, and nu_1j = S4 psi_1j (+ G_hat for nu_12; Solvers.jl:92, 107, 134),
+  // so they are formed in registers by the passes that read them (steps 2, 5 and 6) instead of
+  // stored; phi_21, mu_21 (advection) and psi_11, psi_12, psi_21 (the psi solves) are stored in
+  // the cache.  comb() forms the combinations of step 3 (Solvers.jl:242-259):
+  // x_1 = x_11 + R11 x_12 + R21 x_13 + R31 x_14 for phi and mu (x_11 + R31 x_12 for psi and nu)
+  // and x_2 alike with x_21, R12, R22, R32.
+  struct Splits {
+    const C* k;                // cache
+    const C *xN, *xO, *Hh;     // phi_hat^n, phi_hat^{n-1}, [H1 | G | H2 | H3]_hat
+    const double *kx, *ky, *kz;
+    double S1, S2, S3, S4, Mphi;
+    const DScal* ds;
+    KIdx ki;
     long long NH;
     __device__ __forceinline__ CD at(int f, long long i) const { return k[f * NH + i]; }
+    // ph[j], mu[j] = phi_1(j+1), mu_1(j+1)
+    __device__ __forceinline__ void phi_mu(long long i, CD (&ph)[4], CD (&mu)[4]) const {
+      int ix, iy, iz, f;
+      ki(i, ix, iy, iz, f);
+      const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
+      const double k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+      const double lapd = -k2, bihd = k2 * k2;
+      const double Lphi = S1 * bihd - S2 * lapd + S3;
+      const double lhs = (ds->a / ds->dt) + Mphi * Lphi;
+      const T Lp = (T)Lphi, lh = (T)lhs;
+      const T b = (T)ds->b, cc = (T)ds->c, dt = (T)ds->dt;
+      const CD a = xN[i], o = xO[i];
+      const CD t = CD{-(b * a.x + cc * o.x) / dt, -(b * a.y + cc * o.y) / dt};
+      ph[0] = CD{t.x / lh, t.y / lh};
+      mu[0] = CD{Lp * ph[0].x, Lp * ph[0].y};
+      const T mM = -(T)Mphi;
+      const int hs[3] = {0, 2, 3};
+      for (int j = 0; j < 3; ++j) {
+        const CD h = Hh[hs[j] * NH + i];
+        ph[j + 1] = CD{(mM * h.x) / lh, (mM * h.y) / lh};
+        mu[j + 1] = CD{Lp * ph[j + 1].x + h.x, Lp * ph[j + 1].y + h.y};
+      }
+    }
+    __device__ __forceinline__ void psi_nu(long long i, CD (&ps)[3], CD (&nu)[3]) const {  // 11, 12, 21
+      ps[0] = at(PSI11, i);
+      ps[1] = at(PSI12, i);
+      ps[2] = at(PSI21, i);
+      const CD g = Hh[NH + i];
+      const T s4 = (T)S4;
+      nu[0] = CD{s4 * ps[0].x, s4 * ps[0].y};
+      nu[1] = CD{s4 * ps[1].x + g.x, s4 * ps[1].y + g.y};
+      nu[2] = CD{s4 * ps[2].x, s4 * ps[2].y};
+    }
     static __device__ __forceinline__ CD ax(const CD& a, T r, const CD& b) { return CD{a.x + r * b.x, a.y + r * b.y}; }
-    // which: 0 phi, 1 mu, 2 psi, 3 nu; part: 0 -> x_1, 1 -> x_2
-    __device__ __forceinline__ CD comb(int which, int part, long long i, const double* Rs) const {
-      const T Ra = (T)Rs[3 * part + 0], Rb = (T)Rs[3 * part + 1], Rc = (T)Rs[3 * part + 2];
-      switch (which) {
-        case 0: return ax(ax(ax(at(part ? PHI21 : PHI11, i), Ra, at(PHI12, i)), Rb, at(PHI13, i)), Rc, at(PHI14, i));
-        case 1: return ax(ax(ax(at(part ? MU21 : MU11, i), Ra, at(MU12, i)), Rb, at(MU13, i)), Rc, at(MU14, i));
-        case 2: return ax(at(part ? PSI21 : PSI11, i), Rc, at(PSI12, i));
-        default: return ax(at(part ? NU21 : NU11, i), Rc, at(NU12, i));
+    // x1[which], x2[which] for which = 0 phi, 1 mu, 2 psi, 3 nu (phi and psi only with want_phi_psi)
+    __device__ __forceinline__ void comb(long long i, bool want_phi_psi, CD (&x1)[4], CD (&x2)[4]) const {
+      const double* Rs = ds->Rs;
+      CD ph[4], mu[4], ps[3], nu[3];
+      phi_mu(i, ph, mu);
+      psi_nu(i, ps, nu);
+      const CD mu21 = at(MU21, i);
+      for (int part = 0; part < 2; ++part) {
+        const T Ra = (T)Rs[3 * part + 0], Rb = (T)Rs[3 * part + 1], Rc = (T)Rs[3 * part + 2];
+        CD(&o)[4] = part ? x2 : x1;
+        o[1] = ax(ax(ax(part ? mu21 : mu[0], Ra, mu[1]), Rb, mu[2]), Rc, mu[3]);
+        o[3] = ax(part ? nu[2] : nu[0], Rc, nu[1]);
+        if (want_phi_psi) {
+          o[0] = ax(ax(ax(part ? at(PHI21, i) : ph[0], Ra, ph[1]), Rb, ph[2]), Rc, ph[3]);
+          o[2] = ax(part ? ps[2] : ps[0], Rc, ps[1]);
+        }
       }
     }
   };
+  Splits splits() const {
+    return Splits{cache, shalf[ip], shalf[io], w_Hh, d_kx, d_ky, d_kz, c.S1, c.S2, c.S3, c.S4, c.M_phi, ds,
+                  KIdx{Nxh, Ny, Nz}, (long long)N * H};
+  }
 
   // ==========================================================================
   // Step 4 (Solvers.jl:267-337; Solvers_3d.jl:221-292)
@@ -1985,8 +2011,7 @@ class Solver final : public ISolver {
     // -(u* . grad) u* - lamda sum_n (phi* grad mu* + psi* grad nu*) and w_uth the spectra of
     // u~_1 and u~_2, so lamda sum (coupling . u~_j) + sum (convection . u~_j) = -sum (rhs2 . u~_j).
     const C* ah = w_advh;
-    const Step3 k3{cache, NH};
-    const DScal* dsl = ds;
+    const Splits sp = splits();
     const C* r2h = w_rhs2h;
     const C* uth = w_uth;
     const int Nxh_ = Nxh, Nx_ = Nx, ndd = nd;
@@ -1997,10 +2022,12 @@ class Solver final : public ISolver {
                   const double w = (ix == 0 || 2 * ix == Nx_) ? 1.0 : 2.0;
                   auto re = [&](const CD& a, const CD& b) { return w * ((double)a.x * (double)b.x + (double)a.y * (double)b.y); };
                   const CD ap = ah[i], as = ah[NH + i];
-                  acc[0] += re(ap, k3.comb(1, 0, i, dsl->Rs));  // t11: mu_1
-                  acc[1] += re(as, k3.comb(3, 0, i, dsl->Rs));  // t12: nu_1
-                  acc[2] += re(ap, k3.comb(1, 1, i, dsl->Rs));  // t21: mu_2
-                  acc[3] += re(as, k3.comb(3, 1, i, dsl->Rs));  // t22: nu_2
+                  CD x1[4], x2[4];
+                  sp.comb(i, false, x1, x2);
+                  acc[0] += re(ap, x1[1]);  // t11: mu_1
+                  acc[1] += re(as, x1[3]);  // t12: nu_1
+                  acc[2] += re(ap, x2[1]);  // t21: mu_2
+                  acc[3] += re(as, x2[3]);  // t22: nu_2
                   if (i < Hl)
                     for (int d = 0; d < ndd; ++d) {
                       const CD r = r2h[d * Hl + i];
@@ -2037,7 +2064,7 @@ class Solver final : public ISolver {
   // ==========================================================================
   void step67() {
     const long long NH = (long long)N * H;
-    const Step3 k3{cache, NH};
+    const Splits sp = splits();
     C *phN = fhalf(inew, F_PHI), *psN = fhalf(inew, F_PSI), *muN = fhalf(inew, F_MU), *nuN = fhalf(inew, F_NU);
     C* uN = uhalf(inew);
     C* pN = phalf(inew);
@@ -2052,9 +2079,10 @@ class Solver final : public ISolver {
       const T Q = (T)dsl->Q;
       for (int n = 0; n < Nl; ++n) {
         const long long k = (long long)n * Hl + i;
+        CD x1[4], x2[4];
+        sp.comb(k, true, x1, x2);
         auto cmb = [&](int f) {  // x^{n+1} = x_1 + Q x_2 (Solvers.jl:431-453)
-          const CD a = k3.comb(f, 0, k, dsl->Rs), b = k3.comb(f, 1, k, dsl->Rs);
-          return CD{a.x + Q * b.x, a.y + Q * b.y};
+          return CD{x1[f].x + Q * x2[f].x, x1[f].y + Q * x2[f].y};
         };
         phN[k] = cmb(0);
         muN[k] = cmb(1);
