@@ -1518,20 +1518,11 @@ class Solver final : public ISolver {
         adv[NP + i] = b;
       });
       rfft({job(w_adv, w_advh, 2 * N)});
+      // phi_21 = -adv_phi_hat / lhs_phi and mu_21 = L_phi phi_21 are formed where they are used
+      // (Splits); here the right-hand side of psi_21
       const C* ah = w_advh;
       C* rhs21 = w_rhs + 2 * NH;
       foreach_n(s, NH, [=] __device__(long long i) {
-        int ix, iy, iz, f;
-        ki(i, ix, iy, iz, f);
-        const double kx = sc.kx[ix], ky = sc.ky[iy], kz = sc.kz[iz];
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        const double Lphi = sc.S1 * (k2 * k2) - sc.S2 * (-k2) + sc.S3;
-        const double lhs = (dsl->a / dsl->dt) + sc.Mphi * Lphi;
-        const T Lp = (T)Lphi, lh = (T)lhs;
-        const CD t = ah[i];
-        const CD p21 = CD{(-t.x) / lh, (-t.y) / lh};
-        cch[PHI21 * NH + i] = p21;
-        cch[MU21 * NH + i] = CD{Lp * p21.x, Lp * p21.y};
         const CD t2 = ah[NH + i];
         rhs21[i] = CD{-t2.x, -t2.y};
       });
@@ -1740,9 +1731,9 @@ class Solver final : public ISolver {
                    acc[1] += re(h2, tph);
                    acc[2] += re(h3, tph);
                    acc[3] += re(gg, tps);
-                   CD ph[4], mu[4];
+                   CD ph[5], mu[5];
                    sp.phi_mu(i, ph, mu);
-                   const CD p11 = ph[0], p12 = ph[1], p13 = ph[2], p14 = ph[3], p21 = sp.at(PHI21, i);
+                   const CD p11 = ph[0], p12 = ph[1], p13 = ph[2], p14 = ph[3], p21 = ph[4];
                    const CD s11 = sp.at(PSI11, i), s12 = sp.at(PSI12, i), s21 = sp.at(PSI21, i);
                    acc[4] += re(h1, p12);
                    acc[5] += re(h1, p13);
@@ -1860,21 +1851,23 @@ class Solver final : public ISolver {
   // Solvers.jl:86-115) are pointwise in the state spectra phi_hat^n, phi_hat^{n-1} and in
   // H1_hat, H2_hat, H3_hat, and nu_1j = S4 psi_1j (+ G_hat for nu_12; Solvers.jl:92, 107, 134),
   // so they are formed in registers by the passes that read them (steps 2, 5 and 6) instead of
-  // stored; phi_21, mu_21 (advection) and psi_11, psi_12, psi_21 (the psi solves) are stored in
-  // the cache.  comb() forms the combinations of step 3 (Solvers.jl:242-259):
-  // x_1 = x_11 + R11 x_12 + R21 x_13 + R31 x_14 for phi and mu (x_11 + R31 x_12 for psi and nu)
-  // and x_2 alike with x_21, R12, R22, R32.
+  // stored, and so are phi_21 = -rfft(u* . grad phi*) / lhs_phi and mu_21 = L_phi phi_21
+  // (Solvers.jl:117-131); psi_11, psi_12, psi_21 (the psi solves) are stored in the cache.
+  // comb() forms the combinations of step 3 (Solvers.jl:242-259): x_1 = x_11 + R11 x_12 +
+  // R21 x_13 + R31 x_14 for phi and mu (x_11 + R31 x_12 for psi and nu) and x_2 alike with
+  // x_21, R12, R22, R32.
   struct Splits {
     const C* k;                // cache
     const C *xN, *xO, *Hh;     // phi_hat^n, phi_hat^{n-1}, [H1 | G | H2 | H3]_hat
+    const C* ah;               // rfft(u* . grad phi*) (step 1's advection term)
     const double *kx, *ky, *kz;
     double S1, S2, S3, S4, Mphi;
     const DScal* ds;
     KIdx ki;
     long long NH;
     __device__ __forceinline__ CD at(int f, long long i) const { return k[f * NH + i]; }
-    // ph[j], mu[j] = phi_1(j+1), mu_1(j+1)
-    __device__ __forceinline__ void phi_mu(long long i, CD (&ph)[4], CD (&mu)[4]) const {
+    // ph[j], mu[j] = phi_1(j+1), mu_1(j+1) for j < 4; ph[4], mu[4] = phi_21, mu_21
+    __device__ __forceinline__ void phi_mu(long long i, CD (&ph)[5], CD (&mu)[5]) const {
       int ix, iy, iz, f;
       ki(i, ix, iy, iz, f);
       const double kxv = kx[ix], kyv = ky[iy], kzv = kz[iz];
@@ -1895,6 +1888,9 @@ class Solver final : public ISolver {
         ph[j + 1] = CD{(mM * h.x) / lh, (mM * h.y) / lh};
         mu[j + 1] = CD{Lp * ph[j + 1].x + h.x, Lp * ph[j + 1].y + h.y};
       }
+      const CD t2 = ah[i];  // Solvers.jl:117-131
+      ph[4] = CD{(-t2.x) / lh, (-t2.y) / lh};
+      mu[4] = CD{Lp * ph[4].x, Lp * ph[4].y};
     }
     __device__ __forceinline__ void psi_nu(long long i, CD (&ps)[3], CD (&nu)[3]) const {  // 11, 12, 21
       ps[0] = at(PSI11, i);
@@ -1910,24 +1906,23 @@ class Solver final : public ISolver {
     // x1[which], x2[which] for which = 0 phi, 1 mu, 2 psi, 3 nu (phi and psi only with want_phi_psi)
     __device__ __forceinline__ void comb(long long i, bool want_phi_psi, CD (&x1)[4], CD (&x2)[4]) const {
       const double* Rs = ds->Rs;
-      CD ph[4], mu[4], ps[3], nu[3];
+      CD ph[5], mu[5], ps[3], nu[3];
       phi_mu(i, ph, mu);
       psi_nu(i, ps, nu);
-      const CD mu21 = at(MU21, i);
       for (int part = 0; part < 2; ++part) {
         const T Ra = (T)Rs[3 * part + 0], Rb = (T)Rs[3 * part + 1], Rc = (T)Rs[3 * part + 2];
         CD(&o)[4] = part ? x2 : x1;
-        o[1] = ax(ax(ax(part ? mu21 : mu[0], Ra, mu[1]), Rb, mu[2]), Rc, mu[3]);
+        o[1] = ax(ax(ax(part ? mu[4] : mu[0], Ra, mu[1]), Rb, mu[2]), Rc, mu[3]);
         o[3] = ax(part ? nu[2] : nu[0], Rc, nu[1]);
         if (want_phi_psi) {
-          o[0] = ax(ax(ax(part ? at(PHI21, i) : ph[0], Ra, ph[1]), Rb, ph[2]), Rc, ph[3]);
+          o[0] = ax(ax(ax(part ? ph[4] : ph[0], Ra, ph[1]), Rb, ph[2]), Rc, ph[3]);
           o[2] = ax(part ? ps[2] : ps[0], Rc, ps[1]);
         }
       }
     }
   };
   Splits splits() const {
-    return Splits{cache, shalf[ip], shalf[io], w_Hh, d_kx, d_ky, d_kz, c.S1, c.S2, c.S3, c.S4, c.M_phi, ds,
+    return Splits{cache, shalf[ip], shalf[io], w_Hh, w_advh, d_kx, d_ky, d_kz, c.S1, c.S2, c.S3, c.S4, c.M_phi, ds,
                   KIdx{Nxh, Ny, Nz}, (long long)N * H};
   }
 
