@@ -138,7 +138,7 @@ class Solver final : public ISolver {
   // single-workgroup finishes and launch gaps); their transforms use their own scratch
   // (tmp4, w_grad).  The psi solves wait for the advection terms, step 5 for step 4.
   hipStream_t s4 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_adv = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_gphi = nullptr, ev_adv = nullptr, ev_join = nullptr;
   bool fork_step4 = false, fork_enabled = true;
   void set_step_fork(bool on) override { fork_enabled = on; }
   // The second stream and its scratch are made at the first step that forks: HIP maps
@@ -149,12 +149,13 @@ class Solver final : public ISolver {
     if (s4) return;
     SAV_HIP(hipStreamCreateWithFlags(&s4, hipStreamNonBlocking));
     SAV_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    SAV_HIP(hipEventCreateWithFlags(&ev_gphi, hipEventDisableTiming));
     SAV_HIP(hipEventCreateWithFlags(&ev_adv, hipEventDisableTiming));
     SAV_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    // step 4's transforms: nd^2 velocity derivatives + 2 nd N derivatives of mu, nu, and
-    // the 2 nd velocity predictors; the advection terms: 2 nd N gradients
-    tmp4 = dalloc<C>((long long)std::max(nd * nd + 2 * nd * N, 2 * nd) * H);
-    w_grad = dalloc<R>(2LL * nd * N * P);
+    // step 4's transforms: nd^2 velocity derivatives + 2 nd N derivatives of mu, nu; the
+    // advection terms: nd N gradients of psi*
+    tmp4 = dalloc<C>((long long)(nd * nd + 2 * nd * N) * H);
+    w_grad = dalloc<R>((long long)nd * N * P);
     SAV_HIP(hipStreamSynchronize(s));
   }
   FftGeom<R> g{};
@@ -173,7 +174,7 @@ class Solver final : public ISolver {
   double* partials = nullptr;
   double* red = nullptr;  // reduction outputs (512 doubles)
   // work
-  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_om, *w_H, *w_w, *w_adv, *w_du, *w_dmn,
+  R *w_phis, *w_psis, *w_M, *w_us, *w_d4, *w_gphi, *w_om, *w_H, *w_w, *w_adv, *w_du, *w_dmn,
       *w_rhs2;
   C *w_phat, *w_omh, *w_Hh, *cache, *w_wh, *w_rhs, *w_advh, *w_rhs2h, *w_uth, *tmp;
   // scratch of the forked branch (made by ensure_fork)
@@ -306,6 +307,7 @@ class Solver final : public ISolver {
     w_M = dalloc<R>(NP);
     w_us = dalloc<R>(nd * P);
     w_d4 = dalloc<R>(6 * NP);
+    w_gphi = dalloc<R>((long long)nd * NP);
     w_om = dalloc<R>(NP);
     w_H = dalloc<R>(4 * NP);
     w_w = dalloc<R>(3 * NP);
@@ -427,6 +429,7 @@ class Solver final : public ISolver {
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
     if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_gphi) hipEventDestroy(ev_gphi);
     if (ev_adv) hipEventDestroy(ev_adv);
     if (ev_join) hipEventDestroy(ev_join);
     if (s4) {
@@ -545,29 +548,38 @@ class Solver final : public ISolver {
   // (Utils.jl:36-249).  When set_A0 is true (SAV initialisation) A0 := area first, as
   // simulation.jl:25-26 does.
   // =========================================================================
-  void functionals(const R* phi, const R* psi, bool set_A0, const C* spec = nullptr, const C* spec_old = nullptr,
-                   bool phys_H = false) {
-    const long long NP = (long long)N * P, NH = (long long)N * H;
-    const PhysC q = pc();
-    const int ndl = nd;
-    // derivatives gx, gy, lap of phi from rfft(phi) (Utils.jl:37, 149, 167, 209).  In a
+  // grad phi (nd components -> w_gphi) and Lap phi (-> w_d4 slot 2) from rfft(phi)
+  // (Utils.jl:37, 149, 167, 209; the functionals use the x and y components, step 1's
+  // advection term all of them).  In a
     // step (`spec` given) phi = 2 phi^n - phi^{n-1} is the extrapolation of two states whose
     // physical fields are the c2r of their spectra, so rfft(phi) is the part of
     // 2 phi_hat^n - phi_hat^{n-1} that a c2r keeps (its Hermitian part on the kx = 0 and
     // kx = Nx/2 planes), formed here in one pass instead of a transform.  In FP32 this also
     // keeps the FFT roundoff floor of rfft(phi) (~1e-7 of the largest coefficient at every k)
     // out of the S1 k^4 biharmonic term.
+  void functionals_grad(const R* phi, const C* spec, const C* spec_old) {
+    const long long NP = (long long)N * P;
     if (spec) {
       hermitian_part(spec, spec_old, w_phat, N);
     } else {
       rfft({job(phi, w_phat, N)});
     }
-    const C* src = w_phat;
-    const C* src2 = nullptr;
-    R* gx = w_d4;
-    R* gy = w_d4 + NP;
-    R* lap = w_d4 + 2 * NP;
-    irfft({job(src, gx, N, MUL_DX, src2), job(src, gy, N, MUL_DY, src2), job(src, lap, N, MUL_LAP, src2)});
+    std::vector<SpecJob> jl;
+    for (int d = 0; d < nd; ++d) jl.push_back(job(w_phat, w_gphi + d * NP, N, DX(d)));
+    jl.push_back(job(w_phat, w_d4 + 2 * NP, N, MUL_LAP));
+    irfft_v(jl);
+  }
+
+  // grads_ready: functionals_grad has run for this phi
+  void functionals(const R* phi, const R* psi, bool set_A0, const C* spec = nullptr, const C* spec_old = nullptr,
+                   bool phys_H = false, bool grads_ready = false) {
+    const long long NP = (long long)N * P, NH = (long long)N * H;
+    const PhysC q = pc();
+    const int ndl = nd;
+    if (!grads_ready) functionals_grad(phi, spec, spec_old);
+    const R* gx = w_gphi;
+    const R* gy = w_gphi + NP;
+    const R* lap = w_d4 + 2 * NP;
     // per-vesicle sums: f_surf, f_bend, E2 integrand, f_osm, psi^2 ; omega -> w_om
     R* om = w_om;
     const double* pin = d_psi_in;
@@ -1489,30 +1501,34 @@ class Solver final : public ISolver {
   // 2 X^n - X^{n-1}.  Reads only the state and u* (step 1's extrapolation); writes w_adv,
   // w_advh, the 21 slots of the cache and the third psi right-hand side.  `grad` holds the
   // 2 nd N gradient fields (w_d4 in the serial order, where it is free at this point).
-  void step1_advection(R* grad) {
+  // grad psi* (nd N-fields) into gpsi
+  void step1_advection_grad(R* gpsi) {
+    const long long NP = (long long)N * P;
+    const C *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
+    std::vector<SpecJob> jl;
+    for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, gpsi + d * NP, N, DX(d), pshO));
+    irfft_v(jl);
+  }
+  // The advection terms from grad psi* (gpsi) and grad phi* (w_gphi, computed by
+  // functionals_grad from the Hermitian part of 2 phi_hat^n - phi_hat^{n-1}: the reference
+  // differentiates 2 phi_hat^n - phi_hat^{n-1} itself, whose c2r is the same field up to
+  // round-off, since phi_hat is the rfft of a real field).
+  void step1_advection(const R* gpsi) {
     const long long NP = (long long)N * P, NH = (long long)N * H;
-    const C *phhN = fhalf(ip, F_PHI), *phhO = fhalf(io, F_PHI), *pshN = fhalf(ip, F_PSI), *pshO = fhalf(io, F_PSI);
     const R* us = w_us;
-    C* cch = cache;
-    const SpecC sc = spc();
-    const KIdx ki{Nxh, Ny, Nz};
-    const DScal* dsl = ds;
     {
-      std::vector<SpecJob> jl;
-      for (int d = 0; d < nd; ++d) jl.push_back(job(phhN, grad + d * NP, N, DX(d), phhO));
-      for (int d = 0; d < nd; ++d) jl.push_back(job(pshN, grad + (nd + d) * NP, N, DX(d), pshO));
-      irfft_v(jl);
       R* adv = w_adv;
-      const R* gr = grad;
+      const R* gph = w_gphi;
+      const R* gps = gpsi;
       const int ndd = nd;
       const long long Pl = P;
       foreach_n(s, NP, [=] __device__(long long i) {
         const long long sp = i % Pl;
-        R a = us[sp] * gr[i];
-        R b = us[sp] * gr[ndd * NP + i];
+        R a = us[sp] * gph[i];
+        R b = us[sp] * gps[i];
         for (int d = 1; d < ndd; ++d) {
-          a = a + us[d * Pl + sp] * gr[d * NP + i];
-          b = b + us[d * Pl + sp] * gr[(ndd + d) * NP + i];
+          a = a + us[d * Pl + sp] * gph[d * NP + i];
+          b = b + us[d * Pl + sp] * gps[d * NP + i];
         }
         adv[i] = a;
         adv[NP + i] = b;
@@ -1546,15 +1562,12 @@ class Solver final : public ISolver {
       M[i] = T(0.1) * (T(1.0) - (T)M0 * (t * t));
     });
     foreach_n(s, (long long)nd * P, [=] __device__(long long i) { us[i] = T(2.0) * uN[i] - uO[i]; });
-    if (fork_step4) {
-      SAV_HIP(hipEventRecord(ev_fork, s));
-      SAV_HIP(hipStreamWaitEvent(s4, ev_fork, 0));
+    // on the second stream: grad psi*, then (once grad phi* is there) the advection terms and step 4
+    auto on_s4 = [&](auto&& f) {
       std::swap(s, s4);
       std::swap(tmp, tmp4);
       try {
-        step1_advection(w_grad);
-        SAV_HIP(hipEventRecord(ev_adv, s));
-        step4();
+        f();
       } catch (...) {
         std::swap(s, s4);
         std::swap(tmp, tmp4);
@@ -1562,14 +1575,29 @@ class Solver final : public ISolver {
       }
       std::swap(s, s4);
       std::swap(tmp, tmp4);
-      SAV_HIP(hipEventRecord(ev_join, s4));
+    };
+    if (fork_step4) {
+      SAV_HIP(hipEventRecord(ev_fork, s));
+      SAV_HIP(hipStreamWaitEvent(s4, ev_fork, 0));
+      on_s4([&] { step1_advection_grad(w_grad); });
     }
     if (nd == 2) {
       // M_max = maximum(M_psi) (Solvers.jl:27)
       max_reduce(M, NP, &ds->Mmax);
       stencil_stale = true;
     }
-    functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI));  // -> w_Hh
+    functionals_grad(phs, fhalf(ip, F_PHI), fhalf(io, F_PHI));
+    if (fork_step4) {
+      SAV_HIP(hipEventRecord(ev_gphi, s));
+      SAV_HIP(hipStreamWaitEvent(s4, ev_gphi, 0));
+      on_s4([&] {
+        step1_advection(w_grad);
+        SAV_HIP(hipEventRecord(ev_adv, s));
+        step4();
+      });
+      SAV_HIP(hipEventRecord(ev_join, s4));
+    }
+    functionals(phs, pss, false, fhalf(ip, F_PHI), fhalf(io, F_PHI), false, true);  // -> w_Hh
     // 11, 12, 13, 14 spectral divides (Solvers.jl:86-115): phi_1j and mu_1j are pointwise in the
     // state spectra and H_hat and are formed where they are used (Splits); here only the right-hand
     // side of psi_11, -(b psi_hat^n + c psi_hat^{n-1}) / dt
@@ -1653,10 +1681,12 @@ class Solver final : public ISolver {
       });
     }
     // 21: u* . grad(phi*), u* . grad(psi*) (forked with step 4 in an eager step)
-    if (fork_step4)
+    if (fork_step4) {
       SAV_HIP(hipStreamWaitEvent(s, ev_adv, 0));
-    else
-      step1_advection(w_d4);
+    } else {
+      step1_advection_grad(w_d4 + (long long)nd * NP);
+      step1_advection(w_d4 + (long long)nd * NP);
+    }
     // psi solves
     if (nd == 2) {
       if (!use_learned && !collecting) {
