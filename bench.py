@@ -210,8 +210,9 @@ def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False, 
         # psi iteration count
         sb = 4 if precision == jv.SAV_PREC_FP32 else 8
         B = sum(algorithmic_bytes(w.conf, sb, float(k) / steps) * steps for w, k in zip(wls, kit))
+        Bb = sum(algorithmic_bytes_built(w.conf, sb, float(k) / steps) * steps for w, k in zip(wls, kit))
         res[th] = dict(seconds=el, steps_per_s=len(wls) * steps / el, rc=int(rc), psi_iters=int(kit.sum()),
-                       algorithmic_tb_per_s=B / el / 1e12)
+                       algorithmic_tb_per_s=B / el / 1e12, built_kernels_tb_per_s=Bb / el / 1e12)
     # size classes: each step's psi solves batched over the members of equal grid and N
     # (one host thread per member); one warm-up step tunes the groups' matvecs
     if not batched:
@@ -229,8 +230,10 @@ def run_ensemble(jv, steps, precision, threads_list=(1, 4, 8, 16), graph=False, 
     gof, _ = ens.batched_info()
     sb = 4 if precision == jv.SAV_PREC_FP32 else 8
     B = sum(algorithmic_bytes(w.conf, sb, float(k) / steps) * steps for w, k in zip(wls, kit))
+    Bb = sum(algorithmic_bytes_built(w.conf, sb, float(k) / steps) * steps for w, k in zip(wls, kit))
     batched = dict(seconds=el, steps_per_s=len(wls) * steps / el, rc=int(rc), psi_iters=int(kit.sum()),
-                   algorithmic_tb_per_s=B / el / 1e12, groups=int(groups), grouped_members=int((gof >= 0).sum()))
+                   algorithmic_tb_per_s=B / el / 1e12, built_kernels_tb_per_s=Bb / el / 1e12, groups=int(groups),
+                   grouped_members=int((gof >= 0).sum()))
     ens.close()
     best = max(res, key=lambda k: res[k]["steps_per_s"])
     sizes = sorted({w.conf.Nx for w in wls})
@@ -275,16 +278,27 @@ def algorithmic_bytes(conf, s, k):
 
 
 def algorithmic_bytes_built(conf, s, k):
-    """The same model with the ψ iterations counted by the kernels this library launches,
-    per system-iteration and real N-field u: FP64 72u (k_mv_cols_inv with the s / p update
-    on load 10u + 12u, two k_mv_rows 18u, k_mv_cols_fwd 10u + 8u, k_kry_upd2 14u), FP32 76u
-    (separate updates: column inverse 6u + 6u, rows 18u, column forward 10u + 8u, s update 6u,
-    k_kry_upd2 14u, p update 8u), in place of the survey's 88u."""
+    """HBM bytes of one SAV step as this library's kernels move them (u = one real N-field,
+    v = one scalar field, a half spectrum counted as u or v; a transform of one field is two
+    passes that each read and write it, 4u, plus u when it extrapolates 2 X - Y on load).
+    Outside the psi iterations, kernel by kernel:
+    * 2D: step 1 148u + 8v (extrapolations 7u, M_max u, Hermitian part of phi* 3u, grad phi* and
+      Lap phi* 12u, grad psi* 10u, functionals 6u, rfft omega 4u, G/H2/H3 6u + rfft 12u, spectral
+      H1/H2 5u, advection 6u + 2v + rfft 8u, right-hand sides 5u, M grad G 24u, full right-hand
+      sides 9u, Krylov set-up 24u, solution read-out 6u; velocity extrapolation 6v), step 2 12u,
+      step 4 26u + 47v, step 5 11u + 6v, steps 6-7 14u + 8v, back-transform 8u + 8v, monitor
+      6u + 5v: 225u + 82v;
+    * 3D: step 1 103u + 12v, step 2 12u, step 4 38u + 88v, step 5 11u + 9v, steps 6-7 14u + 11v,
+      back-transform 8u + 12v, monitor 6u + 7v: 192u + 139v;
+    and per psi system-iteration FP64 72u (k_mv_cols_inv with the s / p update on load 10u + 12u,
+    two k_mv_rows 18u, k_mv_cols_fwd 10u + 8u, k_kry_upd2 14u), FP32 76u (separate updates:
+    column inverse 6u + 6u, rows 18u, column forward 10u + 8u, s update 6u, k_kry_upd2 14u,
+    p update 8u), in place of the survey's 88u."""
     pts = conf.Nx * conf.Ny * (conf.Nz if conf.ndim == 3 else 1)
     u, v = s * pts * conf.N, s * pts
     if conf.ndim == 3:
-        return 280 * u + 144 * v
-    return (282 + (72 if s == 8 else 76) * k) * u + 94 * v
+        return 192 * u + 139 * v
+    return (225 + (72 if s == 8 else 76) * k) * u + 82 * v
 
 
 def cpu_report(jv, which="all", steps=20, warm=2):
