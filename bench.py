@@ -648,6 +648,8 @@ def main():
                                     "speedup_vs_sequential": round(ens["speedup_vs_sequential"], 3),
                                     "algorithmic_tb_per_s_by_threads": {
                                         k: round(v["algorithmic_tb_per_s"], 3) for k, v in ens["by_threads"].items()},
+                                    "built_kernels_tb_per_s_by_threads": {
+                                        k: round(v["built_kernels_tb_per_s"], 3) for k, v in ens["by_threads"].items()},
                                     "byte_share_by_grid": ens["byte_share_by_grid"],
                                     "graph_replay_by_threads": {k: round(v["steps_per_s"], 2)
                                                                 for k, v in ensg["by_threads"].items()}}
@@ -655,6 +657,7 @@ def main():
         line["ensemble_64_fp64"]["batched_size_classes"] = {
             "steps_per_s": round(bt["steps_per_s"], 2), "groups": bt["groups"],
             "grouped_members": bt["grouped_members"], "algorithmic_tb_per_s": round(bt["algorithmic_tb_per_s"], 3),
+            "built_kernels_tb_per_s": round(bt["built_kernels_tb_per_s"], 3),
             "speedup_vs_best_threads": round(bt["steps_per_s"] / ens["by_threads"][ens["best_threads"]]["steps_per_s"], 3)}
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(jv, wl_head if "2d" in out else jv.workload_c4(n=128, steps=1),
