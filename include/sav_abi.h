@@ -109,6 +109,11 @@ const char* sav_last_error(const sav_handle_t* h);
  * Copies a FieldState in or out (src/Types.jl:167-201).  which = SAV_PRESENT or SAV_OLD.
  * scalars = {Q, R1, R2, R3}; A0 has N entries (present only; ignored for OLD); dt is
  * FieldState.dt.  Any array pointer may be NULL on get (skipped); on set all are required.
+ * As in the reference's loop (simulation.jl:99-104), the physical fields of a state set here
+ * are expected to be the back-transforms of its spectra (up to round-off, e.g. phi_hat =
+ * rfft(phi) of an initial condition): a step takes its inner products and the functionals'
+ * derivatives from the spectra.  mu, nu and p of a stepped state are back-transformed when
+ * it is read.
  */
 int sav_set_state(sav_handle_t* h, int which, const double* phi, const double* phi_hat, const double* psi,
                   const double* psi_hat, const double* u, const double* u_hat, const double* p,
