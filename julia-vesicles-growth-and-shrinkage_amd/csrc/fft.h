@@ -528,6 +528,8 @@ struct SpecJob {
   int count;         // number of scalar fields in this job
   double scale;      // extra real factor applied on load
   long long toff;    // offset (complex elements) of this job inside the irfft scratch
+  int pmul;          // inverse 3D plane path: multiplier constant along z (MUL_DX / MUL_DY)
+                     // applied by the plane pass to a z pass it may share with other jobs
 };
 
 struct SpecJobs {
@@ -549,6 +551,7 @@ struct FftGeom {
   int colw_fwd = 0;       // the same for the forward column transforms
   const cplx<R>* tw_hp = nullptr;  // hplane_twiddles() for 128 x 128 planes of 3D fields
   int plane_kind = 0;     // 128 x 128 planes: 0 = register-column kernels, 1 = whole plane in LDS
+  double dkx = 0.0, dky = 0.0;  // wavenumber spacings 2 pi / Lx, 2 pi / Ly
 };
 
 // Columns per workgroup that width cw (0, 4, 8, 16) resolves to for a column transform of

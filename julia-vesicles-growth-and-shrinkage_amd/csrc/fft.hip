@@ -592,8 +592,14 @@ __device__ __forceinline__ bool hplane_locate(const SpecJobs& jobs, int nz, long
 
 // c2r column loads v[M0 .. M0 + 7] of column x (rows y = t + 8m) of the half-spectrum plane
 // at src; column 0 takes c0 + i c64 from the Hermitian parts (c(y) + conj c(-y)) / 2.
+// With pm = MUL_DX / MUL_DY the plane is that of a derivative, i kx X or i ky X, whose z pass
+// is the plain field's (kx = x dkx, ky = (y < 64 ? y : y - 128) dky): the other columns are
+// multiplied after the loads (hplane_c2r_mul); on the packed column, i ky commutes with the
+// Hermitian parts except at the Nyquist row y = 64, set here to -ky(64) (Im c0 + i Im c64),
+// and i kx (0 at x = 0, kN at x = 64) leaves -kN times the anti-Hermitian part of c64.
 template <typename R, int M0>
-__device__ __forceinline__ void hplane_c2r_load(const cplx<R>* __restrict__ src, int x, int t, cplx<R> (&v)[16]) {
+__device__ __forceinline__ void hplane_c2r_load(const cplx<R>* __restrict__ src, int x, int t, cplx<R> (&v)[16], int pm,
+                                                double dkx, double dky) {
   if (x != 0) {
 #pragma unroll
     for (int mm = 0; mm < 8; ++mm) v[M0 + mm] = src[x + 65 * (t + 8 * (M0 + mm))];
@@ -603,8 +609,28 @@ __device__ __forceinline__ void hplane_c2r_load(const cplx<R>* __restrict__ src,
       const int y = t + 8 * (M0 + mm), ym = (128 - y) & 127;
       const cplx<R> a = src[65 * y], am = src[65 * ym];
       const cplx<R> b = src[64 + 65 * y], bm = src[64 + 65 * ym];
-      v[M0 + mm] = cplx<R>{R(0.5) * (a.x + am.x) - R(0.5) * (b.y - bm.y), R(0.5) * (a.y - am.y) + R(0.5) * (b.x + bm.x)};
+      if (pm == MUL_DX) {
+        const R kn = (R)(dkx * 64.0);
+        v[M0 + mm] = cplx<R>{-kn * (R(0.5) * (b.x - bm.x)), -kn * (R(0.5) * (b.y + bm.y))};
+      } else if (pm == MUL_DY && y == 64) {
+        const R kq = (R)(dky * -64.0);
+        v[M0 + mm] = cplx<R>{-kq * a.y, -kq * b.y};
+      } else {
+        v[M0 + mm] = cplx<R>{R(0.5) * (a.x + am.x) - R(0.5) * (b.y - bm.y), R(0.5) * (a.y - am.y) + R(0.5) * (b.x + bm.x)};
+      }
     }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void hplane_c2r_mul(cplx<R> (&v)[16], int x, int t, int pm, double dkx, double dky) {
+  if (pm == MUL_NONE || (x == 0 && pm == MUL_DX)) return;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int y = t + 8 * e;
+    if (x == 0 && y == 64) continue;  // set by the load
+    const R k = (R)(pm == MUL_DX ? dkx * (double)x : dky * (double)(y < 64 ? y : y - 128));
+    v[e] = cplx<R>{-k * v[e].y, k * v[e].x};
   }
 }
 
@@ -617,7 +643,8 @@ __device__ __forceinline__ void hplane_c2r_load(const cplx<R>* __restrict__ src,
 template <typename R>
 __global__ __launch_bounds__(kHPlaneNT, 4) void k_hplane_c2r(SpecJobs jobs, int nz, long long total,
                                                              const cplx<R>* __restrict__ tmp, double scale,
-                                                             const cplx<R>* __restrict__ twp) {
+                                                             const cplx<R>* __restrict__ twp, double dkx,
+                                                             double dky) {
   __shared__ cplx<R> lds[64 * 64];
   __shared__ cplx<R> tws[kHPlaneTw];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -635,8 +662,10 @@ __global__ __launch_bounds__(kHPlaneNT, 4) void k_hplane_c2r(SpecJobs jobs, int 
     const cplx<R>* src = tmp + jobs.j[j].toff + plane * (65LL * 128);
     R* dst = reinterpret_cast<R*>(jobs.j[j].dst) + plane * (128LL * 128);
     cplx<R> v[16];
-    hplane_c2r_load<R, 0>(src, x, t, v);
-    hplane_c2r_load<R, 8>(src, x, t, v);
+    const int pm = jobs.j[j].pmul;
+    hplane_c2r_load<R, 0>(src, x, t, v, pm, dkx, dky);
+    hplane_c2r_load<R, 8>(src, x, t, v, pm, dkx, dky);
+    hplane_c2r_mul<R>(v, x, t, pm, dkx, dky);
     __syncthreads();  // twiddles
     bfly<R, true, 16>(v);
     cplx<R> r0[8], r1[8];
@@ -843,7 +872,8 @@ static void launch_plane(const FftGeom<R>& g, const SpecJobs& jobs, bool inverse
       for (int i = 0; i < jobs.n; ++i) total += (long long)jobs.j[i].count * g.Nz;
       if (total == 0) return;
       const dim3 pgrid((unsigned)total);
-      hipLaunchKernelGGL((k_hplane_c2r<R>), pgrid, dim3(kHPlaneNT), 0, s, jobs, g.Nz, total, tmp, scale, g.tw_hp);
+      hipLaunchKernelGGL((k_hplane_c2r<R>), pgrid, dim3(kHPlaneNT), 0, s, jobs, g.Nz, total, tmp, scale, g.tw_hp,
+                         g.dkx, g.dky);
     } else
       hipLaunchKernelGGL((k_hplane_r2c<R>), grid, dim3(kHPlaneNT), 0, s, jobs, g.tw_hp);
     SAV_LAUNCH_CHECK();
@@ -899,8 +929,44 @@ void irfft_jobs(const FftGeom<R>& g, const SpecJobs& jobs_in, cplx<R>* tmp, hipS
   SpecJobs jobs = jobs_in;
   SpecJobs tj = jobs_in;
   long long off = 0;
+  if (g.ndim == 3 && plane_path(g) && g.Nx == 128 && g.tw_hp && g.plane_kind == 0) {
+    // i kx and i ky are constant along z: such a job's z pass is the plain field's, and jobs
+    // on the same source share it; the plane pass applies the multiplier (pmul)
+    tj.n = 0;
+    for (int i = 0; i < jobs.n; ++i) {
+      SpecJob& jb = jobs.j[i];
+      const bool zc = jb.mul == MUL_NONE || jb.mul == MUL_DX || jb.mul == MUL_DY;
+      jb.pmul = zc ? jb.mul : MUL_NONE;
+      int share = -1;
+      if (zc)
+        for (int k = 0; k < i; ++k) {
+          const SpecJob& o = jobs.j[k];
+          if ((o.mul == MUL_NONE || o.mul == MUL_DX || o.mul == MUL_DY) && o.src == jb.src && o.src2 == jb.src2 &&
+              o.count == jb.count && o.scale == jb.scale) {
+            share = k;
+            break;
+          }
+        }
+      if (share >= 0) {
+        jb.toff = jobs.j[share].toff;
+        continue;
+      }
+      jb.toff = off;
+      SpecJob z = jb;
+      z.mul = zc ? MUL_NONE : jb.mul;
+      z.dst = tmp + off;
+      tj.j[tj.n++] = z;
+      off += nh * jb.count;
+    }
+    ColGeom cz = geom_z(g);
+    cz.use_src = 1;
+    launch_cols<R, true>(g.Nz, tj, cz, g.tw_z, g.k, s, g.colw);
+    launch_plane<R>(g, jobs, true, tmp, 1.0 / ((double)g.Nx * g.Ny * g.Nz), s);
+    return;
+  }
   for (int i = 0; i < jobs.n; ++i) {
     jobs.j[i].toff = off;
+    jobs.j[i].pmul = MUL_NONE;
     tj.j[i].dst = tmp + off;
     off += nh * jobs.j[i].count;
   }

@@ -286,6 +286,8 @@ class Solver final : public ISolver {
     g.tw_z = nd == 3 ? upload(column_twiddles<R>(Nz)) : g.tw_y;
     if (nd == 3 && Nx == 128 && Ny == 128) g.tw_hp = upload(hplane_twiddles<R>());
     g.k = KTables{d_kx, d_ky, d_kz, d_kx0};
+    g.dkx = 2.0 * M_PI / c.Lx;
+    g.dky = 2.0 * M_PI / c.Ly;
     // states
     const long long nphys = (4LL * N + nd + 1) * P, nhalf = (4LL * N + nd + 1) * H;
     for (int b = 0; b < 3; ++b) {
