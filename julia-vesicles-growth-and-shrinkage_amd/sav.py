@@ -8,9 +8,8 @@ Names follow the reference: ``Config`` (src/Types.jl:27), ``FieldState``
 order, so ``state.phi.shape == (Nx, Ny, N)`` and the memory handed to the C ABI
 is byte-identical to what Julia's ``ccall`` would pass.
 
-Everything numeric in a time step runs in the HIP library; this module marshals arrays and
-prepares inputs on the host (the initial conditions of Init.jl, and the FP64 spectra of initial
-data for FP32-field handles, see ``Operators.rfft_accurate``).
+Everything numeric runs in the HIP library; this module marshals arrays and prepares the
+initial conditions of Init.jl on the host.
 """
 from __future__ import annotations
 
@@ -265,6 +264,7 @@ class Operators:
         self.lib = load_library()
         self.conf = conf
         self.precision = precision
+        self.device = device
         self._owned = _borrow is None
         if _borrow is not None:
             self.h = _borrow
@@ -380,16 +380,17 @@ class Operators:
         return out
 
     def rfft_accurate(self, u: np.ndarray) -> np.ndarray:
-        """Spectra of initial data: the device transform for an FP64 or mixed handle; for an
-        FP32 handle the FP64 transform is taken on the host (numpy), because the
-        FP32 FFT roundoff floor at high k is amplified by the S1 k^4 stabiliser.
+        """Spectra of initial data in FP64: this handle's transform for an FP64 or mixed handle;
+        for an FP32 handle the transform of a temporary FP64 handle of the same configuration
+        (the FP32 FFT roundoff floor at high k would be amplified by the S1 k^4 stabiliser).
         Used to prepare inputs only, never inside a time step."""
         if self.precision in (SAV_PREC_FP64, SAV_PREC_MIXED):
             return self.rfft(u)
-        u = np.asarray(u, dtype=np.float64)
-        nd = self.conf.ndim
-        axes = tuple(range(nd - 1, -1, -1))      # Julia (x, y[, z], ...): x is the halved axis
-        return np.asfortranarray(np.fft.rfftn(u, axes=axes))
+        h = Operators(self.conf, precision=SAV_PREC_FP64, device=self.device)
+        try:
+            return h.rfft(u)
+        finally:
+            h.close()
 
     def irfft(self, uh: np.ndarray) -> np.ndarray:
         uh = _f(uh, np.complex128)
