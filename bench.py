@@ -288,8 +288,9 @@ def algorithmic_bytes_built(conf, s, k):
       sides 9u, Krylov set-up 24u, solution read-out 6u; velocity extrapolation 6v), step 2 12u,
       step 4 26u + 47v, step 5 11u + 6v, steps 6-7 14u + 8v, back-transform 8u + 8v, monitor
       6u + 5v: 225u + 82v;
-    * 3D: step 1 103u + 12v, step 2 12u, step 4 38u + 88v, step 5 11u + 9v, steps 6-7 14u + 11v,
-      back-transform 8u + 12v, monitor 6u + 7v: 192u + 139v;
+    * 3D: step 1 98u + 12v, step 2 12u, step 4 32u + 79v, step 5 11u + 9v, steps 6-7 14u + 11v,
+      back-transform 8u + 12v, monitor 6u + 7v: 181u + 130v (the back-transforms of x and y
+      derivatives share the field's z pass);
     and per psi system-iteration FP64 72u (k_mv_cols_inv with the s / p update on load 10u + 12u,
     two k_mv_rows 18u, k_mv_cols_fwd 10u + 8u, k_kry_upd2 14u), FP32 76u (separate updates:
     column inverse 6u + 6u, rows 18u, column forward 10u + 8u, s update 6u, k_kry_upd2 14u,
@@ -297,7 +298,7 @@ def algorithmic_bytes_built(conf, s, k):
     pts = conf.Nx * conf.Ny * (conf.Nz if conf.ndim == 3 else 1)
     u, v = s * pts * conf.N, s * pts
     if conf.ndim == 3:
-        return 192 * u + 139 * v
+        return 181 * u + 130 * v
     return (225 + (72 if s == 8 else 76) * k) * u + 82 * v
 
 
