@@ -354,9 +354,11 @@ def run_planes(jv, fields=8, repeats=5):
 
 def _run_full(jv, wl, precision):
     """The workload's whole step schedule (uniform dt, BDF1 then BDF2) after the SAV
-    initialisation; the steps are timed together."""
+    initialisation; the steps are timed together, with step replay chosen by measurement as
+    run_simulation does (graph_choice: 1 replay kept, 0 eager, -1 still measuring)."""
     conf = wl.conf
     ops = _setup(jv, wl, precision)
+    ops.set_graph(jv.GRAPH_AUTO)
     t = np.arange(wl.steps + 1) * conf.dt
     _sync()
     t0 = time.perf_counter()
@@ -364,8 +366,10 @@ def _run_full(jv, wl, precision):
     _sync()
     el = time.perf_counter() - t0
     st = ops.get_state(jv.SAV_PRESENT)
+    choice = ops.graph_info()["choice"]
     ops.close()
-    return dict(rc=int(rc), steps_per_s=wl.steps / el, seconds=el, krylov_iters=int(kit), E=np.asarray(E)), st
+    return dict(rc=int(rc), steps_per_s=wl.steps / el, seconds=el, krylov_iters=int(kit), E=np.asarray(E),
+                graph_choice=int(choice)), st
 
 
 def _rel(a, b):
@@ -396,7 +400,8 @@ def run_configs(jv, cpu=True):
             sb = 4 if name == "fp32" else 8
             k = r["krylov_iters"] / wl.steps
             B = algorithmic_bytes(conf, sb, k)
-            row[name] = dict(steps_per_s=round(r["steps_per_s"], 2), psi_iters_per_step=round(k, 3),
+            row[name] = dict(steps_per_s=round(r["steps_per_s"], 2), graph_choice=r["graph_choice"],
+                             psi_iters_per_step=round(k, 3),
                              algorithmic_gb_per_step=round(B / 1e9, 4),
                              algorithmic_tb_per_s=round(B * r["steps_per_s"] / 1e12, 3),
                              energy_monotone=bool(np.all(np.diff(r["E"]) <= 0)))

@@ -232,9 +232,14 @@ int sav_plane_kernel_info(sav_handle_t* h, int32_t* kind, double* ms_register, d
  * system still iterating after that budget makes the step run again without the graph
  * (present and old are untouched by the replay).  Results are bit-identical to on = 0
  * (the default).  Steps run without the graph while a kernel timing hook is on, with the
- * learned or stencil preconditioner, and before the first 2D psi solve.
- * sav_graph_info: steps replayed, steps repeated without the graph, graphs held. */
+ * learned or stencil preconditioner, and before the first 2D psi solve.  on = 2 decides by
+ * measurement: from the third step on, once steps can be replayed, 5 eager and 5 replayed
+ * steps (not counting those that capture a graph) are timed on the host, and replay is kept
+ * if its median step is more than 3 % shorter (it is for launch-bound small grids).
+ * sav_graph_info: steps replayed, steps repeated without the graph, graphs held.
+ * sav_graph_choice: -1 while on = 2 is still measuring, else 0 (eager) or 1 (replay). */
 int sav_set_graph(sav_handle_t* h, int on);
+int sav_graph_choice(sav_handle_t* h, int32_t* choice);
 int sav_graph_info(sav_handle_t* h, long long* replays, long long* repeats, long long* graphs);
 
 /* Milliseconds of device time of the last sav_run / sav_step (HIP events on the handle's stream). */

@@ -11,6 +11,6 @@ from .sav import (FieldState, Operators, Ensemble, config_c, build_operators, to
                   resume_simulation, SNAPSHOT_KEYS, initial_fields_3d, initial_fields_2d,
                   SavError, WCNegativeError, EXPORTS, LIB_PATH,
                   SAV_OK, SAV_W_KRYLOV_NOT_CONVERGED, SAV_W_THETA2_NONNEG, SAV_E_ARG, SAV_E_HIP,
-                  SAV_E_WC_NEGATIVE, SAV_E_NONFINITE, SAV_E_UNSUPPORTED, SAV_PREC_FP64, SAV_PREC_FP32, SAV_PREC_MIXED, SAV_PLANE_AUTO, SAV_PLANE_REGISTER, SAV_PLANE_LDS,
+                  SAV_E_WC_NEGATIVE, SAV_E_NONFINITE, SAV_E_UNSUPPORTED, SAV_PREC_FP64, SAV_PREC_FP32, SAV_PREC_MIXED, SAV_PLANE_AUTO, SAV_PLANE_REGISTER, SAV_PLANE_LDS, GRAPH_AUTO,
                   SAV_PRECOND_DIAG, SAV_PRECOND_LEARNED, SAV_PRECOND_STENCIL, SAV_PRESENT, SAV_OLD, SAV_NET_NPARAM)
 from . import plots  # noqa: E402  (visualize.jl: matplotlib, host side)

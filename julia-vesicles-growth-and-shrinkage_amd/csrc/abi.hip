@@ -301,6 +301,11 @@ int sav_set_graph(sav_handle_t* h, int on) {
   SAV_GUARD(h, return h->s->set_graph(on));
 }
 
+int sav_graph_choice(sav_handle_t* h, int32_t* choice) {
+  if (!h || !choice) return SAV_E_ARG;
+  SAV_GUARD(h, *choice = h->s->graph_choice(); return SAV_OK);
+}
+
 int sav_graph_info(sav_handle_t* h, long long* replays, long long* repeats, long long* graphs) {
   if (!h) return SAV_E_ARG;
   SAV_GUARD(h, h->s->graph_info(replays, repeats, graphs); return SAV_OK);

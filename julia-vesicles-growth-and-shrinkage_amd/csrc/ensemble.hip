@@ -51,6 +51,7 @@ int sav_ensemble_create(const sav_config_t* cfgs, int n, sav_ensemble_t** out) {
       return rc;
     }
     sav::set_step_fork(h, false);  // members overlap with each other: one stream each
+    sav_set_graph(h, 0);           // eager unless set per member: the members' timings interleave
     e->members.push_back(h);
     e->cfgs.push_back(cfgs[i]);
     const long long pts = (long long)cfgs[i].Nx * cfgs[i].Ny * (cfgs[i].ndim == 3 ? cfgs[i].Nz : 1);

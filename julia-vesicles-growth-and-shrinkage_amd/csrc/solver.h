@@ -99,6 +99,8 @@ class ISolver {
   // ensemble members, which overlap with each other, keep one stream each.
   virtual void set_step_fork(bool on) = 0;
   virtual void graph_info(long long* replays, long long* repeats, long long* graphs) = 0;
+  // -1 while graph mode 2 is still measuring, else 0 (eager) or 1 (replay)
+  virtual int graph_choice() const = 0;
   // targets of the following precond_train calls: 0 the solver's error corrections, 1 the
   // analytic first-order stencil's output on the same windows
   virtual int precond_train_target(int kind) = 0;

@@ -10,6 +10,7 @@
 // swap and the back-transform of a new state is a single batched irfft.
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <mutex>
@@ -2511,13 +2512,45 @@ class Solver final : public ISolver {
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs;
-  bool graph_on = false;
   long long graph_replays = 0, graph_repeats = 0;
+  // Replay mode: 0 off, 1 on, 2 measured.  In mode 2, once steps can be replayed (the
+  // handle's tuning done) and after its first two steps, the wall times of GAUTO_N eager
+  // steps and of GAUTO_N replayed steps are taken (steps that capture a graph are not
+  // counted), and replay is kept when its median step is more than 3 % shorter: launch-bound
+  // small grids gain, large grids and forked 3D steps lose a little.
+  static constexpr int GAUTO_N = 5;
+  int graph_mode = 0;
+  int gauto_phase = 0;  // 0 timing eager steps, 1 timing replayed steps, 2 decided
+  int gauto_tries = 0;
+  long long steps_run = 0;
+  bool gauto_replay = false;
+  std::vector<double> gauto_eager, gauto_graph;
 
-  bool graph_usable() const {
+  bool graph_base_usable() const {
     // a mixed-precision step decides its rounds on the host: eager only
-    return graph_on && kc.timed == 0 && group == nullptr &&
+    return kc.timed == 0 && group == nullptr &&
            (nd == 3 || (!mixed && !use_learned && !collecting && kc.colw != 0 && kc.order != 0));
+  }
+  bool graph_usable() const {
+    const bool on = graph_mode == 1 || (graph_mode == 2 && (gauto_phase == 1 || (gauto_phase == 2 && gauto_replay)));
+    return on && graph_base_usable();
+  }
+  static double median(std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    return v[v.size() / 2];
+  }
+  void gauto_record(double ms, bool captured) {
+    if (gauto_phase == 0) {
+      gauto_eager.push_back(ms);
+      if ((int)gauto_eager.size() == GAUTO_N) gauto_phase = 1;
+    } else if (gauto_phase == 1) {
+      if (!captured) gauto_graph.push_back(ms);
+      if ((int)gauto_graph.size() == GAUTO_N || ++gauto_tries >= 3 * GAUTO_N) {
+        gauto_replay = !gauto_graph.empty() && median(gauto_graph) < 0.97 * median(gauto_eager);
+        gauto_phase = 2;
+        if (!gauto_replay) drop_graphs();
+      }
+    }
   }
   void drop_graphs() {
     if (!graphs.empty()) SAV_HIP(hipStreamSynchronize(s));
@@ -2607,9 +2640,14 @@ class Solver final : public ISolver {
   }
 
   int set_graph(int on) override {
-    if (on != 0 && on != 1) return SAV_E_ARG;
-    graph_on = on == 1;
-    if (!graph_on) drop_graphs();
+    if (on < 0 || on > 2) return SAV_E_ARG;
+    graph_mode = on;
+    gauto_phase = 0;
+    gauto_tries = 0;
+    gauto_replay = false;
+    gauto_eager.clear();
+    gauto_graph.clear();
+    if (on != 1) drop_graphs();
     return 0;
   }
   void graph_info(long long* replays, long long* repeats, long long* ngraphs) override {
@@ -2617,8 +2655,25 @@ class Solver final : public ISolver {
     if (repeats) *repeats = graph_repeats;
     if (ngraphs) *ngraphs = (long long)graphs.size();
   }
+  int graph_choice() const override {
+    if (graph_mode != 2) return graph_mode;
+    if (gauto_phase == 2) return gauto_replay ? 1 : 0;
+    return graph_base_usable() ? -1 : 0;  // a form that never replays (mixed, learned) stays eager
+  }
 
   int step_impl(double dt, int n, sav_step_stats_t* st, bool monitor_on) {
+    ++steps_run;
+    if (graph_mode != 2 || gauto_phase == 2 || steps_run <= 2 || !graph_base_usable())
+      return step_body(dt, n, st, monitor_on);
+    const size_t ng0 = graphs.size();
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = step_body(dt, n, st, monitor_on);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc >= 0) gauto_record(ms, graphs.size() != ng0);
+    return rc;
+  }
+
+  int step_body(double dt, int n, sav_step_stats_t* st, bool monitor_on) {
     autotune_transforms();
     const double a = n == 1 ? 1.0 : 1.5, b = n == 1 ? -1.0 : -2.0, cc = n == 1 ? 0.0 : 0.5;
     present_dt = dt;

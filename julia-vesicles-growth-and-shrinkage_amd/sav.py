@@ -41,6 +41,7 @@ SAV_PREC_MIXED = 2
 SAV_PLANE_AUTO = 0
 SAV_PLANE_REGISTER = 1
 SAV_PLANE_LDS = 2
+GRAPH_AUTO = 2  # Operators.set_graph: replay chosen by measurement
 SAV_PRECOND_DIAG = 0
 SAV_PRECOND_LEARNED = 1
 SAV_PRECOND_STENCIL = 2
@@ -77,7 +78,7 @@ EXPORTS = ("sav_abi_version", "sav_create", "sav_destroy", "sav_last_error", "sa
            "sav_get_state", "sav_init_sav", "sav_step", "sav_run", "sav_rfft", "sav_irfft", "sav_irfft_noalloc",
            "sav_fft_full", "sav_eval_functionals", "sav_psi_solve", "sav_modified_energy",
            "sav_last_elapsed_ms", "sav_kernel_timing", "sav_kernel_timing_get", "sav_set_precond",
-           "sav_set_column_tile", "sav_autotune_info", "sav_column_tile_info", "sav_set_matvec_order", "sav_matvec_order_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_transform_tile_fwd_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
+           "sav_set_column_tile", "sav_autotune_info", "sav_column_tile_info", "sav_set_matvec_order", "sav_matvec_order_info", "sav_set_transform_tile", "sav_transform_tile_info", "sav_transform_tile_fwd_info", "sav_set_plane_kernel", "sav_plane_kernel_info", "sav_set_graph", "sav_graph_info", "sav_graph_choice", "sav_precond_init", "sav_precond_load", "sav_precond_save", "sav_precond_collect",
            "sav_precond_samples", "sav_precond_train", "sav_precond_train_target", "sav_net_forward", "sav_net_grad",
            "sav_ensemble_create", "sav_ensemble_destroy", "sav_ensemble_last_error", "sav_ensemble_size",
            "sav_ensemble_member", "sav_ensemble_run", "sav_ensemble_set_batched", "sav_ensemble_batched_info")
@@ -138,6 +139,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ct.CDLL:
     lib.sav_set_plane_kernel.argtypes = [ct.c_void_p, ct.c_int]
     lib.sav_plane_kernel_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32), _dp, _dp]
     lib.sav_set_graph.argtypes = [ct.c_void_p, ct.c_int]
+    lib.sav_graph_choice.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32)]
     lib.sav_graph_info.argtypes = [ct.c_void_p, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
                                    ct.POINTER(ct.c_longlong)]
     lib.sav_precond_init.argtypes = [ct.c_void_p, ct.c_uint64]
@@ -486,14 +488,17 @@ class Operators:
                     "plane_kernel_info")
         return dict(kind=k.value, ms_register=a.value, ms_lds=b.value)
 
-    def set_graph(self, on: bool):
-        """Replay each step as one captured HIP graph (bit-identical results)."""
-        self._check(self.lib.sav_set_graph(self.h, int(bool(on))), "set_graph")
+    def set_graph(self, on):
+        """Replay each step as one captured HIP graph (bit-identical results): False / 0 off,
+        True / 1 on, GRAPH_AUTO (2) chosen by timing eager against replayed steps."""
+        self._check(self.lib.sav_set_graph(self.h, int(on)), "set_graph")
 
     def graph_info(self):
         a, b, c = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
         self._check(self.lib.sav_graph_info(self.h, ct.byref(a), ct.byref(b), ct.byref(c)), "graph_info")
-        return dict(replays=a.value, repeats=b.value, graphs=c.value)
+        ch = ct.c_int32(0)
+        self._check(self.lib.sav_graph_choice(self.h, ct.byref(ch)), "graph_choice")
+        return dict(replays=a.value, repeats=b.value, graphs=c.value, choice=ch.value)
 
     # -- learned psi preconditioner (no reference counterpart) -----------------
     def set_precond(self, precond: int):
@@ -871,7 +876,8 @@ def read_checkpoint(path: os.PathLike):
 def run_simulation(dt_val: float, T_val: float, state_type: int, save_path: Optional[str] = None,
                    save_interval: float = 0.001, conf: Optional[Config] = None,
                    state: Optional[FieldState] = None, t_nodes: Optional[np.ndarray] = None,
-                   precision: int = SAV_PREC_FP64, checkpoint: bool = False, plots: bool = True):
+                   precision: int = SAV_PREC_FP64, checkpoint: bool = False, plots: bool = True,
+                   graph: int = GRAPH_AUTO):
     """Returns (present, energy_history, area_ratio_history, Dt) like simulation.jl:154.
 
     With ``conf``/``state``/``t_nodes`` left out it runs the reference's own set-up
@@ -879,9 +885,11 @@ def run_simulation(dt_val: float, T_val: float, state_type: int, save_path: Opti
     ``save_path`` it writes the reference's snapshots (``save_path/MAT/case3/``, the
     schedule of simulation.jl:113-114), with ``plots`` the 2D phi image of each snapshot
     (``save_path/phi_t<t>.png``, simulation.jl:130-135) and, with ``checkpoint``, a resumable
-    checkpoint next to each snapshot."""
+    checkpoint next to each snapshot.  ``graph``: step replay mode (Operators.set_graph; by
+    default chosen by measurement)."""
     conf = conf or set_para_base(dt_val, T_val)
     ops = build_operators(conf, precision=precision)
+    ops.set_graph(graph)
     Dt = time_nodes(conf.dt, T_val) if t_nodes is None else np.asarray(t_nodes, dtype=np.float64)
     present = state if state is not None else generate_initial_condition(conf, ops, state_type)
     ops.set_state(SAV_PRESENT, present)

@@ -122,3 +122,29 @@ def test_graph_ensemble_equals_eager_ensemble(jv):
     for a, b in zip(s0, s1):
         _same_state(a, b)
     assert r0 == 0 and r1 >= len(confs) * 3
+
+
+@pytest.mark.parametrize("case", ["2d", "3d"])
+def test_measured_replay_mode(jv, case):
+    """set_graph(GRAPH_AUTO): the handle times eager against replayed steps and keeps one;
+    the run is bit-identical to an eager run whichever it keeps, and the choice is made
+    within the run (graph_info choice 0 or 1)."""
+    wl = jv.workload_c1(n=128, steps=1) if case == "2d" else jv.workload_c4(n=32, steps=1)
+    conf = wl.conf
+    phi, psi = wl.initial_fields()
+    t = np.arange(31) * conf.dt
+    runs, infos = {}, {}
+    for mode in (0, jv.GRAPH_AUTO):
+        ops = _fresh(jv, conf, phi, psi, 0, mode)
+        rc, E, A, kit = ops.run(t, 1)
+        assert rc >= 0
+        runs[mode] = (ops.get_state(jv.SAV_PRESENT), E, kit)
+        infos[mode] = ops.graph_info()
+        ops.close()
+    print(case, infos, flush=True)
+    assert infos[0]["choice"] == 0 and infos[0]["replays"] == 0
+    assert infos[jv.GRAPH_AUTO]["choice"] in (0, 1)
+    assert infos[jv.GRAPH_AUTO]["replays"] >= 5
+    _same_state(runs[0][0], runs[jv.GRAPH_AUTO][0])
+    assert np.array_equal(runs[0][1], runs[jv.GRAPH_AUTO][1])
+    assert runs[0][2] == runs[jv.GRAPH_AUTO][2]
