@@ -2091,7 +2091,6 @@ class Solver final : public ISolver {
   // Steps 6 and 7 (Solvers.jl:431-498) written into the next state buffer
   // ==========================================================================
   void step67() {
-    const long long NH = (long long)N * H;
     const Splits sp = splits();
     C *phN = fhalf(inew, F_PHI), *psN = fhalf(inew, F_PSI), *muN = fhalf(inew, F_MU), *nuN = fhalf(inew, F_NU);
     C* uN = uhalf(inew);
