@@ -835,18 +835,30 @@ def snapshot_name(conf: Config, t: float) -> str:
     return f"state_{conf.ndim}d_t{t:.2e}.mat"
 
 
-def write_snapshot(path: os.PathLike, st: FieldState):
-    """The keys matwrite stores (simulation.jl:118-129), arrays in the Julia shapes.  MAT
-    version 5 (scipy.io); MAT.jl's matread and MATLAB read it like the v7.3 files of the
-    reference."""
-    import scipy.io
-    scipy.io.savemat(str(path), {k: getattr(st, k) for k in SNAPSHOT_KEYS}, do_compression=True,
-                     oned_as="column")
+def write_snapshot(path: os.PathLike, st: FieldState, version: str = "7.3"):
+    """The keys matwrite stores (simulation.jl:118-129), arrays in the Julia shapes, scalars
+    as 1 x 1.  ``version`` "7.3" (the default of MAT.jl's matwrite, which the reference
+    calls): the HDF5-based format, written by `matfile.write_mat73`; "5": MAT version 5
+    through scipy.io (compressed), which MAT.jl's matread reads as well."""
+    data = {k: getattr(st, k) for k in SNAPSHOT_KEYS}
+    if version == "7.3":
+        from .matfile import write_mat73
+        write_mat73(path, data)
+    elif version == "5":
+        import scipy.io
+        scipy.io.savemat(str(path), data, do_compression=True, oned_as="column")
+    else:
+        raise ValueError(f"write_snapshot: MAT version {version!r} is not written (7.3 or 5)")
 
 
 def read_snapshot(path: os.PathLike) -> dict:
-    import scipy.io
-    d = scipy.io.loadmat(str(path))
+    """The snapshot keys of a MAT file written by write_snapshot (either version)."""
+    from .matfile import is_mat73, read_mat73
+    if is_mat73(path):
+        d = read_mat73(path)
+    else:
+        import scipy.io
+        d = scipy.io.loadmat(str(path))
     return {k: (float(d[k].ravel()[0]) if k in ("R1", "R2", "R3", "Q") else d[k]) for k in SNAPSHOT_KEYS}
 
 
