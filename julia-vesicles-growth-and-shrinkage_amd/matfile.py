@@ -17,9 +17,10 @@ MATLAB arrays are column-major and HDF5 dataspaces row-major: an array of size
 (n1, ..., nk) is stored as its column-major bytes under the dataspace (nk, ..., n1), as
 MAT.jl and MATLAB store it; a scalar is a 1 x 1 array, a vector of length n an n x 1 one.
 
-`read_mat73` reads that same subset back (contiguous float64 datasets in a symbol-table
-root group); chunked or compressed datasets, which MATLAB itself may write for large
-arrays, are refused with an error.
+`read_mat73` reads that subset back, whoever wrote it (contiguous float64 datasets in a
+symbol-table root group, version 1 object headers with continuation blocks), e.g. the
+HDF5 library's own files with the 512-byte user block; chunked or compressed datasets,
+which MATLAB itself may write for large arrays, are refused with an error.
 """
 from __future__ import annotations
 
@@ -38,7 +39,7 @@ _INTERNAL_K = 16                # group B-tree nodes hold at most 2 * _INTERNAL_
 _HEAP_FREE_NONE = 1             # local heap free-list offset meaning "no free block"
 
 # object header message types
-_M_DATASPACE, _M_DATATYPE, _M_FILL, _M_LAYOUT, _M_ATTR, _M_STAB = 0x1, 0x3, 0x5, 0x8, 0xC, 0x11
+_M_DATASPACE, _M_DATATYPE, _M_FILL, _M_LAYOUT, _M_ATTR, _M_CONT, _M_STAB = 0x1, 0x3, 0x5, 0x8, 0xC, 0x10, 0x11
 
 # IEEE float64, little-endian: class 1 version 1; bit field: byte order 0, mantissa
 # normalisation 2 (implied leading 1), sign bit 63; properties: offset 0, precision 64,
@@ -217,17 +218,26 @@ class _Reader:
         return self.base + a
 
     def messages(self, oh: int):
+        """The messages of a version 1 object header, following continuation blocks
+        (message 0x10: address and length of a further block of messages)."""
         p = self.at(oh)
         ver, _, nm, _, size = struct.unpack_from("<BBHII", self.b, p)
         if ver != 1:
             raise ValueError(f"object header version {ver} is not read here")
-        q, end = p + 16, p + 16 + size
-        for _ in range(nm):
-            if q >= end:
-                break
-            t, sz, fl = struct.unpack_from("<HHB", self.b, q)
-            yield t, self.b[q + 8:q + 8 + sz]
-            q += 8 + sz
+        chunks = [(p + 16, p + 16 + size)]
+        seen = 0
+        while chunks and seen < nm:
+            q, end = chunks.pop(0)
+            while q + 8 <= end and seen < nm:
+                t, sz = struct.unpack_from("<HH", self.b, q)
+                data = self.b[q + 8:q + 8 + sz]
+                seen += 1
+                if t == _M_CONT:
+                    ca, cl = struct.unpack_from("<QQ", data)
+                    chunks.append((self.at(ca), self.at(ca) + cl))
+                else:
+                    yield t, data
+                q += 8 + sz
 
     def group(self, oh: int) -> Dict[str, int]:
         stab = [d for t, d in self.messages(oh) if t == _M_STAB]

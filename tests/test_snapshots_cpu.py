@@ -103,6 +103,44 @@ def _h5py_available():
     return r.returncode == 0
 
 
+_H5_WRITE = r"""
+import sys
+import h5py, numpy as np
+ref = np.load(sys.argv[2])
+with h5py.File(sys.argv[1], "w", userblock_size=512, libver="earliest") as f:
+    for k in ref.files:
+        a = ref[k]
+        a = a if a.ndim >= 2 else a.reshape(-1, 1)
+        d = f.create_dataset(k, data=np.ascontiguousarray(a.T))
+        d.attrs["MATLAB_class"] = np.bytes_("double")
+"""
+
+
+@pytest.mark.skipif(not _h5py_available(), reason="no HDF5 library (h5py) in this image to write the file")
+def test_mat73_reader_reads_hdf5_library_files(jv, tmp_path):
+    """A 7.3 file written by the HDF5 library (h5py, 512-byte user block, MATLAB_class
+    attributes added after each dataset, as MAT.jl lays out its files), with the MATLAB
+    header put into the user block: read_snapshot returns every variable in its shape."""
+    from importlib import import_module
+    matfile = import_module(jv.__name__ + ".matfile")
+    conf = jv.workload_c1(n=12, steps=1).conf.with_(Ny=10)
+    st = _dummy_state(jv, conf, seed=7)
+    ref = tmp_path / "ref.npz"
+    np.savez(ref, **{k: np.asarray(getattr(st, k)) for k in jv.SNAPSHOT_KEYS})
+    path = tmp_path / "h.mat"
+    env = {k: v for k, v in os.environ.items() if k != "PYTHONPATH"}
+    env["PYTHONHOME"] = "/opt/conda"
+    r = subprocess.run([str(_H5PY), "-c", _H5_WRITE, str(path), str(ref)], env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(path, "r+b") as f:
+        f.write(matfile.matlab_header())
+    d = jv.read_snapshot(path)
+    assert d["R1"] == 1.1 and d["Q"] == 4.4
+    for k in ("phi", "psi", "mu", "nu", "p", "u"):
+        assert np.array_equal(d[k], getattr(st, k)), k
+
+
 @pytest.mark.skipif(not _h5py_available(), reason="no independent HDF5 reader (h5py) in this image")
 @pytest.mark.parametrize("case", ["2d", "3d"])
 def test_mat73_read_by_hdf5_library(jv, tmp_path, case):
