@@ -52,10 +52,13 @@ What pins each function (tests/test_oracle_kat.py, tests/test_golden.py):
   reproduced bit for bit (the Julia package is absent).
 * ``AltOps``: the same transforms by a second route, for the round-off floor between
   two FFT implementations (tests/test_oracle_fft_floor.py).
-* ``solve_step1`` .. ``solve_step7``, ``step``: invariants (spectral div u = 0,
-  modified energy non-increasing, mean psi conserved, BDF1 first step); end-to-end
-  trajectories of the Julia code are parity unpinned (no Julia here, no golden data in
-  the reference).
+* ``solve_step1`` .. ``solve_step7``, ``step``: the coupled BDF-SAV scheme that the
+  seven steps decompose, equation by equation at the new state (phi, mu, psi, nu, R1-R3,
+  the intermediate velocity, Q, the projection), to round-off in 2D and 3D; planting a
+  swapped R index, a dropped term or a wrong sign in any step leaves an O(1e-6..1)
+  residual there; and invariants (spectral div u = 0, modified energy non-increasing,
+  mean psi conserved, BDF1 first step).  A trajectory of the Julia code itself is not
+  available to compare against (no Julia here, no golden data in the reference).
 """
 from __future__ import annotations
 
