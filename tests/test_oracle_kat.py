@@ -498,113 +498,18 @@ def test_bicgstab_identities_every_iteration(jv, so):
 
 
 # --- the split against the coupled scheme it decomposes -----------------------
-def _scheme_residuals(so, orc, pres, old, n):
-    """One oracle step assembled from its seven parts (as Oracle.step does), then the
-    residual of each equation of the coupled BDF-SAV scheme (Solvers.jl:14-498) at the new
-    state, relative to the size of that equation's terms:
-
-      phi:  (a phi' + b phi^n + c phi^{n-1})/dt = -M_phi mu' - Q' (u*.grad phi*)
-      mu:   mu' = L_phi phi' + R1' H1 + R2' H2 + R3' H3
-      psi:  (a psi' + b psi^n + c psi^{n-1})/dt = div(M_psi grad nu') - Q' (u*.grad psi*)
-            (the S4 psi' part of the flux spectral, the R3' G part through physical space)
-      nu:   nu' = S4 psi' + R3' G
-      R_i:  a R_i' + b R_i^n + c R_i^{n-1} = 1/2 int H_i (a phi' + b phi^n + c phi^{n-1})
-            (+ 1/2 int G (a psi' + ...) for R3)
-      u~:   (a u~ + b u^n + c u^{n-1})/dt - eta Lap u~ = -grad p^n - Q' (conv* + lambda coupl*)
-      Q:    (a Q' + b Q^n + c Q^{n-1})/dt = dV [lambda (<adv_phi, mu'> + <adv_psi, nu'> + <coupl*, u~>)
-                                               + <conv*, u~>]
-      u, p: u' = u~ - (dt/a) grad dp, Lap dp = (a/dt) div u~, p' = p^n + dp - eta div u~
-
-    The split solutions, the 3 x 3 R system, the scalar Q equation and the combinations
-    (steps 1-7) are each a different arrangement of these equations, so a wrong sign,
-    coefficient or index in any of them leaves an O(1) residual here."""
-    o, c = orc.ops, orc.conf
-    dt = c.dt
-    pres.dt = dt
-    a, b, cc = so.bdf_coeff(n)
-    cache = orc.solve_step1(pres, old, (a, b, cc))
-    H1, H2, H3, G = orc._fields
-    s2 = orc.solve_step2(pres, old, cache, (a, b, cc), fields=orc._fields)
-    s3 = orc.solve_step3(cache, s2)
-    s4 = orc.solve_step4(pres, old, (a, b, cc))
-    Q = orc.solve_step5(pres, old, s3, s4, (a, b, cc))
-    s6 = orc.solve_step6(s2, s3, s4, Q)
-    s7 = orc.solve_step7(pres, s6, (a, b, cc))
-    nd = o.ndim
-    rf, irf = o.rfft, o.irfft
-    phi_s_hat = 2 * pres.phi_hat - old.phi_hat
-    psi_s_hat = 2 * pres.psi_hat - old.psi_hat
-    u_s = 2 * pres.u - old.u
-    adv_phi = sum(u_s[d] * irf(o.D1[d] * phi_s_hat) for d in range(nd))
-    adv_psi = sum(u_s[d] * irf(o.D1[d] * psi_s_hat) for d in range(nd))
-    L_phi = orc._L_phi()
-    ph, ps, mu, nu = s6["phi_hat"], s6["psi_hat"], s6["mu_hat"], s6["nu_hat"]
-    R = (s6["R1"], s6["R2"], s6["R3"])
-    Hh = [rf(H1), rf(H2), rf(H3)]
-    Gh = rf(G)
-
-    def rel(lhs, rhs):
-        return float(np.max(np.abs(lhs - rhs)) / max(np.max(np.abs(lhs)), np.max(np.abs(rhs)), 1e-300))
-
-    out = {}
-    out["phi"] = rel((a * ph + b * pres.phi_hat + cc * old.phi_hat) / dt, -c.M_phi * mu - Q * rf(adv_phi))
-    out["mu"] = rel(mu, L_phi * ph + R[0] * Hh[0] + R[1] * Hh[1] + R[2] * Hh[2])
-    # psi flux as the reference discretises it: the S4 psi part inside the psi solve's
-    # operator (spectral; constant mobility here), the G part through physical space
-    Mpsi = so.get_M_psi(2 * pres.phi - old.phi, c) if nd == 2 else c.M0_psi
-    Mc = float(np.max(Mpsi))
-    flux_g = sum(o.D1[d] * rf(Mpsi * irf(o.D1[d] * Gh)) for d in range(nd))
-    div_flux = Mc * c.S4 * o.Laplacian * ps + R[2] * flux_g
-    out["psi"] = rel((a * ps + b * pres.psi_hat + cc * old.psi_hat) / dt, div_flux - Q * rf(adv_psi))
-    out["nu"] = rel(nu, c.S4 * ps + R[2] * Gh)
-    dv = o.dvol(c)
-
-    def integ(f, g):
-        return float(np.sum(f * g)) * dv
-    phi_t = irf(a * ph + b * pres.phi_hat + cc * old.phi_hat)
-    psi_t = irf(a * ps + b * pres.psi_hat + cc * old.psi_hat)
-    Rn, Ro = (pres.R1, pres.R2, pres.R3), (old.R1, old.R2, old.R3)
-    for i, Hi in enumerate((H1, H2, H3)):
-        lhs = a * R[i] + b * Rn[i] + cc * Ro[i]
-        rhs = 0.5 * integ(Hi, phi_t) + (0.5 * integ(G, psi_t) if i == 2 else 0.0)
-        out[f"R{i + 1}"] = abs(lhs - rhs) / max(abs(lhs), abs(rhs), abs(b * Rn[i]), 1e-300)
-    ut = s6["u_tilde_hat"]
-    conv, coupl = s4["conv"], s4["coupl"]
-    lhs = np.stack([(a * ut[d] + b * pres.u_hat[d] + cc * old.u_hat[d]) / dt - c.eta * o.Laplacian * ut[d]
-                    for d in range(nd)])
-    rhs = np.stack([-o.D1[d] * pres.p_hat - Q * rf(conv[d] + c.lamda * coupl[d]) for d in range(nd)])
-    out["u_tilde"] = rel(lhs, rhs)
-    ut_phys = [irf(ut[d]) for d in range(nd)]
-    theta = dv * (c.lamda * (float(np.sum(adv_phi * irf(mu))) + float(np.sum(adv_psi * irf(nu)))
-                             + sum(float(np.sum(coupl[d] * ut_phys[d])) for d in range(nd)))
-                  + sum(float(np.sum(conv[d] * ut_phys[d])) for d in range(nd)))
-    lhs_q = (a * Q + b * pres.Q + cc * old.Q) / dt
-    out["Q"] = abs(lhs_q - theta) / max(abs(lhs_q), abs(theta), abs(b * pres.Q / dt), 1e-300)
-    div_t = sum(o.D1[d] * ut[d] for d in range(nd))
-    dp = s7["delta_p_hat"]
-    lap_dp = o.Laplacian * dp
-    out["dp"] = rel(lap_dp.reshape(-1)[1:], ((a / dt) * div_t).reshape(-1)[1:])
-    out["u"] = rel(s7["u_hat"], np.stack([ut[d] - (dt / a) * o.D1[d] * dp for d in range(nd)]))
-    out["p"] = rel(s7["p_hat"], pres.p_hat + dp - c.eta * div_t)
-    # advance as Oracle.step does, for the next call
-    so.update_state(old, pres)
-    pres.phi_hat[...], pres.psi_hat[...], pres.mu_hat[...], pres.nu_hat[...] = ph, ps, mu, nu
-    pres.u_hat[...], pres.p_hat[...] = s7["u_hat"], s7["p_hat"]
-    for k in ("phi", "psi", "mu", "nu", "p"):
-        getattr(pres, k)[...] = irf(getattr(pres, k + "_hat"))
-    pres.u[...] = irf(pres.u_hat)
-    pres.Q, pres.R1, pres.R2, pres.R3 = Q, R[0], R[1], R[2]
-    return out
-
-
 @pytest.mark.parametrize("case", ["2d", "3d"])
 def test_step_solves_the_coupled_scheme(jv, so, case):
-    """Three steps (BDF1, then BDF2 with a non-zero velocity): the state the seven-step
-    split produces satisfies every equation of the coupled scheme to round-off.  In 2D the
-    psi mobility is made constant (M0_psi = 0), where the full-complex psi solve keeps
-    Hermitian symmetry and its half spectrum is the exact solution (to the solve's rtol);
-    with a variable mobility the solution carries an anti-Hermitian part that the half
-    spectrum drops, as the reference's does."""
+    """Three oracle steps (BDF1, then BDF2 with a non-zero velocity): the state that the
+    seven-step split produces satisfies every equation of the coupled scheme it decomposes
+    (helpers.scheme_residuals) to round-off.  In 2D the psi mobility is made constant
+    (M0_psi = 0), where the full-complex psi solve keeps Hermitian symmetry and its half
+    spectrum is the whole solution (to the solve's rtol); with a variable mobility the
+    solution carries an anti-Hermitian part that the half spectrum drops, as the
+    reference's does.  Planting a swapped R index in step 3, a dropped G term in the R
+    system, a wrong sign of phi_12, or a dropped coupling force in u~_2 or in theta_1 leaves
+    residuals of 1e-6 to 1 here."""
+    from helpers import scheme_residuals
     if case == "2d":
         wl = jv.workload_c1(n=32, steps=3)
         conf = wl.conf.with_(M0_psi=0.0)
@@ -615,7 +520,10 @@ def test_step_solves_the_coupled_scheme(jv, so, case):
     orc = so.Oracle(conf)
     pres, old = orc.init_sav(so.state_from_physical(phi, psi, orc.ops, conf.ndim))
     for n in (1, 2, 3):
-        res = _scheme_residuals(so, orc, pres, old, n)
+        p0, o0 = pres.copy(), old.copy()
+        p0.dt = conf.dt
+        orc.step(pres, old, conf.dt, n)
+        res = scheme_residuals(so, conf, orc.ops, p0, o0, pres, n)
         # round-off of the transforms against terms up to S1 |k|^4 phi; the 2D psi solve
         # stops at rtol 1e-10
         tol = {"psi": 1e-8 if case == "2d" else 1e-10}
