@@ -48,3 +48,37 @@ def test_device_step_solves_the_coupled_scheme(jv, so, case, graph):
         assert np.max(np.abs(pres.u)) > 0 and pres.Q != 1.0
     finally:
         ops.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_headline_workload_solves_the_coupled_scheme(jv, so, precision):
+    """The two-vesicle 1024^2 workload, variable psi mobility, two steps (BDF1, BDF2), FP64
+    and the mixed form: every equation but psi's holds to round-off for the psi the step
+    computed (psi' enters nu, R3, mu through H3 and G linearly).  psi's own equation is not
+    checked here: with a variable mobility the full-complex solve carries an anti-Hermitian
+    part that the half spectrum drops (as the reference's does), and the mixed form solves
+    it only to the stopping test's tolerance."""
+    wl = jv.workload_c3(n=1024, steps=2)
+    conf = wl.conf
+    phi, psi = wl.initial_fields()
+    prec = jv.SAV_PREC_FP64 if precision == "fp64" else jv.SAV_PREC_MIXED
+    ops = jv.Operators(conf, precision=prec)
+    try:
+        ops.set_state(jv.SAV_PRESENT, device_state(jv, ops, conf, phi, psi))
+        ops.init_sav()
+        o = so.Ops(conf)
+        pres = oracle_view(so, ops.get_state(jv.SAV_PRESENT))
+        old = oracle_view(so, ops.get_state(jv.SAV_OLD))
+        for n in (1, 2):
+            rc, _ = ops.step(conf.dt, n)
+            assert rc >= 0, n
+            new = oracle_view(so, ops.get_state(jv.SAV_PRESENT))
+            pres.dt = conf.dt
+            res = scheme_residuals(so, conf, o, pres, old, new, n)
+            res.pop("psi")
+            # round-off against S1 |k|^4 phi up to k = 512 (2π/L): 1.0e-10 in phi at step 2
+            for k, v in res.items():
+                assert v < 1e-9, (precision, n, k, v)
+            old, pres = pres, new
+    finally:
+        ops.close()
