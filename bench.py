@@ -466,8 +466,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
-    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble", "fp64-2d", "mixed-2d", "fp64-3d", "planes"],
-                    default="all", help="fp64-2d / mixed-2d / fp64-3d: one run of that form, for profiler runs")
+    ap.add_argument("--only", choices=["all", "2d", "3d", "ensemble", "fp64-2d", "mixed-2d", "fp64-3d", "planes",
+                                       "roofline-2d"],
+                    default="all", help="fp64-2d / mixed-2d / fp64-3d: one run of that form, for profiler runs; "
+                                        "roofline-2d: the headline's live-timed run of the roofline kernel alone")
     ap.add_argument("--ensemble-steps", type=int, default=10)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--column-tile", type=int, default=0, choices=[0, 4, 8, 16],
@@ -515,6 +517,15 @@ def main():
         return
     wl3 = jv.workload_c3(n=1024, steps=a.steps)
     wl4 = jv.workload_c4(n=128, steps=a.steps)
+    if a.only == "roofline-2d":
+        # the same run as the headline's roofline leg: k_mv_cols_inv timed live with HIP events
+        r, _ = run_case(jv, wl3, jv.SAV_PREC_FP64, a.steps, a.warmup, timed_kernel=1)
+        k = r["kernel"]
+        print(json.dumps({"only": a.only, "kernel": k["name"], "avg_launch_ms": k["avg_ms"],
+                          "launches": k["launches"], "bytes_per_launch": k["bytes"],
+                          "achieved_gb_s": k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9,
+                          "steps_per_s": r["steps_per_s"]}), flush=True)
+        return
     if a.only in ("fp64-2d", "mixed-2d", "fp64-3d"):
         wl, prec = {"fp64-2d": (wl3, jv.SAV_PREC_FP64), "mixed-2d": (wl3, jv.SAV_PREC_MIXED),
                     "fp64-3d": (wl4, jv.SAV_PREC_FP64)}[a.only]
